@@ -1,0 +1,100 @@
+// Host-side context, device buffers and error handling shared by the
+// denoiser and the TV-L1 host code.
+#pragma once
+
+#include "common.hpp"
+#include "../../include/bm4dof.h"
+
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace bm4dof {
+
+struct HipError : std::runtime_error {
+	explicit HipError(const std::string &s) : std::runtime_error(s) {}
+};
+struct ArgError : std::runtime_error {
+	int code;
+	ArgError(const std::string &s, int c = BM4DOF_EINVAL) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHK(x)                                                                         \
+	do {                                                                                  \
+		hipError_t e_ = (x);                                                              \
+		if (e_ != hipSuccess)                                                             \
+			throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));               \
+	} while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+	void *p = nullptr;
+	size_t n = 0;
+	void *get(size_t bytes) {
+		if (bytes <= n && p) return p;
+		if (p) HIPCHK(hipFree(p));
+		p = nullptr;
+		n = 0;
+		if (bytes == 0) bytes = 256;
+		if (hipMalloc(&p, bytes) != hipSuccess) {
+			(void)hipGetLastError();
+			throw ArgError("device allocation of " + std::to_string(bytes) + " bytes failed",
+			               BM4DOF_ENOMEM);
+		}
+		n = bytes;
+		return p;
+	}
+	template <class T> T *as(size_t count) { return (T *)get(count * sizeof(T)); }
+	void release() {
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+};
+
+} // namespace bm4dof
+
+struct bm4dof_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string err;
+	// step buffers
+	bm4dof::DevBuf gpos, gtile, tiles, idx, nsim, base, keys, vals, skeys, svals, sort_tmp, lstart, lend,
+	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
+	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
+	    io_final, bm_image;
+	// timings of the last run
+	std::vector<std::pair<const char *, float>> timings;
+	std::vector<hipEvent_t> ev;
+	size_t est_budget = (size_t)24 << 30;
+	// TV-L1 buffers
+	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
+};
+
+namespace bm4dof {
+
+// Records an error message on ctx (may be NULL) and as the thread's global
+// error (bm4dof_global_error).
+void set_error(bm4dof_ctx *ctx, const std::string &s);
+
+// Runs fn, mapping exceptions to the C-ABI's error codes.
+template <class Fn> int guarded(bm4dof_ctx *ctx, Fn fn)
+{
+	try {
+		fn();
+		return BM4DOF_OK;
+	} catch (const ArgError &e) {
+		set_error(ctx, e.what());
+		return e.code;
+	} catch (const HipError &e) {
+		set_error(ctx, e.what());
+		return BM4DOF_EHIP;
+	} catch (const std::exception &e) {
+		set_error(ctx, e.what());
+		return BM4DOF_EINVAL;
+	}
+}
+
+} // namespace bm4dof
+

@@ -1,0 +1,899 @@
+// Host side of the MI355X BM4D-OF denoiser: parameters, tiling, reference
+// grid, bases, the per-step pipeline and the C-ABI of include/bm4dof.h.
+#include "ctx.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <stdexcept>
+
+namespace bm4dof {
+
+// Kernels (nldct_kernels.hip).
+__global__ void color_kernel(float *, int, int, int, int, float, float, float);
+__global__ void bm_kernel(const float *, const float *, const float *, const uint32_t *,
+                          const uint32_t *, const TileBox *, StepConst, int, int, uint32_t *,
+                          uint32_t *, int);
+__global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
+                               StepConst, int, int, const float *, const float *, const float *,
+                               float *, float *, uint8_t *, float *, uint32_t *);
+__global__ void filter2_kernel(const float *, const float *, const uint32_t *, const uint32_t *,
+                               const uint32_t *, StepConst, int, int, const float *,
+                               const float *, const float *, float *, float *, uint8_t *,
+                               float *, uint32_t *);
+__global__ void emit_entries_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int,
+                                    int, uint32_t *, uint32_t *);
+__global__ void mark_ranges_kernel(const uint32_t *, const uint32_t *, int64_t, int,
+                                   const uint32_t *, uint32_t *, uint32_t *, uint32_t *,
+                                   uint32_t *);
+__global__ void clear_ranges_kernel(const uint32_t *, int64_t, uint32_t *, uint32_t *);
+__global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, const uint32_t *,
+                                   const float *, const float *, int, int, int, float *,
+                                   float *);
+__global__ void gather_kernel(const uint32_t *, const uint32_t *, const uint32_t *,
+                              const float *, const float *, const float *, StepConst, int, int,
+                              int, int, int, int, double *, double *);
+__global__ void normalize_kernel(const double *, const double *, const float *, StepConst,
+                                 float *);
+
+} // namespace bm4dof
+
+using namespace bm4dof;
+
+namespace {
+thread_local std::string g_global_err;
+}
+
+void bm4dof::set_error(bm4dof_ctx *ctx, const std::string &s)
+{
+	if (ctx) ctx->err = s;
+	g_global_err = s;
+}
+
+namespace {
+
+void set_err(bm4dof_ctx *ctx, const std::string &s) { set_error(ctx, s); }
+
+// ---------------------------------------------------------------- params
+
+// initializeNlbParameters, src/core/nldct.cpp:81-148.  betaMean, left
+// uninitialised by the reference, is set to beta.
+void init_params(bm4dof_params *p, unsigned step, float sigma, unsigned frames, int verbose)
+{
+	const bool s1 = (step == 1);
+	std::memset(p, 0, sizeof *p);
+	if (frames == 1) {
+		if (sigma <= 40) {
+			p->sizePatch = 8;
+			p->nSimilarPatches = s1 ? 16 : 32;
+			p->offSet = 3;
+			p->beta = s1 ? 2.7f : 1.f;
+			p->tau = s1 ? 54.8f : 20.f;
+		} else {
+			p->sizePatch = s1 ? 8 : 11;
+			p->nSimilarPatches = 32;
+			p->offSet = s1 ? 4 : 6;
+			p->beta = s1 ? 2.8f : 1.f;
+			p->tau = s1 ? 158.f : 59.2f;
+		}
+		p->sizePatchTime = 1;
+		p->sizeSearchTimeRangeFwd = 0;
+		p->sizeSearchTimeRangeBwd = 0;
+		p->sizeSearchWindowPred = 0;
+		p->offSetTime = 1;
+		p->dsub = 0;
+		p->nSimilarPatchesPred = p->nSimilarPatches;
+		p->sizeSearchWindow = 39;
+	} else {
+		p->sizePatch = (s1 || sigma >= 30) ? 8 : 7;
+		p->sizePatchTime = 1;
+		p->nSimilarPatches = 8;
+		p->sizeSearchWindow = 7;
+		p->sizeSearchWindowPred = 5;
+		p->sizeSearchTimeRangeFwd = 4;
+		p->sizeSearchTimeRangeBwd = 4;
+		p->nSimilarPatchesPred = 2;
+		p->offSet = s1 ? 6 : 4;
+		p->offSetTime = 1;
+		p->beta = s1 ? 2.7f : 1.f;
+		p->tau = s1 ? 67.1f : 54.8f;
+		p->dsub = s1 ? 7 : 3;
+	}
+	p->betaMean = p->beta;
+	p->orderInvariance = 0;
+	p->isFirstStep = s1;
+	p->sigma = sigma;
+	p->verbose = verbose;
+	p->colorSpace = BM4DOF_YUV;
+	p->doPasteBoost = 0;
+	p->agg_window = 1;
+	p->boundary = 2 * (p->sizeSearchWindow / 2) + (p->sizePatch - 1);
+	p->transform = s1 ? BM4DOF_BIOR1_5 : BM4DOF_DCT;
+}
+
+// ---------------------------------------------------------------- tiling
+
+void determine_factor(unsigned n, unsigned &a, unsigned &b) // utilities.cpp:137-158
+{
+	if (n == 1) { a = 1; b = 1; return; }
+	b = 2;
+	while (n % b > 0) b++;
+	a = n / b;
+	if (b > a) { a = b; b = n / a; }
+}
+
+struct Tiling {
+	int nW, nH, wTmp, hTmp;
+	std::vector<TileBox> crop;
+	std::vector<int> ix0, iy0, ix1, iy1; // interiors
+};
+
+// subDivideTight geometry, lib_videot.hpp:854-916.
+Tiling make_tiling(int W, int H, int bdry, int nparts)
+{
+	Tiling t;
+	unsigned a, b;
+	determine_factor((unsigned)nparts, a, b);
+	t.nW = (int)a;
+	t.nH = (int)b;
+	t.wTmp = (int)std::ceil((float)W / (float)t.nW);
+	t.hTmp = (int)std::ceil((float)H / (float)t.nH);
+	for (int p = 0; p < t.nH; p++)
+		for (int q = 0; q < t.nW; q++) {
+			TileBox c;
+			c.ox = std::max(0, q * t.wTmp - bdry);
+			c.oy = std::max(0, p * t.hTmp - bdry);
+			c.ex = std::min(W, (q + 1) * t.wTmp + bdry);
+			c.ey = std::min(H, (p + 1) * t.hTmp + bdry);
+			if (c.ex <= c.ox || c.ey <= c.oy)
+				throw ArgError("degenerate tiling: " + std::to_string(nparts) +
+				               " parts leave a tile with no pixels (the reference reads out of "
+				               "bounds here)");
+			t.crop.push_back(c);
+			t.ix0.push_back(q * t.wTmp);
+			t.iy0.push_back(p * t.hTmp);
+			t.ix1.push_back(std::min((q + 1) * t.wTmp, W));
+			t.iy1.push_back(std::min((p + 1) * t.hTmp, H));
+		}
+	return t;
+}
+
+// Reference grid of one tile (processNlBayes, nldct.cpp:513-553).  The
+// selection rule is a product of per-axis rules, so the raster-ordered grid
+// is the cartesian product of three axis lists.
+void axis_list(int len, int ori, int end, int step, bool b0, bool b1, std::vector<int> &out)
+{
+	out.clear();
+	for (int v = ori, d = 0; v < end; v++, d++)
+		if ((d % step == 0) || (!b1 && v == end - 1) || (b0 ? false : v == ori))
+			out.push_back(v);
+	(void)len;
+}
+
+void tile_grid(const bm4dof_params &p, int Wt, int Ht, int F, bool bx0, bool by0, bool bx1,
+               bool by1, std::vector<int> &xs, std::vector<int> &ys, std::vector<int> &ts)
+{
+	const int sWx = (int)p.sizeSearchWindow;
+	const int sWt = (int)(p.sizeSearchTimeRangeFwd + p.sizeSearchTimeRangeBwd + 1);
+	const int sPx = (int)p.sizePatch, sPt = (int)p.sizePatchTime;
+	const int ori_x = bx0 ? sPx - 1 + sWx / 2 : 0;
+	const int ori_y = by0 ? sPx - 1 + sWx / 2 : 0;
+	const int ori_f = 0; // tiles never split time (border_t0 = false)
+	const int end_x = Wt - (bx1 ? sPx - 1 + sWx / 2 : sPx - 1);
+	const int end_y = Ht - (by1 ? sPx - 1 + sWx / 2 : sPx - 1);
+	const int end_f = F - (sPt - 1);
+	(void)sWt;
+	axis_list(Wt, ori_x, end_x, (int)p.offSet, bx0, bx1, xs);
+	axis_list(Ht, ori_y, end_y, (int)p.offSet, by0, by1, ys);
+	// time: (df % stepf == 0) || (!border_t1 && f == end_f - 1); border_t1 is false.
+	ts.clear();
+	for (int f = ori_f, d = 0; f < end_f; f++, d++)
+		if ((d % (int)p.offSetTime == 0) || f == end_f - 1) ts.push_back(f);
+}
+
+// ---------------------------------------------------------------- bases
+
+const double kBiorFwd[64] = { // nldct.cpp:640-647
+	 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274, 0.353553390593274,
+	 0.219417649252501, 0.449283757993216, 0.449283757993216, 0.219417649252501,-0.219417649252501,-0.449283757993216,-0.449283757993216,-0.219417649252501,
+	 0.569359398342846, 0.402347308162278,-0.402347308162278,-0.569359398342846,-0.083506045090284, 0.083506045090284,-0.083506045090284, 0.083506045090284,
+	-0.083506045090284, 0.083506045090284,-0.083506045090284, 0.083506045090284, 0.569359398342846, 0.402347308162278,-0.402347308162278,-0.569359398342846,
+	 0.707106781186547,-0.707106781186547, 0, 0, 0, 0, 0, 0,
+	 0, 0, 0.707106781186547,-0.707106781186547, 0, 0, 0, 0,
+	 0, 0, 0, 0, 0.707106781186547,-0.707106781186547, 0, 0,
+	 0, 0, 0, 0, 0, 0, 0.707106781186547,-0.707106781186547 };
+const double kBiorInv[64] = { // nldct.cpp:670-677
+	0.353553390593274, 0.353553390593274, 0.5, 0.0, 0.707106781186547,-0.121533978016438,-0.000000000000000, 0.121533978016438,
+	0.353553390593274, 0.353553390593274, 0.5,-0.0,-0.707106781186547,-0.121533978016438, 0.000000000000000, 0.121533978016438,
+	0.353553390593274, 0.353553390593274,-0.5,-0.0, 0.121533978016438, 0.707106781186548,-0.121533978016438, 0.000000000000000,
+	0.353553390593274, 0.353553390593274,-0.5, 0.0, 0.121533978016438,-0.707106781186548,-0.121533978016438, 0.000000000000000,
+	0.353553390593274,-0.353553390593274, 0.0, 0.5, 0.000000000000000, 0.121533978016438, 0.707106781186547,-0.121533978016438,
+	0.353553390593274,-0.353553390593274, 0.0, 0.5, 0.000000000000000, 0.121533978016438,-0.707106781186547,-0.121533978016438,
+	0.353553390593274,-0.353553390593274, 0.0,-0.5,-0.121533978016438,-0.000000000000000, 0.121533978016438, 0.707106781186547,
+	0.353553390593274,-0.353553390593274, 0.0,-0.5,-0.121533978016438,-0.000000000000000, 0.121533978016438,-0.707106781186547 };
+const float kKaiser8[8] = {0.1924f, 0.2989f, 0.3846f, 0.4325f, 0.4325f, 0.3846f, 0.2989f, 0.1924f};
+const float kKaiser8Rows[64] = { // nldct.cpp:706-713
+	0.1924f, 0.2989f, 0.3846f, 0.4325f, 0.4325f, 0.3846f, 0.2989f, 0.1924f,
+	0.2989f, 0.4642f, 0.5974f, 0.6717f, 0.6717f, 0.5974f, 0.4642f, 0.2989f,
+	0.3846f, 0.5974f, 0.7688f, 0.8644f, 0.8644f, 0.7688f, 0.5974f, 0.3846f,
+	0.4325f, 0.6717f, 0.8644f, 0.9718f, 0.9718f, 0.8644f, 0.6717f, 0.4325f,
+	0.4325f, 0.6717f, 0.8644f, 0.9718f, 0.9718f, 0.8644f, 0.6717f, 0.4325f,
+	0.3846f, 0.5974f, 0.7688f, 0.8644f, 0.8644f, 0.7688f, 0.5974f, 0.3846f,
+	0.2989f, 0.4642f, 0.5974f, 0.6717f, 0.6717f, 0.5974f, 0.4642f, 0.2989f,
+	0.1924f, 0.2989f, 0.3846f, 0.4325f, 0.4325f, 0.3846f, 0.2989f, 0.1924f };
+const float kKaiser10Rows[100] = { // nldct.cpp:720-729
+	0.1924f, 0.2763f, 0.3503f, 0.4055f, 0.4349f, 0.4349f, 0.4055f, 0.3503f, 0.2763f, 0.1924f,
+	0.2763f, 0.3967f, 0.5030f, 0.5822f, 0.6245f, 0.6245f, 0.5822f, 0.5030f, 0.3967f, 0.2763f,
+	0.3503f, 0.5030f, 0.6377f, 0.7381f, 0.7917f, 0.7917f, 0.7381f, 0.6377f, 0.5030f, 0.3503f,
+	0.4055f, 0.5822f, 0.7381f, 0.8544f, 0.9164f, 0.9164f, 0.8544f, 0.7381f, 0.5822f, 0.4055f,
+	0.4349f, 0.6245f, 0.7917f, 0.9164f, 0.9829f, 0.9829f, 0.9164f, 0.7917f, 0.6245f, 0.4349f,
+	0.4349f, 0.6245f, 0.7917f, 0.9164f, 0.9829f, 0.9829f, 0.9164f, 0.7917f, 0.6245f, 0.4349f,
+	0.4055f, 0.5822f, 0.7381f, 0.8544f, 0.9164f, 0.9164f, 0.8544f, 0.7381f, 0.5822f, 0.4055f,
+	0.3503f, 0.5030f, 0.6377f, 0.7381f, 0.7917f, 0.7917f, 0.7381f, 0.6377f, 0.5030f, 0.3503f,
+	0.2763f, 0.3967f, 0.5030f, 0.5822f, 0.6245f, 0.6245f, 0.5822f, 0.5030f, 0.3967f, 0.2763f,
+	0.1924f, 0.2763f, 0.3503f, 0.4055f, 0.4349f, 0.4349f, 0.4055f, 0.3503f, 0.2763f, 0.1924f };
+
+// Patch bases, processNlBayes set-up (nldct.cpp:571-737): basis[k*P+pix],
+// binv[k*P+pix] (the matrix the inverse productMatrix reads with transB),
+// window[pix].
+void build_bases(const bm4dof_params &p, std::vector<float> &basis, std::vector<float> &binv,
+                 std::vector<float> &win)
+{
+	const unsigned sPx = p.sizePatch, sPt = p.sizePatchTime;
+	const size_t P = (size_t)sPx * sPx * sPt;
+	basis.assign(P * P, 0.f);
+	binv.assign(P * P, 0.f);
+	win.assign(P, 1.f);
+	if (p.transform == BM4DOF_DCT) {
+		std::vector<float> cosx(sPx * sPx), cost(sPt * sPt);
+		for (unsigned kx = 0, i = 0; kx < sPx; kx++)
+			for (unsigned nx = 0; nx < sPx; nx++, i++)
+				cosx[i] = (float)(std::cos((0.5 + (float)nx) * (float)kx * M_PI / (float)sPx) *
+				                  std::sqrt((kx == 0 ? 1. : 2.) / (float)sPx));
+		for (unsigned kt = 0, i = 0; kt < sPt; kt++)
+			for (unsigned nt = 0; nt < sPt; nt++, i++)
+				cost[i] = (float)(std::cos((0.5 + (float)nt) * (float)kt * M_PI / (float)sPt) *
+				                  std::sqrt((kt == 0 ? 1. : 2.) / (float)sPt));
+		size_t i = 0;
+		for (unsigned kt = 0; kt < sPt; kt++)
+			for (unsigned ky = 0; ky < sPx; ky++)
+				for (unsigned kx = 0; kx < sPx; kx++)
+					for (unsigned nt = 0; nt < sPt; nt++)
+						for (unsigned ny = 0; ny < sPx; ny++)
+							for (unsigned nx = 0; nx < sPx; nx++, i++) {
+								const float v = cosx[kx * sPx + nx] * cosx[ky * sPx + ny];
+								basis[i] = v * cost[kt * sPt + nt];
+							}
+		binv = basis;
+	} else {
+		if (sPx != 8)
+			throw ArgError("VideoNLB::processNlBayes: bior1_5 transform is only implemented for "
+			               "patches of size 8 x 8 x pt.");
+		float bx[64], bxi[64];
+		for (int k = 0; k < 64; k++) { bx[k] = (float)kBiorFwd[k]; bxi[k] = (float)kBiorInv[k]; }
+		size_t i = 0;
+		for (unsigned kt = 0; kt < sPt; kt++)
+			for (unsigned ky = 0; ky < 8; ky++)
+				for (unsigned kx = 0; kx < 8; kx++)
+					for (unsigned nt = 0; nt < sPt; nt++)
+						for (unsigned ny = 0; ny < 8; ny++)
+							for (unsigned nx = 0; nx < 8; nx++, i++) {
+								const float v = bx[kx * 8 + nx] * bx[ky * 8 + ny];
+								basis[i] = v * (nt == kt ? 1.f : 0.f);
+							}
+		i = 0;
+		for (unsigned nt = 0; nt < sPt; nt++)
+			for (unsigned ny = 0; ny < 8; ny++)
+				for (unsigned nx = 0; nx < 8; nx++)
+					for (unsigned kt = 0; kt < sPt; kt++)
+						for (unsigned ky = 0; ky < 8; ky++)
+							for (unsigned kx = 0; kx < 8; kx++, i++) {
+								const float v = bxi[kx * 8 + nx] * bxi[ky * 8 + ny];
+								binv[i] = v * (nt == kt ? 1.f : 0.f);
+							}
+	}
+	if (p.agg_window && sPx == 8)
+		for (unsigned t = 0; t < sPt; t++) std::copy(kKaiser8Rows, kKaiser8Rows + 64, win.begin() + 64 * t);
+	else if (p.agg_window && sPx == 10)
+		for (unsigned t = 0; t < sPt; t++) std::copy(kKaiser10Rows, kKaiser10Rows + 100, win.begin() + 100 * t);
+	(void)kKaiser8;
+}
+
+// ---------------------------------------------------------------- pipeline
+
+struct Timer {
+	bm4dof_ctx *ctx;
+	explicit Timer(bm4dof_ctx *c) : ctx(c) {}
+	void mark(const char *name) {
+		hipEvent_t e;
+		HIPCHK(hipEventCreate(&e));
+		HIPCHK(hipEventRecord(e, ctx->stream));
+		ctx->ev.push_back(e);
+		ctx->timings.push_back({name, 0.f});
+	}
+};
+
+void check_step_params(const bm4dof_params &p, unsigned C)
+{
+	if (p.doPasteBoost)
+		throw ArgError("doPasteBoost is not supported (the reference never enables it)", BM4DOF_EUNSUPP);
+	if (p.nSimilarPatches > 64)
+		throw ArgError("nSimilarPatches above 64 is not supported", BM4DOF_EUNSUPP);
+	const unsigned P = p.sizePatch * p.sizePatch * p.sizePatchTime;
+	if (P == 0 || P > 256)
+		throw ArgError("patch of more than 256 samples is not supported", BM4DOF_EUNSUPP);
+	if (p.sizeSearchTimeRangeFwd + p.sizeSearchTimeRangeBwd + 1 > 64)
+		throw ArgError("temporal search range above 64 frames is not supported", BM4DOF_EUNSUPP);
+	if (p.offSet == 0 || p.offSetTime == 0) throw ArgError("offSet and offSetTime must be positive");
+	if (p.sizeSearchWindow == 0) throw ArgError("sizeSearchWindow must be positive");
+	(void)C;
+}
+
+struct StepGeom {
+	Tiling til;
+	std::vector<uint32_t> gpos, gtile;
+	std::vector<uint32_t> tile_first; // first group of each tile
+};
+
+StepGeom make_geometry(const bm4dof_params &p, int W, int H, int F, int nparts)
+{
+	StepGeom g;
+	g.til = make_tiling(W, H, (int)p.boundary, nparts);
+	std::vector<int> xs, ys, ts;
+	const uint32_t WH = (uint32_t)W * (uint32_t)H;
+	for (int n = 0; n < nparts; n++) {
+		const TileBox &c = g.til.crop[n];
+		g.tile_first.push_back((uint32_t)g.gpos.size());
+		tile_grid(p, c.ex - c.ox, c.ey - c.oy, F, c.ox > 0, c.oy > 0, c.ex < W, c.ey < H, xs, ys, ts);
+		for (int t : ts)
+			for (int y : ys)
+				for (int x : xs) {
+					g.gpos.push_back((uint32_t)t * WH + (uint32_t)(y + c.oy) * (uint32_t)W + (uint32_t)(x + c.ox));
+					g.gtile.push_back((uint32_t)n);
+				}
+	}
+	return g;
+}
+
+StepConst make_const(const bm4dof_params &p, int W, int H, int F, int C, bool use_flow,
+                     const Tiling &t)
+{
+	StepConst sc;
+	sc.W = W; sc.H = H; sc.F = F; sc.C = C;
+	sc.WH = W * H;
+	sc.WHC = W * H * C;
+	sc.sPx = (int)p.sizePatch;
+	sc.sPt = (int)p.sizePatchTime;
+	sc.P = sc.sPx * sc.sPx * sc.sPt;
+	sc.sWx = (int)p.sizeSearchWindow;
+	sc.wtf = (int)p.sizeSearchTimeRangeFwd;
+	sc.wtb = (int)p.sizeSearchTimeRangeBwd;
+	sc.npmax = (int)std::max(1u, p.nSimilarPatches);
+	sc.tau2 = p.tau * p.tau * sc.sPx * sc.sPx * sc.sPt;
+	const float beta_sigma = p.beta * p.sigma;
+	sc.beta_sigma2 = beta_sigma * beta_sigma;
+	const float betaM_sigma = p.betaMean * p.sigma;
+	sc.betaM_sigma2 = betaM_sigma * betaM_sigma;
+	sc.sigma2 = p.sigma * p.sigma;
+	sc.order_inv = p.orderInvariance ? 1 : 0;
+	sc.use_flow = use_flow ? 1 : 0;
+	sc.nW = t.nW;
+	sc.wTmp = t.wTmp;
+	sc.hTmp = t.hTmp;
+	return sc;
+}
+
+inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// Colour transform on device, constants as transformColorSpace computes them.
+void color(bm4dof_ctx *ctx, float *v, int W, int H, int F, int C, bool forward)
+{
+	if (C == 1) return;
+	float a, b, c;
+	if (C == 3) {
+		a = 1.f / sqrtf(3.f);
+		b = 1.f / sqrtf(2.f);
+		c = forward ? 2.f * a * sqrtf(2.f) : a / b;
+	} else {
+		a = 0.5f;
+		b = 1.f / sqrtf(2.f);
+		c = 0.f;
+	}
+	const int64_t n = (int64_t)W * H * F;
+	hipLaunchKernelGGL(color_kernel, dim3(std::min<int64_t>(nblk(n, 256), 65535 * 4)), dim3(256), 0,
+	                   ctx->stream, v, W * H, C, F, forward ? 1 : 0, a, b, c);
+	HIPCHK(hipGetLastError());
+}
+
+// Block matching of all groups of a step.
+void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, const float *ff,
+            const float *bf, const uint32_t *d_gpos, const uint32_t *d_gtile, const TileBox *d_tiles,
+            int G, uint32_t *d_idx, uint32_t *d_nsim)
+{
+	const int sWt = sc.wtf + sc.wtb + 1;
+	const int maxCand = sc.sWx * sc.sWx * sWt;
+	const size_t lds = (size_t)maxCand * 12 + (size_t)sc.C * sc.P * 4;
+	if (lds > 60 * 1024) throw ArgError("search region too large for the block-matching kernel", BM4DOF_EUNSUPP);
+	if (G == 0) return;
+	hipLaunchKernelGGL(bm_kernel, dim3(G), dim3(256), lds, ctx->stream, img, ff, bf, d_gpos, d_gtile,
+	                   d_tiles, sc, step, maxCand, d_idx, d_nsim, G);
+	HIPCHK(hipGetLastError());
+}
+
+// One denoising step on device buffers in the working colour space.
+//   step1: noisy_w -> out_w (basic)
+//   step2: noisy_w, basic_w -> out_w (final)
+// Returns the number of groups processed.
+uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
+                  const float *noisy_w, const float *basic_w, const float *ff, const float *bf,
+                  float *out_w, int nparts, Timer &tm)
+{
+	const bool step1 = p.isFirstStep != 0;
+	const bool use_flow = ff != nullptr && bf != nullptr;
+	check_step_params(p, (unsigned)C);
+	std::vector<float> hb, hbi, hw;
+	build_bases(p, hb, hbi, hw);
+	StepGeom geo = make_geometry(p, W, H, F, nparts);
+	StepConst sc = make_const(p, W, H, F, C, use_flow, geo.til);
+	const int G = (int)geo.gpos.size();
+	const int np = sc.npmax;
+	const size_t WHF = (size_t)W * H * F;
+	const size_t N_all = (size_t)WHF * C;
+
+	float *d_basis = ctx->basis.as<float>(hb.size());
+	float *d_binv = ctx->binv.as<float>(hbi.size());
+	float *d_win = ctx->win.as<float>(hw.size());
+	HIPCHK(hipMemcpyAsync(d_basis, hb.data(), hb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(hipMemcpyAsync(d_binv, hbi.data(), hbi.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(hipMemcpyAsync(d_win, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+	TileBox *d_tiles = ctx->tiles.as<TileBox>(geo.til.crop.size());
+	HIPCHK(hipMemcpyAsync(d_tiles, geo.til.crop.data(), geo.til.crop.size() * sizeof(TileBox),
+	                      hipMemcpyHostToDevice, ctx->stream));
+	uint32_t *d_gpos = ctx->gpos.as<uint32_t>(std::max(G, 1));
+	uint32_t *d_gtile = ctx->gtile.as<uint32_t>(std::max(G, 1));
+	HIPCHK(hipMemcpyAsync(d_gpos, geo.gpos.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(hipMemcpyAsync(d_gtile, geo.gtile.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+
+	uint32_t *d_idx = ctx->idx.as<uint32_t>((size_t)std::max(G, 1) * np);
+	uint32_t *d_nsim = ctx->nsim.as<uint32_t>(std::max(G, 1));
+	uint32_t *d_rank = ctx->rank_of.as<uint32_t>((size_t)std::max(G, 1) * np);
+	uint8_t *d_early = ctx->early.as<uint8_t>(std::max(G, 1));
+	float *d_gw = ctx->gweight.as<float>(std::max(G, 1));
+	uint32_t *d_cnt = ctx->counters.as<uint32_t>(16);
+	double *d_num = ctx->num.as<double>(N_all);
+	double *d_wsum = ctx->wsum.as<double>(WHF);
+	uint32_t *d_lstart = ctx->lstart.as<uint32_t>(WHF);
+	uint32_t *d_lend = ctx->lend.as<uint32_t>(WHF);
+	HIPCHK(hipMemsetAsync(d_cnt, 0, 64, ctx->stream));
+	HIPCHK(hipMemsetAsync(d_num, 0, N_all * 8, ctx->stream));
+	HIPCHK(hipMemsetAsync(d_wsum, 0, WHF * 8, ctx->stream));
+	HIPCHK(hipMemsetAsync(d_lstart, 0, WHF * 4, ctx->stream));
+	HIPCHK(hipMemsetAsync(d_lend, 0, WHF * 4, ctx->stream));
+	tm.mark(step1 ? "s1_setup" : "s2_setup");
+
+	run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, d_gpos, d_gtile, d_tiles, G,
+	       d_idx, d_nsim);
+	tm.mark(step1 ? "s1_bm" : "s2_bm");
+
+	// Chunks bounded by the estimate buffer budget.
+	const size_t row_bytes = (size_t)C * sc.P * 4;
+	const size_t max_rows = std::max<size_t>(np, ctx->est_budget / row_bytes);
+	const int chunk_groups = (int)std::min<size_t>((size_t)std::max(G, 1), max_rows / np);
+	uint32_t n_early_seen = 0;
+	std::vector<float> slot_state; // host-side stale-weight resolution (rare path)
+
+	const size_t filt_lds = step1 ? ((size_t)C * sc.P * np + (size_t)sc.P * np) * 4
+	                              : ((size_t)C * sc.P * np + 2 * (size_t)sc.P * np) * 4;
+	if (filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
+
+	for (int g0 = 0; g0 < G; g0 += chunk_groups) {
+		const int Gc = std::min(chunk_groups, G - g0);
+		// entry offsets of the chunk
+		uint32_t *d_base = ctx->base.as<uint32_t>(Gc + 1);
+		size_t tmp_bytes = 0;
+		HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_nsim + g0, d_base, Gc, ctx->stream));
+		void *d_tmp = ctx->sort_tmp.get(tmp_bytes);
+		HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_nsim + g0, d_base, Gc, ctx->stream));
+		uint32_t last_base = 0, last_n = 0;
+		HIPCHK(hipMemcpyAsync(&last_base, d_base + Gc - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipMemcpyAsync(&last_n, d_nsim + g0 + Gc - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		const int64_t M = (int64_t)last_base + last_n;
+
+		uint32_t *d_keys = ctx->keys.as<uint32_t>(M);
+		uint32_t *d_vals = ctx->vals.as<uint32_t>(M);
+		uint32_t *d_skeys = ctx->skeys.as<uint32_t>(M);
+		uint32_t *d_svals = ctx->svals.as<uint32_t>(M);
+		uint32_t *d_etile = ctx->etile.as<uint32_t>(M);
+		float *d_ew = ctx->entry_w.as<float>(M);
+		float *d_est = ctx->est.as<float>((size_t)M * C * sc.P);
+		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk((int64_t)Gc * np, 256)), dim3(256), 0,
+		                   ctx->stream, d_idx, d_nsim, d_base, np, g0, Gc, d_keys, d_vals);
+		HIPCHK(hipGetLastError());
+		int end_bit = 1;
+		while (end_bit < 32 && (1ull << end_bit) < WHF) end_bit++;
+		size_t sort_bytes = 0;
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, d_keys, d_skeys, d_vals, d_svals,
+		                                          (int)M, 0, end_bit, ctx->stream));
+		d_tmp = ctx->sort_tmp.get(sort_bytes);
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, sort_bytes, d_keys, d_skeys, d_vals, d_svals,
+		                                          (int)M, 0, end_bit, ctx->stream));
+		hipLaunchKernelGGL(mark_ranges_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream,
+		                   d_skeys, d_svals, M, np, d_gtile, d_lstart, d_lend, d_rank, d_etile);
+		HIPCHK(hipGetLastError());
+		tm.mark(step1 ? "s1_sort" : "s2_sort");
+
+		if (step1)
+			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
+			                   d_idx, d_nsim, d_rank, sc, g0, Gc, d_basis, d_binv, d_win, d_est,
+			                   d_ew, d_early, d_gw, d_cnt);
+		else
+			hipLaunchKernelGGL(filter2_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
+			                   basic_w, d_idx, d_nsim, d_rank, sc, g0, Gc, d_basis, d_binv, d_win,
+			                   d_est, d_ew, d_early, d_gw, d_cnt);
+		HIPCHK(hipGetLastError());
+		tm.mark(step1 ? "s1_filter" : "s2_filter");
+
+		// Early-out groups reuse the previous group's weights (rare).
+		uint32_t n_early = 0;
+		HIPCHK(hipMemcpyAsync(&n_early, d_cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		if (n_early != n_early_seen) {
+			n_early_seen = n_early;
+			const int gend = g0 + Gc;
+			std::vector<uint8_t> h_early(gend);
+			std::vector<float> h_gw(gend);
+			std::vector<uint32_t> h_nsim(gend);
+			HIPCHK(hipMemcpy(h_early.data(), d_early, gend, hipMemcpyDeviceToHost));
+			HIPCHK(hipMemcpy(h_gw.data(), d_gw, (size_t)gend * 4, hipMemcpyDeviceToHost));
+			HIPCHK(hipMemcpy(h_nsim.data(), d_nsim, (size_t)gend * 4, hipMemcpyDeviceToHost));
+			std::vector<uint32_t> glist;
+			std::vector<float> stale((size_t)gend * np, 0.f);
+			// aggreWeights start at 1 in every tile (nldct.cpp:749, 818) and keep
+			// the last written value per slot.
+			std::vector<float> slots(np, 1.f);
+			uint32_t tile_cur = 0xffffffffu;
+			for (int g = 0; g < gend; g++) {
+				if (geo.gtile[g] != tile_cur) { tile_cur = geo.gtile[g]; std::fill(slots.begin(), slots.end(), 1.f); }
+				if (h_early[g]) {
+					if (g >= g0) {
+						glist.push_back((uint32_t)g);
+						for (uint32_t n = 0; n < h_nsim[g]; n++) stale[(size_t)g * np + n] = slots[n];
+					}
+				} else
+					for (uint32_t n = 0; n < h_nsim[g]; n++) slots[n] = h_gw[g];
+			}
+			if (!glist.empty()) {
+				uint32_t *d_gl = ctx->elist.as<uint32_t>(glist.size());
+				float *d_st = ctx->stale.as<float>(stale.size());
+				HIPCHK(hipMemcpyAsync(d_gl, glist.data(), glist.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+				HIPCHK(hipMemcpyAsync(d_st, stale.data(), stale.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+				hipLaunchKernelGGL(fixup_early_kernel, dim3((unsigned)glist.size()), dim3(256), 0, ctx->stream,
+				                   d_gl, (int)glist.size(), d_nsim, d_rank, d_st, d_win, np, C, sc.P,
+				                   d_est, d_ew);
+				HIPCHK(hipGetLastError());
+				HIPCHK(hipStreamSynchronize(ctx->stream));
+			}
+		}
+
+		// Gather over the frames this chunk touches.
+		uint32_t kfirst = 0, klast = 0;
+		HIPCHK(hipMemcpyAsync(&kfirst, d_skeys, 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipMemcpyAsync(&klast, d_skeys + M - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		const int t0 = (int)(kfirst / (uint32_t)sc.WH);
+		const int t1 = std::min(F, (int)(klast / (uint32_t)sc.WH) + sc.sPt);
+		const int64_t npx = (int64_t)W * H * (t1 - t0);
+		hipLaunchKernelGGL(gather_kernel, dim3(nblk(npx, 256)), dim3(256), 0, ctx->stream, d_lstart,
+		                   d_lend, d_etile, d_ew, d_est, d_win, sc, 0, 0, t0, W, H, t1 - t0, d_num, d_wsum);
+		HIPCHK(hipGetLastError());
+		hipLaunchKernelGGL(clear_ranges_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, d_skeys,
+		                   M, d_lstart, d_lend);
+		HIPCHK(hipGetLastError());
+		tm.mark(step1 ? "s1_aggregate" : "s2_aggregate");
+	}
+	hipLaunchKernelGGL(normalize_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream,
+	                   d_num, d_wsum, noisy_w, sc, out_w);
+	HIPCHK(hipGetLastError());
+	tm.mark(step1 ? "s1_normalize" : "s2_normalize");
+	return (uint64_t)G;
+}
+
+// Whole runNlBayes on device buffers (nldct.cpp:267-478).
+void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, const float *noisy,
+             int W, int H, int F, int C, const float *ff, const float *bf, float *basic, float *final,
+             int nparts, float *ratio)
+{
+	if (!(C == 1 || C == 3 || C == 4))
+		throw ArgError("VideoNLB::runNlBayes: Wrong number of channels. Must be 1, 3 or 4.");
+	const int step = p1->sizePatch ? (p2->sizePatch ? 0 : 1) : (p2->sizePatch ? 2 : -1);
+	if (step == -1) throw ArgError("VideoNLB::runNlBayes: Both patch sizes are zero.");
+	if (nparts < 1) throw ArgError("nparts must be >= 1");
+	if (W <= 0 || H <= 0 || F <= 0) throw ArgError("empty video");
+	if ((uint64_t)W * H * F * C >= (1ull << 32))
+		throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
+	if (step != 1 && !final) throw ArgError("final output buffer is NULL");
+	HIPCHK(hipSetDevice(ctx->device));
+	for (auto e : ctx->ev) (void)hipEventDestroy(e);
+	ctx->ev.clear();
+	ctx->timings.clear();
+	Timer tm(ctx);
+	tm.mark("start");
+	const size_t n = (size_t)W * H * F * C;
+	float gr[2] = {0.f, 0.f};
+	const float whf = (float)((size_t)W * H * F);
+
+	if (step != 2) {
+		const float *nw = noisy;
+		if (C > 1 && p1->colorSpace == BM4DOF_YUV) {
+			float *t = ctx->noisy_w.as<float>(n);
+			HIPCHK(hipMemcpyAsync(t, noisy, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+			color(ctx, t, W, H, F, C, true);
+			nw = t;
+		}
+		const uint64_t G = run_step(ctx, *p1, W, H, F, C, nw, nullptr, ff, bf, basic, nparts, tm);
+		if (C > 1 && p1->colorSpace == BM4DOF_YUV) color(ctx, basic, W, H, F, C, false);
+		// groupsRatio[0] += 100 * groups / whf per tile (nldct.cpp:403-404); the
+		// per-tile counts are summed here in the same order.
+		gr[0] = 0.f;
+		(void)G;
+	}
+	if (step != 1) {
+		const float *nw = noisy;
+		if (C > 1 && p2->colorSpace == BM4DOF_YUV) {
+			float *t = ctx->noisy_w.as<float>(n);
+			HIPCHK(hipMemcpyAsync(t, noisy, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+			color(ctx, t, W, H, F, C, true);
+			color(ctx, basic, W, H, F, C, true);
+			nw = t;
+		}
+		run_step(ctx, *p2, W, H, F, C, nw, basic, ff, bf, final, nparts, tm);
+		if (C > 1 && p2->colorSpace == BM4DOF_YUV) {
+			color(ctx, basic, W, H, F, C, false);
+			color(ctx, final, W, H, F, C, false);
+		}
+	}
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	// group ratios: per tile 100*groups/whf, summed over tiles in order
+	for (int s = 0; s < 2; s++) {
+		if ((s == 0 && step == 2) || (s == 1 && step == 1)) continue;
+		const bm4dof_params &p = s == 0 ? *p1 : *p2;
+		StepGeom geo = make_geometry(p, W, H, F, nparts);
+		float r = 0.f;
+		for (int t = 0; t < nparts; t++) {
+			const uint32_t g0 = geo.tile_first[t];
+			const uint32_t g1 = t + 1 < nparts ? geo.tile_first[t + 1] : (uint32_t)geo.gpos.size();
+			r += 100.f * (float)(g1 - g0) / whf;
+		}
+		gr[s] = r;
+	}
+	if (ratio) { ratio[0] = gr[0]; ratio[1] = gr[1]; }
+	// timings
+	for (size_t i = 1; i < ctx->ev.size(); i++) {
+		float ms = 0.f;
+		HIPCHK(hipEventElapsedTime(&ms, ctx->ev[i - 1], ctx->ev[i]));
+		ctx->timings[i].second = ms;
+	}
+}
+
+} // namespace
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+__attribute__((visibility("default")))
+void bm4dof_init_params(bm4dof_params *p, unsigned step, float sigma, unsigned width,
+                        unsigned height, unsigned frames, unsigned channels, int verbose)
+{
+	(void)width; (void)height; (void)channels;
+	init_params(p, step, sigma, frames, verbose);
+}
+
+__attribute__((visibility("default")))
+void bm4dof_set_search_window(bm4dof_params *p, unsigned sw)
+{
+	p->sizeSearchWindow = sw;
+	p->boundary = 2 * (sw / 2) + (p->sizePatch - 1);
+	p->nSimilarPatches = std::min(p->nSimilarPatches,
+	                              sw * sw * (p->sizeSearchTimeRangeFwd + p->sizeSearchTimeRangeBwd + 1));
+}
+
+__attribute__((visibility("default")))
+void bm4dof_set_patch_size(bm4dof_params *p, unsigned frames, unsigned px)
+{
+	p->sizePatch = px;
+	p->boundary = 2 * (p->sizeSearchWindow / 2) + (p->sizePatch - 1);
+	if (p->sizePatch != 8) p->transform = BM4DOF_DCT;
+	if (frames > 1)
+		p->offSet = p->isFirstStep ? (unsigned)((double)(float)px / 4. * 3.) : px / 2;
+}
+
+__attribute__((visibility("default")))
+void bm4dof_set_n_similar(bm4dof_params *p, unsigned np)
+{
+	p->nSimilarPatches = std::min(np, p->sizeSearchWindow * p->sizeSearchWindow *
+	                                      (p->sizeSearchTimeRangeFwd + p->sizeSearchTimeRangeBwd + 1));
+	if (p->sizeSearchTimeRangeFwd + p->sizeSearchTimeRangeFwd == 0)
+		p->nSimilarPatchesPred = p->nSimilarPatches;
+}
+
+__attribute__((visibility("default")))
+const char *bm4dof_global_error(void) { return g_global_err.c_str(); }
+
+__attribute__((visibility("default")))
+int bm4dof_ctx_create(bm4dof_ctx **out, int device)
+{
+	if (!out) return BM4DOF_EINVAL;
+	*out = nullptr;
+	bm4dof_ctx *ctx = new bm4dof_ctx();
+	ctx->device = device;
+	int rc = guarded(ctx, [&] {
+		int n = 0;
+		HIPCHK(hipGetDeviceCount(&n));
+		if (device < 0 || device >= n) throw ArgError("no HIP device " + std::to_string(device));
+		HIPCHK(hipSetDevice(device));
+		HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+	});
+	if (rc) {
+		g_global_err = ctx->err;
+		delete ctx;
+		return rc;
+	}
+	*out = ctx;
+	return BM4DOF_OK;
+}
+
+__attribute__((visibility("default")))
+void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
+{
+	if (!ctx) return;
+	(void)hipSetDevice(ctx->device);
+	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+	for (auto e : ctx->ev) (void)hipEventDestroy(e);
+	DevBuf *bufs[] = {&ctx->gpos, &ctx->gtile, &ctx->tiles, &ctx->idx, &ctx->nsim, &ctx->base,
+	                  &ctx->keys, &ctx->vals, &ctx->skeys, &ctx->svals, &ctx->sort_tmp, &ctx->lstart,
+	                  &ctx->lend, &ctx->rank_of, &ctx->etile, &ctx->entry_w, &ctx->est, &ctx->early,
+	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
+	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
+	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
+	                  &ctx->bm_image};
+	for (DevBuf *b : bufs) b->release();
+	DevBuf *tvb[] = {&ctx->tv_frames, &ctx->tv_pairs, &ctx->tv_part, &ctx->tv_mm, &ctx->tv_state,
+	                 &ctx->tv_work, &ctx->tv_pyr, &ctx->tv_taps};
+	for (DevBuf *b : tvb) b->release();
+	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+__attribute__((visibility("default")))
+const char *bm4dof_last_error(const bm4dof_ctx *ctx) { return ctx ? ctx->err.c_str() : g_global_err.c_str(); }
+
+__attribute__((visibility("default")))
+int bm4dof_run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+                   const float *noisy, unsigned w, unsigned h, unsigned f, unsigned c,
+                   const float *ff, const float *bf, float *basic, float *final, int nparts,
+                   float groups_ratio[2])
+{
+	if (!ctx || !p1 || !p2 || !noisy || !basic) { set_err(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		run_dev(ctx, p1, p2, noisy, (int)w, (int)h, (int)f, (int)c, (ff && bf) ? ff : nullptr,
+		        (ff && bf) ? bf : nullptr, basic, final, nparts, groups_ratio);
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_run(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+               const float *noisy, unsigned w, unsigned h, unsigned f, unsigned c,
+               const float *ff, const float *bf, float *basic, float *final, int nparts,
+               float groups_ratio[2])
+{
+	if (!ctx || !p1 || !p2 || !noisy || !basic) { set_err(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		const size_t n = (size_t)w * h * f * c;
+		const size_t nf = (size_t)w * h * f * 2;
+		const bool flow = ff && bf;
+		const int step = p1->sizePatch ? (p2->sizePatch ? 0 : 1) : (p2->sizePatch ? 2 : -1);
+		float *dn = ctx->io_noisy.as<float>(n);
+		float *db = ctx->io_basic.as<float>(n);
+		float *dfin = step != 1 ? ctx->io_final.as<float>(n) : nullptr;
+		float *dff = flow ? ctx->io_fflow.as<float>(nf) : nullptr;
+		float *dbf = flow ? ctx->io_bflow.as<float>(nf) : nullptr;
+		HIPCHK(hipMemcpyAsync(dn, noisy, n * 4, hipMemcpyHostToDevice, ctx->stream));
+		if (step == 2) HIPCHK(hipMemcpyAsync(db, basic, n * 4, hipMemcpyHostToDevice, ctx->stream));
+		if (flow) {
+			HIPCHK(hipMemcpyAsync(dff, ff, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+			HIPCHK(hipMemcpyAsync(dbf, bf, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+		}
+		run_dev(ctx, p1, p2, dn, (int)w, (int)h, (int)f, (int)c, dff, dbf, db, dfin, nparts, groups_ratio);
+		HIPCHK(hipMemcpyAsync(basic, db, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+		if (step != 1 && final) HIPCHK(hipMemcpyAsync(final, dfin, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_bm_count(bm4dof_ctx *ctx, const bm4dof_params *p, unsigned w, unsigned h, unsigned f,
+                    unsigned c, int nparts, int64_t *n_groups)
+{
+	(void)c;
+	return guarded(ctx, [&] {
+		StepGeom geo = make_geometry(*p, (int)w, (int)h, (int)f, nparts);
+		*n_groups = (int64_t)geo.gpos.size();
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image, unsigned w, unsigned h,
+              unsigned f, unsigned c, const float *ff, const float *bf, int nparts,
+              uint32_t *out_index, uint32_t *out_nsim, uint32_t *out_ref, uint32_t *out_tile)
+{
+	if (!ctx || !p || !image) { set_err(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		check_step_params(*p, c);
+		const int W = (int)w, H = (int)h, F = (int)f, C = (int)c;
+		const bool flow = ff && bf;
+		StepGeom geo = make_geometry(*p, W, H, F, nparts);
+		StepConst sc = make_const(*p, W, H, F, C, flow, geo.til);
+		const int G = (int)geo.gpos.size();
+		const size_t n = (size_t)W * H * F * C, nf = (size_t)W * H * F * 2;
+		float *dimg = ctx->bm_image.as<float>(n);
+		HIPCHK(hipMemcpyAsync(dimg, image, n * 4, hipMemcpyHostToDevice, ctx->stream));
+		float *dff = nullptr, *dbf = nullptr;
+		if (flow) {
+			dff = ctx->io_fflow.as<float>(nf);
+			dbf = ctx->io_bflow.as<float>(nf);
+			HIPCHK(hipMemcpyAsync(dff, ff, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+			HIPCHK(hipMemcpyAsync(dbf, bf, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+		}
+		TileBox *d_tiles = ctx->tiles.as<TileBox>(geo.til.crop.size());
+		HIPCHK(hipMemcpyAsync(d_tiles, geo.til.crop.data(), geo.til.crop.size() * sizeof(TileBox),
+		                      hipMemcpyHostToDevice, ctx->stream));
+		uint32_t *d_gpos = ctx->gpos.as<uint32_t>(std::max(G, 1));
+		uint32_t *d_gtile = ctx->gtile.as<uint32_t>(std::max(G, 1));
+		HIPCHK(hipMemcpyAsync(d_gpos, geo.gpos.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+		HIPCHK(hipMemcpyAsync(d_gtile, geo.gtile.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+		uint32_t *d_idx = ctx->idx.as<uint32_t>((size_t)std::max(G, 1) * sc.npmax);
+		uint32_t *d_nsim = ctx->nsim.as<uint32_t>(std::max(G, 1));
+		run_bm(ctx, sc, p->isFirstStep ? 1 : 2, dimg, dff, dbf, d_gpos, d_gtile, d_tiles, G, d_idx, d_nsim);
+		std::vector<uint32_t> hidx((size_t)G * sc.npmax), hn(G);
+		HIPCHK(hipMemcpyAsync(hidx.data(), d_idx, hidx.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipMemcpyAsync(hn.data(), d_nsim, hn.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		const uint32_t WH = (uint32_t)(W * H);
+		auto enc = [&](uint32_t pos) { // 3-index position -> reference patch index
+			const uint32_t t = pos / WH;
+			return t * WH * (uint32_t)C + (pos - t * WH);
+		};
+		for (int g = 0; g < G; g++) {
+			for (int k = 0; k < sc.npmax; k++) {
+				const size_t o = (size_t)g * sc.npmax + k;
+				out_index[o] = k < (int)hn[g] ? enc(hidx[o]) : 0xffffffffu;
+			}
+			out_nsim[g] = hn[g];
+			out_ref[g] = enc(geo.gpos[g]);
+			out_tile[g] = geo.gtile[g];
+		}
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, int max_slots)
+{
+	if (!ctx) return 0;
+	int n = 0;
+	for (size_t i = 1; i < ctx->timings.size() && n < max_slots; i++, n++) {
+		if (ms) ms[n] = ctx->timings[i].second;
+		if (names) names[n] = ctx->timings[i].first;
+	}
+	return n;
+}
+
+} // extern "C"

@@ -1,0 +1,771 @@
+// HIP kernels of the BM4D-OF / NL-DCT denoiser for gfx950 (MI355X).
+//
+// Arithmetic that decides a discontinuous outcome is evaluated in the
+// reference's exact operation order with separately rounded IEEE float
+// operations (this file is compiled with -ffp-contract=off):
+//   * patch distances, sequential over (c, ht, hy, hx)  (nldct.cpp:1262-1274,
+//     1723-1737);
+//   * the forward patch transform, a sequential-K dot product per
+//     coefficient (nldct.cpp:1840-1844 via productMatrix/cblas_sgemm);
+//   * the Haar butterflies and the hard threshold (nldct.cpp:1847-1907).
+// Smooth quantities (inverse transform, Wiener weights, aggregation sums) use
+// a fixed but different order; they match the reference to rounding.
+#include "common.hpp"
+
+namespace bm4dof {
+
+#define BM_THREADS 256
+#define BM_MAX_FRAMES 64
+#define FILTER_THREADS 256
+
+// std::min / std::max as the reference calls them on floats.
+__device__ __forceinline__ float fmin_ref(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmax_ref(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ uint32_t video_addr(int x, int y, int t, int c, const StepConst &sc)
+{
+	return (uint32_t)t * (uint32_t)sc.WHC + (uint32_t)c * (uint32_t)sc.WH + (uint32_t)y * (uint32_t)sc.W + (uint32_t)x;
+}
+
+// ---------------------------------------------------------------------------
+// Colour transform, VideoUtils::transformColorSpace (lib_videot.hpp:728-838).
+// k: coefficients computed on the host exactly as the reference does.
+// ---------------------------------------------------------------------------
+__global__ void color_kernel(float *__restrict__ v, int WH, int C, int F, int forward,
+                             float a, float b, float c)
+{
+	const int64_t n = (int64_t)WH * F;
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t t = i / WH, k = i - t * WH;
+		float *p = v + t * (int64_t)WH * C + k;
+		if (C == 3) {
+			const float x0 = p[0], x1 = p[WH], x2 = p[2 * (int64_t)WH];
+			if (forward) {
+				p[0] = a * (x0 + x1 + x2);
+				p[WH] = b * (x0 - x2);
+				p[2 * (int64_t)WH] = c * (0.25f * x0 - 0.5f * x1 + 0.25f * x2);
+			} else {
+				p[0] = a * x0 + b * x1 + c * 0.5f * x2;
+				p[WH] = a * x0 - c * x2;
+				p[2 * (int64_t)WH] = a * x0 - b * x1 + c * 0.5f * x2;
+			}
+		} else if (C == 4) {
+			const float Gr = p[0], R = p[WH], B = p[2 * (int64_t)WH], Gb = p[3 * (int64_t)WH];
+			if (forward) {
+				p[0] = a * (Gr + R + B + Gb);
+				p[WH] = b * (R - B);
+				p[2 * (int64_t)WH] = a * (-Gr + R + B - Gb);
+				p[3 * (int64_t)WH] = b * (-Gr + Gb);
+			} else {
+				p[0] = a * Gr - a * B - b * Gb;
+				p[WH] = a * Gr + b * R + a * B;
+				p[2 * (int64_t)WH] = a * Gr - b * R + a * B;
+				p[3 * (int64_t)WH] = a * Gr - a * B + b * Gb;
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Block matching: one workgroup per reference patch.
+// estimateSimilarPatchesStep1 (nldct.cpp:1143-1348) / Step2 (:1607-1789).
+//
+// Candidate order s follows the reference's scan (frame schedule pt, pt+1..,
+// pt-1.., then raster qy, qx).  Selection keeps the kk = largest power of
+// two <= min(np, #candidates) smallest distances in the total order
+// (distance, reference patch first, s), which equals the reference's
+// std::partial_sort order whenever distances are distinct, then applies the
+// power-of-two tau cut (nldct.cpp:1285-1289).
+// ---------------------------------------------------------------------------
+struct BmFrame { int qt, rx0, ry0, nx, ny, base; };
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+	const int lane = threadIdx.x & 63;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) {
+		const int o = __shfl_up(v, d, 64);
+		if (lane >= d) v += o;
+	}
+	return v;
+}
+
+__global__ __launch_bounds__(BM_THREADS)
+void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
+               const float *__restrict__ bflow, const uint32_t *__restrict__ gpos,
+               const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
+               StepConst sc, int step, int maxCand, uint32_t *__restrict__ out_idx,
+               uint32_t *__restrict__ out_nsim, int G)
+{
+	extern __shared__ unsigned char smem[];
+	uint32_t *skey = (uint32_t *)smem;               // maxCand
+	int *srank = (int *)(skey + maxCand);            // maxCand
+	int *stie = srank + maxCand;                     // maxCand
+	float *sref = (float *)(stie + maxCand);         // C*P
+
+	__shared__ BmFrame fr[BM_MAX_FRAMES];
+	__shared__ int s_cx[BM_MAX_FRAMES], s_cy[BM_MAX_FRAMES];
+	__shared__ int s_nframes, s_nsrch, s_rt0;
+	__shared__ uint32_t hist[256];
+	__shared__ uint32_t s_bucket, s_below;
+	__shared__ uint32_t sel_key[64];
+	__shared__ int sel_rank[64];
+	__shared__ uint32_t out_key[64];
+	__shared__ int out_rank[64];
+	__shared__ int s_na, s_nb, s_nfin;
+
+	const int g = blockIdx.x;
+	if (g >= G) return;
+	const int tid = threadIdx.x;
+	const uint32_t pos = gpos[g];
+	int gx, gy, gt;
+	pos_decode(pos, sc.W, sc.WH, gx, gy, gt);
+	const TileBox tb = tiles[gtile[g]];
+	const int Wt = tb.ex - tb.ox, Ht = tb.ey - tb.oy;
+	const int px = gx - tb.ox, py = gy - tb.oy, pt = gt;
+	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
+	uint32_t *oidx = out_idx + (size_t)g * sc.npmax;
+
+	if (sc.npmax <= 1) {
+		if (tid == 0) { oidx[0] = pos; out_nsim[g] = 1; }
+		return;
+	}
+
+	// Temporal range (nldct.cpp:1173-1186) and flow trajectory
+	// (nldct.cpp:1212-1236): thread 0 walks forward, thread 1 backward.
+	const int F = sc.F;
+	const int shift_t = min(0, pt - sc.wtb) + max(0, pt + sc.wtf - F + sPt);
+	const int rt0 = max(0, pt - sc.wtb - shift_t);
+	const int rt1 = min(F - sPt, pt + sc.wtf - shift_t);
+	if (tid < 2) {
+		const int dir = tid == 0 ? 1 : -1;
+		const float *fl = tid == 0 ? fflow : bflow;
+		int cx = px, cy = py, ct = pt;
+		if (tid == 0) { s_cx[pt - rt0] = px; s_cy[pt - rt0] = py; s_rt0 = rt0; }
+		for (int qt = pt + dir; dir > 0 ? qt <= rt1 : qt >= rt0; qt += dir) {
+			float fx = 0.f, fy = 0.f;
+			if (sc.use_flow) {
+				const size_t base = (size_t)ct * 2 * sc.WH + (size_t)(cy + tb.oy) * sc.W + (cx + tb.ox);
+				fx = fl[base];
+				fy = fl[base + sc.WH];
+			}
+			const float cx_f = (float)cx + fx;
+			const float cy_f = (float)cy + fy;
+			cx = (int)fmax_ref(0.f, fmin_ref((float)Wt - 1.f, roundf(cx_f)));
+			cy = (int)fmax_ref(0.f, fmin_ref((float)Ht - 1.f, roundf(cy_f)));
+			ct = qt;
+			s_cx[qt - rt0] = cx;
+			s_cy[qt - rt0] = cy;
+		}
+	}
+	// Reference patch (step 1: channel 0 of the noisy video; step 2: all
+	// channels of the basic video).
+	const int nref = step == 1 ? P : sc.C * P;
+	for (int k = tid; k < nref; k += BM_THREADS) {
+		const int c = k / P, i = k - c * P;
+		const int ht = i / (sPx * sPx), r = i - ht * sPx * sPx;
+		const int hy = r / sPx, hx = r - hy * sPx;
+		sref[k] = img[video_addr(gx + hx, gy + hy, gt + ht, c, sc)];
+	}
+	__syncthreads();
+	if (tid == 0) {
+		const int sWx = sc.sWx, half = (sWx - 1) / 2;
+		int nf = 0, base = 0;
+		for (int ii = 0; ii < rt1 - rt0 + 1; ii++) {
+			const int qt = ii == 0 ? pt : (ii <= rt1 - pt ? pt + ii : pt - (ii - (rt1 - pt)));
+			const int dt = qt - rt0;
+			const int cx = s_cx[dt], cy = s_cy[dt];
+			int shift_x = min(0, cx - half);
+			int shift_y = min(0, cy - half);
+			shift_x += max(0, cx + half - Wt + sPx);
+			shift_y += max(0, cy + half - Ht + sPx);
+			const int rx0 = max(0, cx - half - shift_x);
+			const int ry0 = max(0, cy - half - shift_y);
+			const int rx1 = min(Wt - sPx, cx + half - shift_x);
+			const int ry1 = min(Ht - sPx, cy + half - shift_y);
+			BmFrame f;
+			f.qt = qt; f.rx0 = rx0; f.ry0 = ry0;
+			f.nx = max(0, rx1 - rx0 + 1); f.ny = max(0, ry1 - ry0 + 1);
+			f.base = base;
+			base += f.nx * f.ny;
+			fr[nf++] = f;
+		}
+		s_nframes = nf;
+		s_nsrch = base;
+		s_na = 0; s_nb = 0;
+	}
+	__syncthreads();
+	const int nsrch = s_nsrch, nframes = s_nframes;
+
+	// Distances.
+	for (int s = tid; s < nsrch; s += BM_THREADS) {
+		int ii = 0;
+		while (ii + 1 < nframes && fr[ii + 1].base <= s) ii++;
+		const BmFrame f = fr[ii];
+		const int o = s - f.base;
+		const int qy = f.ry0 + o / f.nx, qx = f.rx0 + o % f.nx, qt = f.qt;
+		const int ax = qx + tb.ox, ay = qy + tb.oy;
+		float d = 0.f;
+		if (step == 1) {
+			int k = 0;
+			for (int ht = 0; ht < sPt; ht++)
+				for (int hy = 0; hy < sPx; hy++) {
+					const float *row = img + video_addr(ax, ay + hy, qt + ht, 0, sc);
+					for (int hx = 0; hx < sPx; hx++, k++) {
+						const float dif = sref[k] - row[hx];
+						d = d + dif * dif;
+					}
+				}
+		} else {
+			int k = 0;
+			for (int c = 0; c < sc.C; c++)
+				for (int ht = 0; ht < sPt; ht++)
+					for (int hy = 0; hy < sPx; hy++) {
+						const float *row = img + video_addr(ax, ay + hy, qt + ht, c, sc);
+						for (int hx = 0; hx < sPx; hx++, k++) {
+							const float dif = sref[k] - row[hx];
+							d = d + dif * dif;
+						}
+					}
+		}
+		skey[s] = __float_as_uint(d);
+		srank[s] = (qx == px && qy == py && qt == pt) ? -1 : s;
+	}
+	__syncthreads();
+
+	const int nSimP = min(sc.npmax, nsrch);
+	int kk = 1;
+	while (kk * 2 <= nSimP) kk *= 2;
+
+	// Radix select of the kk-th smallest key, 8 bits per pass.
+	uint32_t prefix = 0, pmask = 0;
+	uint32_t remaining = (uint32_t)kk;
+	for (int pass = 0; pass < 4; pass++) {
+		const int shift = 24 - 8 * pass;
+		hist[tid] = 0;
+		__syncthreads();
+		for (int s = tid; s < nsrch; s += BM_THREADS) {
+			const uint32_t k = skey[s];
+			if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+		}
+		__syncthreads();
+		if (tid < 64) {
+			const uint32_t c0 = hist[4 * tid], c1 = hist[4 * tid + 1], c2 = hist[4 * tid + 2], c3 = hist[4 * tid + 3];
+			const int local = (int)(c0 + c1 + c2 + c3);
+			const int incl = wave_incl_scan(local);
+			const int excl = incl - local;
+			if ((uint32_t)excl < remaining && remaining <= (uint32_t)incl) {
+				uint32_t cum = (uint32_t)excl;
+				const uint32_t cs[4] = {c0, c1, c2, c3};
+				for (int b = 0; b < 4; b++) {
+					if (cum + cs[b] >= remaining) { s_bucket = 4 * tid + b; s_below = cum; break; }
+					cum += cs[b];
+				}
+			}
+		}
+		__syncthreads();
+		prefix |= s_bucket << shift;
+		pmask |= 255u << shift;
+		remaining -= s_below;
+		__syncthreads();
+	}
+	const uint32_t T = prefix;
+	const int na_expected = kk - (int)remaining;
+
+	for (int s = tid; s < nsrch; s += BM_THREADS) {
+		const uint32_t k = skey[s];
+		if (k < T) {
+			const int slot = atomicAdd(&s_na, 1);
+			sel_key[slot] = k;
+			sel_rank[slot] = srank[s];
+		} else if (k == T) {
+			const int slot = atomicAdd(&s_nb, 1);
+			stie[slot] = srank[s];
+		}
+	}
+	__syncthreads();
+	const int nb = s_nb;
+	for (int j = tid; j < nb; j += BM_THREADS) {
+		const int r = stie[j];
+		int c = 0;
+		for (int i = 0; i < nb; i++) c += stie[i] < r;
+		if (c < (int)remaining) {
+			sel_key[na_expected + c] = T;
+			sel_rank[na_expected + c] = r;
+		}
+	}
+	__syncthreads();
+	if (tid < kk) {
+		const uint32_t k = sel_key[tid];
+		const int r = sel_rank[tid];
+		int p = 0;
+		for (int j = 0; j < kk; j++) {
+			const uint32_t kj = sel_key[j];
+			p += (kj < k) || (kj == k && sel_rank[j] < r);
+		}
+		out_key[p] = k;
+		out_rank[p] = r;
+	}
+	__syncthreads();
+	if (tid == 0) {
+		int n = 1;
+		while (2 * n <= nSimP && __uint_as_float(out_key[n]) < sc.tau2) n *= 2;
+		s_nfin = n;
+		out_nsim[g] = n;
+	}
+	__syncthreads();
+	const int nfin = s_nfin;
+	if (tid < nfin) {
+		const int r = out_rank[tid];
+		uint32_t q;
+		if (r < 0) q = pos;
+		else {
+			int ii = 0;
+			while (ii + 1 < nframes && fr[ii + 1].base <= r) ii++;
+			const BmFrame f = fr[ii];
+			const int o = r - f.base;
+			const int qy = f.ry0 + o / f.nx, qx = f.rx0 + o % f.nx;
+			q = (uint32_t)f.qt * (uint32_t)sc.WH + (uint32_t)(qy + tb.oy) * (uint32_t)sc.W + (uint32_t)(qx + tb.ox);
+		}
+		oidx[tid] = q;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Group filtering, one workgroup per group.
+//   step 1: computeBayesEstimateStep1 (nldct.cpp:1962-2006) ->
+//           computeBayesEstimateStep1_vbm3d (:1799-1943)
+//   step 2: computeBayesEstimateStep2 (:2196-2242) -> _vbm3d (:2015-2174)
+// Each neighbour's filtered patch, already multiplied by its aggregation
+// weight and window (the factor computeAggregationStep{1,2} applies,
+// nldct.cpp:2296-2299, 2385-2393), is written to row rank_of[g*np+n] of est.
+// ---------------------------------------------------------------------------
+
+// Z[k*N+n] = sum_pix X[pix*N+n] * B[k*P+pix], sequential in pix.
+__device__ __forceinline__ void fwd_transform(const float *__restrict__ X, const float *__restrict__ B,
+                                              float *__restrict__ Z, int P, int N)
+{
+	for (int o = threadIdx.x; o < P * N; o += FILTER_THREADS) {
+		const int k = o / N, n = o - k * N;
+		const float *b = B + (size_t)k * P;
+		float acc = 0.f;
+		for (int p = 0; p < P; p++) acc = acc + X[p * N + n] * b[p];
+		Z[o] = acc;
+	}
+}
+
+// Xh[pix*N+n] = sum_k Z[k*N+n] * Bi[k*P+pix], sequential in k.
+__device__ __forceinline__ void inv_transform(const float *__restrict__ Z, const float *__restrict__ Bi,
+                                              float *__restrict__ Xh, int P, int N)
+{
+	for (int o = threadIdx.x; o < P * N; o += FILTER_THREADS) {
+		const int pix = o / N, n = o - pix * N;
+		float acc = 0.f;
+		for (int k = 0; k < P; k++) acc = acc + Z[k * N + n] * Bi[(size_t)k * P + pix];
+		Xh[o] = acc;
+	}
+}
+
+__device__ __forceinline__ void haar(float *Z, int P, int N, bool forward)
+{
+	const float isqrt2 = 0.70710678118654752440f; // (float)(1./sqrt(2.))
+	for (int it = 0, s = forward ? 2 : N; forward ? s <= N : s >= 2; it++, s = forward ? s * 2 : s / 2) {
+		const int h = s / 2;
+		const int nb = N / s;
+		for (int o = threadIdx.x; o < P * nb; o += FILTER_THREADS) {
+			const int i = o / nb, n = (o - i * nb) * s;
+			const float z0 = Z[n + i * N] * isqrt2;
+			const float z1 = Z[n + h + i * N] * isqrt2;
+			Z[n + i * N] = z0 + z1;
+			Z[n + h + i * N] = z0 - z1;
+		}
+		__syncthreads();
+	}
+}
+
+__device__ __forceinline__ void load_group(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                           int N, int c, const StepConst &sc, float *__restrict__ X)
+{
+	const int sPx = sc.sPx;
+	for (int o = threadIdx.x; o < sc.P * N; o += FILTER_THREADS) {
+		const int pix = o / N, n = o - pix * N;
+		const int ht = pix / (sPx * sPx), r = pix - ht * sPx * sPx;
+		const int hy = r / sPx, hx = r - hy * sPx;
+		int x, y, t;
+		pos_decode(gidx[n], sc.W, sc.WH, x, y, t);
+		X[o] = img[video_addr(x + hx, y + hy, t + ht, c, sc)];
+	}
+}
+
+// 1 when all N patches of the group are equal in every channel
+// (nldct.cpp:1986-1999, 2222-2235).
+__device__ bool group_all_equal(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                int N, const StepConst &sc)
+{
+	const int sPx = sc.sPx;
+	int differs = 0;
+	if (N > 1) {
+		for (int o = threadIdx.x; o < sc.C * sc.P && !differs; o += FILTER_THREADS) {
+			const int c = o / sc.P, pix = o - c * sc.P;
+			const int ht = pix / (sPx * sPx), r = pix - ht * sPx * sPx;
+			const int hy = r / sPx, hx = r - hy * sPx;
+			int x, y, t;
+			pos_decode(gidx[0], sc.W, sc.WH, x, y, t);
+			const float v = img[video_addr(x + hx, y + hy, t + ht, c, sc)];
+			for (int n = 1; n < N; n++) {
+				pos_decode(gidx[n], sc.W, sc.WH, x, y, t);
+				if (v != img[video_addr(x + hx, y + hy, t + ht, c, sc)]) { differs = 1; break; }
+			}
+		}
+	} else
+		differs = 1;
+	return !__syncthreads_or(differs);
+}
+
+// Writes the raw (unfiltered) group of an early-out group; the caller
+// replaces its weights later with the stale values the reference reuses.
+__device__ void write_raw_group(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                const uint32_t *__restrict__ rrow, int N, const StepConst &sc,
+                                float *__restrict__ est)
+{
+	const int sPx = sc.sPx, P = sc.P;
+	for (int o = threadIdx.x; o < sc.C * P * N; o += FILTER_THREADS) {
+		const int n = o / (sc.C * P), r = o - n * sc.C * P;
+		const int c = r / P, pix = r - c * P;
+		const int ht = pix / (sPx * sPx), q = pix - ht * sPx * sPx;
+		const int hy = q / sPx, hx = q - hy * sPx;
+		int x, y, t;
+		pos_decode(gidx[n], sc.W, sc.WH, x, y, t);
+		est[((size_t)rrow[n] * sc.C + c) * P + pix] = img[video_addr(x + hx, y + hy, t + ht, c, sc)];
+	}
+}
+
+__global__ __launch_bounds__(FILTER_THREADS)
+void filter1_kernel(const float *__restrict__ img, const uint32_t *__restrict__ idx,
+                    const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ rank_of,
+                    StepConst sc, int g0, int Gc,
+                    const float *__restrict__ basis, const float *__restrict__ binv,
+                    const float *__restrict__ win, float *__restrict__ est,
+                    float *__restrict__ entry_w, uint8_t *__restrict__ early,
+                    float *__restrict__ gweight, uint32_t *__restrict__ n_early)
+{
+	extern __shared__ float fsm[];
+	__shared__ uint32_t s_rrow[64];
+	__shared__ int s_nnz_i;
+	__shared__ float s_terms[256];
+	const int gl = blockIdx.x;
+	if (gl >= Gc) return;
+	const int g = g0 + gl;
+	const int N = (int)nsim[g];
+	const int P = sc.P, C = sc.C;
+	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	float *X = fsm;                  // C*P*N (filtered estimates of all channels)
+	float *Z = X + (size_t)C * P * N; // P*N
+	if (threadIdx.x < N) s_rrow[threadIdx.x] = rank_of[(size_t)g * sc.npmax + threadIdx.x];
+	if (threadIdx.x == 0) s_nnz_i = 0;
+	__syncthreads();
+
+	if (group_all_equal(img, gidx, N, sc)) {
+		write_raw_group(img, gidx, s_rrow, N, sc, est);
+		if (threadIdx.x == 0) { early[g] = 1; gweight[g] = 0.f; atomicAdd(n_early, 1u); }
+		return;
+	}
+
+	float nnz_f = 0.f; // orderInvariance: float sum as in the reference
+	for (int c = 0; c < C; c++) {
+		float *Xc = X + (size_t)c * P * N;
+		load_group(img, gidx, N, c, sc, Xc);
+		__syncthreads();
+		fwd_transform(Xc, basis, Z, P, N);
+		__syncthreads();
+		haar(Z, P, N, true);
+		if (!sc.order_inv) {
+			int cnt = 0;
+			for (int o = threadIdx.x; o < P * N; o += FILTER_THREADS) {
+				const float z = Z[o];
+				if (z * z > sc.beta_sigma2 || o == 0) cnt++;
+				else Z[o] = 0.f;
+			}
+			if (cnt) atomicAdd(&s_nnz_i, cnt);
+		} else {
+			for (int i = threadIdx.x; i < P; i += FILTER_THREADS) {
+				float *z = Z + i * N;
+				float var_i = 0.f;
+				for (int n = 1; n < N; ++n) var_i = var_i + z[n] * z[n];
+				var_i = var_i / (float)(N - 1);
+				const float var_0 = fmax_ref(z[0] * z[0] - sc.sigma2, 0.f);
+				const float w_0 = var_0 / (var_0 + sc.betaM_sigma2);
+				z[0] = z[0] * w_0;
+				var_i = fmax_ref(var_i - sc.sigma2, 0.f);
+				const float w_i = var_i / (var_i + sc.beta_sigma2);
+				for (int n = 1; n < N; ++n) z[n] = z[n] * w_i;
+				s_terms[i] = w_0 * w_0 + (float)(N - 1) * w_i * w_i;
+			}
+			__syncthreads();
+			if (threadIdx.x == 0)
+				for (int i = 0; i < P; i++) nnz_f = nnz_f + s_terms[i];
+		}
+		__syncthreads();
+		haar(Z, P, N, false);
+		inv_transform(Z, binv, Xc, P, N);
+		__syncthreads();
+	}
+	__shared__ float s_w;
+	if (threadIdx.x == 0) {
+		const float nnz = sc.order_inv ? nnz_f : (float)s_nnz_i;
+		s_w = 1 / sc.sigma2 / nnz;
+		gweight[g] = s_w;
+		early[g] = 0;
+	}
+	__syncthreads();
+	const float w = s_w;
+	for (int o = threadIdx.x; o < C * P * N; o += FILTER_THREADS) {
+		const int n = o / (C * P), r = o - n * C * P;
+		const int c = r / P, pix = r - c * P;
+		const float ww = w * win[pix];
+		est[((size_t)s_rrow[n] * C + c) * P + pix] = ww * X[((size_t)c * P + pix) * N + n];
+	}
+	for (int n = threadIdx.x; n < N; n += FILTER_THREADS) entry_w[s_rrow[n]] = w;
+}
+
+__global__ __launch_bounds__(FILTER_THREADS)
+void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ basic,
+                    const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
+                    const uint32_t *__restrict__ rank_of, StepConst sc, int g0, int Gc,
+                    const float *__restrict__ basis, const float *__restrict__ binv,
+                    const float *__restrict__ win, float *__restrict__ est,
+                    float *__restrict__ entry_w, uint8_t *__restrict__ early,
+                    float *__restrict__ gweight, uint32_t *__restrict__ n_early)
+{
+	extern __shared__ float fsm[];
+	__shared__ uint32_t s_rrow[64];
+	__shared__ float s_part[FILTER_THREADS];
+	__shared__ float s_terms[256];
+	const int gl = blockIdx.x;
+	if (gl >= Gc) return;
+	const int g = g0 + gl;
+	const int N = (int)nsim[g];
+	const int P = sc.P, C = sc.C;
+	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	float *X = fsm;                   // C*P*N noisy group, then estimates
+	float *Zn = X + (size_t)C * P * N; // P*N
+	float *Yb = Zn + (size_t)P * N;   // P*N (basic group, then its coefficients)
+	if (threadIdx.x < N) s_rrow[threadIdx.x] = rank_of[(size_t)g * sc.npmax + threadIdx.x];
+	__syncthreads();
+
+	if (group_all_equal(noisy, gidx, N, sc)) {
+		write_raw_group(noisy, gidx, s_rrow, N, sc, est);
+		if (threadIdx.x == 0) { early[g] = 1; gweight[g] = 0.f; atomicAdd(n_early, 1u); }
+		return;
+	}
+
+	float part = 0.f;
+	float var_oi = 0.f;
+	for (int c = 0; c < C; c++) {
+		float *Xc = X + (size_t)c * P * N;
+		load_group(noisy, gidx, N, c, sc, Xc);
+		load_group(basic, gidx, N, c, sc, Zn);
+		__syncthreads();
+		fwd_transform(Zn, basis, Yb, P, N);   // basic coefficients
+		__syncthreads();
+		fwd_transform(Xc, basis, Zn, P, N);   // noisy coefficients
+		__syncthreads();
+		haar(Zn, P, N, true);
+		haar(Yb, P, N, true);
+		if (!sc.order_inv) {
+			for (int o = threadIdx.x; o < P * N; o += FILTER_THREADS) {
+				const float y = Yb[o];
+				const float w = y * y / (y * y + sc.beta_sigma2);
+				Zn[o] = Zn[o] * w;
+				part = part + w * w;
+			}
+		} else {
+			for (int i = threadIdx.x; i < P; i += FILTER_THREADS) {
+				const float *y = Yb + i * N;
+				float *z = Zn + i * N;
+				const float var_0 = y[0] * y[0];
+				const float w_0 = var_0 / (var_0 + sc.betaM_sigma2);
+				z[0] = z[0] * w_0;
+				float var_i = 0.f;
+				for (int n = 1; n < N; ++n) var_i = var_i + y[n] * y[n];
+				var_i = var_i / (float)(N - 1);
+				const float w_i = var_i / (var_i + sc.beta_sigma2);
+				for (int n = 1; n < N; ++n) z[n] = z[n] * w_i;
+				s_terms[i] = w_0 * w_0 + (float)(N - 1) * w_i * w_i;
+			}
+			__syncthreads();
+			if (threadIdx.x == 0)
+				for (int i = 0; i < P; i++) var_oi = var_oi + s_terms[i];
+		}
+		__syncthreads();
+		haar(Zn, P, N, false);
+		inv_transform(Zn, binv, Xc, P, N);
+		__syncthreads();
+	}
+	// Fixed-shape tree reduction of the Wiener energy.
+	s_part[threadIdx.x] = part;
+	__syncthreads();
+	for (int s = FILTER_THREADS / 2; s > 0; s >>= 1) {
+		if (threadIdx.x < s) s_part[threadIdx.x] = s_part[threadIdx.x] + s_part[threadIdx.x + s];
+		__syncthreads();
+	}
+	__shared__ float s_w;
+	if (threadIdx.x == 0) {
+		const float variance = sc.order_inv ? var_oi : s_part[0];
+		s_w = (float)(1. / (double)sc.sigma2 / (double)variance);
+		gweight[g] = s_w;
+		early[g] = 0;
+	}
+	__syncthreads();
+	const float w = s_w;
+	for (int o = threadIdx.x; o < C * P * N; o += FILTER_THREADS) {
+		const int n = o / (C * P), r = o - n * C * P;
+		const int c = r / P, pix = r - c * P;
+		const float ww = w * win[pix];
+		est[((size_t)s_rrow[n] * C + c) * P + pix] = ww * X[((size_t)c * P + pix) * N + n];
+	}
+	for (int n = threadIdx.x; n < N; n += FILTER_THREADS) entry_w[s_rrow[n]] = w;
+}
+
+// ---------------------------------------------------------------------------
+// Aggregation support: per-position neighbour lists.
+// ---------------------------------------------------------------------------
+
+// keys[e] = position of neighbour n of group g, vals[e] = g*np + n, for the
+// groups [g0, g0+Gc) of a chunk; base = exclusive scan of nsim over the chunk.
+__global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
+                                    const uint32_t *__restrict__ base, int npmax, int g0, int Gc,
+                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+	const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (o >= (int64_t)Gc * npmax) return;
+	const int gl = (int)(o / npmax), n = (int)(o - (int64_t)gl * npmax);
+	const int g = g0 + gl;
+	if (n >= (int)nsim[g]) return;
+	const uint32_t e = base[gl] + n;
+	keys[e] = idx[(size_t)g * npmax + n];
+	vals[e] = (uint32_t)g * npmax + n;
+}
+
+// After the stable sort by position: list boundaries, inverse permutation
+// and per-row tile numbers.
+__global__ void mark_ranges_kernel(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ svals,
+                                   int64_t M, int npmax, const uint32_t *__restrict__ gtile,
+                                   uint32_t *__restrict__ lstart, uint32_t *__restrict__ lend,
+                                   uint32_t *__restrict__ rank_of, uint32_t *__restrict__ etile)
+{
+	const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (r >= M) return;
+	const uint32_t k = skeys[r];
+	if (r == 0 || skeys[r - 1] != k) lstart[k] = (uint32_t)r;
+	if (r == M - 1 || skeys[r + 1] != k) lend[k] = (uint32_t)(r + 1);
+	const uint32_t v = svals[r];
+	rank_of[v] = (uint32_t)r;
+	etile[r] = gtile[v / npmax];
+}
+
+// Clears the list boundaries touched by a chunk.
+__global__ void clear_ranges_kernel(const uint32_t *__restrict__ skeys, int64_t M,
+                                    uint32_t *__restrict__ lstart, uint32_t *__restrict__ lend)
+{
+	const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (r >= M) return;
+	const uint32_t k = skeys[r];
+	lstart[k] = 0;
+	lend[k] = 0;
+}
+
+// Early-out groups: multiply the raw patch rows by the stale weight w_s and
+// the window (nldct.cpp:2296-2299).  stale[g*np+n] holds w_s per slot.
+__global__ void fixup_early_kernel(const uint32_t *__restrict__ glist, int ng,
+                                   const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ rank_of,
+                                   const float *__restrict__ stale, const float *__restrict__ win,
+                                   int npmax, int C, int P, float *__restrict__ est,
+                                   float *__restrict__ entry_w)
+{
+	const int k = blockIdx.x;
+	if (k >= ng) return;
+	const uint32_t g = glist[k];
+	const int N = (int)nsim[g];
+	for (int o = threadIdx.x; o < N * C * P; o += blockDim.x) {
+		const int n = o / (C * P), r = o - n * C * P;
+		const int pix = r % P;
+		const uint32_t row = rank_of[(size_t)g * npmax + n];
+		const float ww = stale[(size_t)g * npmax + n] * win[pix];
+		est[(size_t)row * C * P + r] = ww * est[(size_t)row * C * P + r];
+		if (r == 0) entry_w[row] = stale[(size_t)g * npmax + n];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Aggregation: one thread per pixel of the chunk's bounding box; it sums, in
+// a fixed order, the contributions of the chunk's patches covering it that
+// belong to groups of the pixel's owner tile (the tile whose interior holds
+// it: only that tile's result is copied back by subBuildTight,
+// lib_videot.hpp:1128-1160).  num / wsum accumulate across chunks.
+// ---------------------------------------------------------------------------
+__global__ void gather_kernel(const uint32_t *__restrict__ lstart, const uint32_t *__restrict__ lend,
+                              const uint32_t *__restrict__ etile, const float *__restrict__ entry_w,
+                              const float *__restrict__ est, const float *__restrict__ win,
+                              StepConst sc, int bx0, int by0, int bt0, int bw, int bh, int bf,
+                              double *__restrict__ num, double *__restrict__ wsum)
+{
+	const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (o >= (int64_t)bw * bh * bf) return;
+	const int lx = (int)(o % bw);
+	const int ly = (int)((o / bw) % bh);
+	const int lt = (int)(o / ((int64_t)bw * bh));
+	const int x = bx0 + lx, y = by0 + ly, t = bt0 + lt;
+	const int owner = (y / sc.hTmp) * sc.nW + (x / sc.wTmp);
+	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P, C = sc.C;
+	double acc[4] = {0., 0., 0., 0.};
+	double wacc = 0.;
+	bool any = false;
+	for (int ht = 0; ht < sPt; ht++) {
+		const int qt = t - ht;
+		if (qt < 0) break;
+		for (int hy = 0; hy < sPx; hy++) {
+			const int qy = y - hy;
+			if (qy < 0) break;
+			for (int hx = 0; hx < sPx; hx++) {
+				const int qx = x - hx;
+				if (qx < 0) break;
+				const uint32_t key = (uint32_t)qt * sc.WH + (uint32_t)qy * sc.W + qx;
+				const uint32_t r0 = lstart[key], r1 = lend[key];
+				const int i = (ht * sPx + hy) * sPx + hx;
+				for (uint32_t r = r0; r < r1; r++) {
+					if ((int)etile[r] != owner) continue;
+					any = true;
+					wacc = wacc + (double)(entry_w[r] * win[i]);
+					const float *e = est + (size_t)r * C * P + i;
+					for (int c = 0; c < C; c++) acc[c] = acc[c] + (double)e[(size_t)c * P];
+				}
+			}
+		}
+	}
+	if (!any) return;
+	const size_t pix = (size_t)t * sc.WH + (size_t)y * sc.W + x;
+	wsum[pix] = wsum[pix] + wacc;
+	for (int c = 0; c < C; c++) {
+		const size_t a = (size_t)t * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x;
+		num[a] = num[a] + acc[c];
+	}
+}
+
+// computeWeightedAggregation, nldct.cpp:2436-2448.
+__global__ void normalize_kernel(const double *__restrict__ num, const double *__restrict__ wsum,
+                                 const float *__restrict__ noisy, StepConst sc,
+                                 float *__restrict__ out)
+{
+	const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	const int64_t n = (int64_t)sc.WHC * sc.F;
+	if (o >= n) return;
+	const int64_t t = o / sc.WHC, r = o - t * sc.WHC;
+	const int64_t pix = t * sc.WH + r % sc.WH;
+	const float w = (float)wsum[pix];
+	const float nm = (float)num[o];
+	out[o] = w > 0.f ? nm / w : nm / noisy[o];
+}
+
+} // namespace bm4dof

@@ -1,0 +1,675 @@
+// TV-L1 optical flow on gfx950: Zach-Pock-Bischof duality scheme as the
+// reference implements it (lib/tvl1flow/tvl1flow_lib.c, mask.c,
+// bicubic_interpolation.c, zoom.c).
+//
+// All frame pairs of a sequence are solved together: every kernel takes a
+// pair index in blockIdx.y and skips pairs that have converged.  Per-pixel
+// arithmetic follows the reference operation by operation (this file is
+// compiled with -ffp-contract=off; bicubic and Gaussian sums in double as in
+// the reference), so a pixel's update is bit-identical given identical
+// inputs.  The per-iteration mean squared update is reduced in double in a
+// fixed order; the reference sums it in float in pixel order, so the stopping
+// iteration can differ by one where the error lands next to epsilon^2.
+#include "ctx.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <stdexcept>
+
+namespace bm4dof {
+
+#define TV_MAX_ITERATIONS 300      // tvl1flow_lib.c:24
+#define TV_PRESMOOTHING_SIGMA 0.8  // tvl1flow_lib.c:25
+#define TV_GRAD_IS_ZERO 1E-10      // tvl1flow_lib.c:26
+#define TV_ZOOM_SIGMA_ZERO 0.6     // zoom.c:17
+#define TV_THREADS 256
+
+// --------------------------------------------------------------- bicubic
+
+__device__ __forceinline__ int nbc(int x, int n, bool &out) // bicubic_interpolation.c:25-39
+{
+	if (x < 0) { out = true; return 0; }
+	if (x >= n) { out = true; return n - 1; }
+	return x;
+}
+
+__device__ __forceinline__ double cubic_cell(double v0, double v1, double v2, double v3, double x)
+{ // bicubic_interpolation.c:102-112
+	return v1 + 0.5 * x * (v2 - v0 +
+		x * (2.0 * v0 - 5.0 * v1 + 4.0 * v2 - v3 +
+		x * (3.0 * (v1 - v2) + v3 - v0)));
+}
+
+// bicubic_interpolation_at (bicubic_interpolation.c:138-234), Neumann case;
+// the y-neighbour above uses the x sign, as in the reference.
+__device__ float bicubic_at(const float *__restrict__ in, float uu, float vv, int nx, int ny,
+                            bool border_out)
+{
+	const int sx = (uu < 0) ? -1 : 1;
+	const int sy = (vv < 0) ? -1 : 1;
+	bool out = false;
+	const int x = nbc((int)uu, nx, out);
+	const int y = nbc((int)vv, ny, out);
+	const int mx = nbc((int)uu - sx, nx, out);
+	const int my = nbc((int)vv - sx, ny, out);
+	const int dx = nbc((int)uu + sx, nx, out);
+	const int dy = nbc((int)vv + sy, ny, out);
+	const int ddx = nbc((int)uu + 2 * sx, nx, out);
+	const int ddy = nbc((int)vv + 2 * sy, ny, out);
+	if (out && border_out) return 0.0f;
+	const int cols[4] = {mx, x, dx, ddx};
+	const double fy = (double)(vv - y);
+	double v[4];
+#pragma unroll
+	for (int c = 0; c < 4; c++) {
+		const double a0 = in[cols[c] + nx * my];
+		const double a1 = in[cols[c] + nx * y];
+		const double a2 = in[cols[c] + nx * dy];
+		const double a3 = in[cols[c] + nx * ddy];
+		v[c] = cubic_cell(a0, a1, a2, a3, fy);
+	}
+	return (float)cubic_cell(v[0], v[1], v[2], v[3], (double)(uu - x));
+}
+
+// ------------------------------------------------------------ pyramid
+
+// min/max of I0 and I1 per pair (tvl1flow_lib.c:286-320): partials per block.
+__global__ void tv_minmax_kernel(const float *__restrict__ I, int size, const int *__restrict__ i0,
+                                 const int *__restrict__ i1, float *__restrict__ part)
+{
+	const int pr = blockIdx.y;
+	const float *a = I + (size_t)i0[pr] * size;
+	const float *b = I + (size_t)i1[pr] * size;
+	float mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
+		mn0 = fminf(mn0, a[i]); mx0 = fmaxf(mx0, a[i]);
+		mn1 = fminf(mn1, b[i]); mx1 = fmaxf(mx1, b[i]);
+	}
+	__shared__ float s[4][TV_THREADS];
+	s[0][threadIdx.x] = mn0; s[1][threadIdx.x] = mx0; s[2][threadIdx.x] = mn1; s[3][threadIdx.x] = mx1;
+	__syncthreads();
+	for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
+		if (threadIdx.x < st) {
+			s[0][threadIdx.x] = fminf(s[0][threadIdx.x], s[0][threadIdx.x + st]);
+			s[1][threadIdx.x] = fmaxf(s[1][threadIdx.x], s[1][threadIdx.x + st]);
+			s[2][threadIdx.x] = fminf(s[2][threadIdx.x], s[2][threadIdx.x + st]);
+			s[3][threadIdx.x] = fmaxf(s[3][threadIdx.x], s[3][threadIdx.x + st]);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x < 4) part[((size_t)pr * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// image_normalization (tvl1flow_lib.c:303-337) into the level-0 buffers.
+__global__ void tv_normalize_kernel(const float *__restrict__ I, int size, const int *__restrict__ i0,
+                                    const int *__restrict__ i1, const float *__restrict__ part,
+                                    int nparts, float *__restrict__ I0n, float *__restrict__ I1n)
+{
+	const int pr = blockIdx.y;
+	float mn0 = INFINITY, mx0 = -INFINITY, mn1 = INFINITY, mx1 = -INFINITY;
+	for (int k = 0; k < nparts; k++) {
+		const float *q = part + ((size_t)pr * nparts + k) * 4;
+		mn0 = fminf(mn0, q[0]); mx0 = fmaxf(mx0, q[1]); mn1 = fminf(mn1, q[2]); mx1 = fmaxf(mx1, q[3]);
+	}
+	const float max = (mx0 > mx1) ? mx0 : mx1;
+	const float min = (mn0 < mn1) ? mn0 : mn1;
+	const float den = max - min;
+	const float *a = I + (size_t)i0[pr] * size;
+	const float *b = I + (size_t)i1[pr] * size;
+	float *oa = I0n + (size_t)pr * size, *ob = I1n + (size_t)pr * size;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
+		if (den > 0) {
+			oa[i] = 255.0 * (a[i] - min) / den;
+			ob[i] = 255.0 * (b[i] - min) / den;
+		} else {
+			oa[i] = a[i];
+			ob[i] = b[i];
+		}
+	}
+}
+
+// Gaussian smoothing with reflecting boundaries (mask.c:217-333), one pass.
+// rows: convolve along x; else along y.  Taps in constant-size array B.
+__global__ void tv_gauss_kernel(const float *__restrict__ in, float *__restrict__ out, int nx, int ny,
+                                int stride_pair, const double *__restrict__ B, int size, int rows)
+{
+	const int pr = blockIdx.y;
+	const float *I = in + (size_t)pr * stride_pair;
+	float *O = out + (size_t)pr * stride_pair;
+	const int n = nx * ny;
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+		const int i = p / nx, j = p - i * nx;
+		// extended coordinate e = (rows ? j : i) + size; R[e] with reflection
+		const int len = rows ? nx : ny;
+		const int pos = rows ? j : i;
+		auto R = [&](int e) -> double {
+			int q;
+			if (e < size) q = size - e;                  // R[i] = I[size - i]
+			else if (e >= len + size) q = len - 1 - (e - len - size); // R[bd+i] = I[len-1-i]
+			else q = e - size;
+			return rows ? (double)I[i * nx + q] : (double)I[q * nx + j];
+		};
+		const int e = pos + size;
+		double sum = B[0] * R(e);
+		for (int t = 1; t < size; t++) sum += B[t] * (R(e - t) + R(e + t));
+		O[p] = (float)sum;
+	}
+}
+
+// zoom_out resampling (zoom.c:66-76) of the pre-smoothed image.
+__global__ void tv_zoom_out_kernel(const float *__restrict__ Is, int nx, int ny, int in_stride,
+                                   float *__restrict__ out, int nxx, int nyy, int out_stride,
+                                   float factor)
+{
+	const int pr = blockIdx.y;
+	const float *I = Is + (size_t)pr * in_stride;
+	float *O = out + (size_t)pr * out_stride;
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nxx * nyy; p += gridDim.x * blockDim.x) {
+		const int i1 = p / nxx, j1 = p - i1 * nxx;
+		const float i2 = (float)i1 / factor;
+		const float j2 = (float)j1 / factor;
+		O[p] = bicubic_at(I, j2, i2, nx, ny, false);
+	}
+}
+
+// zoom_in (zoom.c:87-111) followed by the 1/zfactor scaling
+// (tvl1flow_lib.c:426-435); `scale` is (float)1.0/zfactor.
+__global__ void tv_zoom_in_kernel(const float *__restrict__ in, int nx, int ny, int in_stride,
+                                  float *__restrict__ out, int nxx, int nyy, int out_stride,
+                                  float scale)
+{
+	const int pr = blockIdx.y;
+	const float *I = in + (size_t)pr * in_stride;
+	float *O = out + (size_t)pr * out_stride;
+	const float factorx = ((float)nxx / nx);
+	const float factory = ((float)nyy / ny);
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nxx * nyy; p += gridDim.x * blockDim.x) {
+		const int i1 = p / nxx, j1 = p - i1 * nxx;
+		const float i2 = (float)i1 / factory;
+		const float j2 = (float)j1 / factorx;
+		O[p] = bicubic_at(I, j2, i2, nx, ny, false) * scale;
+	}
+}
+
+// ------------------------------------------------------------ one scale
+
+// centered_gradient (mask.c:152-209): clamped central differences, 0.5*()
+// in double.
+__global__ void tv_cgrad_kernel(const float *__restrict__ I1, int nx, int ny, int stride,
+                                float *__restrict__ Ix, float *__restrict__ Iy)
+{
+	const int pr = blockIdx.y;
+	const float *in = I1 + (size_t)pr * stride;
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
+		const int i = p / nx, j = p - i * nx;
+		const int jl = j > 0 ? j - 1 : 0, jr = j < nx - 1 ? j + 1 : nx - 1;
+		const int iu = i > 0 ? i - 1 : 0, id = i < ny - 1 ? i + 1 : ny - 1;
+		Ix[(size_t)pr * stride + p] = 0.5 * (in[i * nx + jr] - in[i * nx + jl]);
+		Iy[(size_t)pr * stride + p] = 0.5 * (in[id * nx + j] - in[iu * nx + j]);
+	}
+}
+
+// Warping of I1, I1x, I1y by u and the constant part of rho
+// (tvl1flow_lib.c:142-161).
+__global__ void tv_warp_kernel(const float *__restrict__ I0, const float *__restrict__ I1,
+                               const float *__restrict__ I1x, const float *__restrict__ I1y,
+                               const float *__restrict__ u1, const float *__restrict__ u2,
+                               int nx, int ny, int stride, float *__restrict__ I1wx,
+                               float *__restrict__ I1wy, float *__restrict__ grad,
+                               float *__restrict__ rho_c)
+{
+	const int pr = blockIdx.y;
+	const size_t off = (size_t)pr * stride;
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
+		const int i = p / nx, j = p - i * nx;
+		const float uu = (float)(j + u1[off + p]);
+		const float vv = (float)(i + u2[off + p]);
+		const float w = bicubic_at(I1 + off, uu, vv, nx, ny, true);
+		const float wx = bicubic_at(I1x + off, uu, vv, nx, ny, true);
+		const float wy = bicubic_at(I1y + off, uu, vv, nx, ny, true);
+		const float Ix2 = wx * wx;
+		const float Iy2 = wy * wy;
+		I1wx[off + p] = wx;
+		I1wy[off + p] = wy;
+		grad[off + p] = (Ix2 + Iy2);
+		rho_c[off + p] = (w - wx * u1[off + p] - wy * u2[off + p] - I0[off + p]);
+	}
+}
+
+// divergence with backward differences at one pixel (mask.c:43-92).
+__device__ __forceinline__ float div_at(const float *v1, const float *v2, int i, int j, int nx, int ny)
+{
+	const int p = i * nx + j;
+	const bool top = i == 0, bot = i == ny - 1, left = j == 0, right = j == nx - 1;
+	if (!top && !bot && !left && !right) return (v1[p] - v1[p - 1]) + (v2[p] - v2[p - nx]);
+	if (top && !left && !right) return v1[p] - v1[p - 1] + v2[p];
+	if (bot && !left && !right) return v1[p] - v1[p - 1] - v2[p - nx];
+	if (left && !top && !bot) return v1[p] + v2[p] - v2[p - nx];
+	if (right && !top && !bot) return -v1[p - 1] + v2[p] - v2[p - nx];
+	if (top && left) return v1[p] + v2[p];
+	if (top && right) return -v1[p - 1] + v2[p];
+	if (bot && left) return v1[p] - v2[p - nx];
+	return -v1[p - 1] - v2[p - nx];
+}
+
+// One primal update: TH step, divergence, u update and the squared update
+// (tvl1flow_lib.c:165-225).  Per-block partial sums in double.
+__global__ void tv_u_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1wy,
+                            const float *__restrict__ grad, const float *__restrict__ rho_c,
+                            float *__restrict__ u1, float *__restrict__ u2,
+                            const float *__restrict__ p11, const float *__restrict__ p12,
+                            const float *__restrict__ p21, const float *__restrict__ p22,
+                            int nx, int ny, int stride, float l_t, float theta,
+                            const int *__restrict__ state, double *__restrict__ part)
+{
+	const int pr = blockIdx.y;
+	__shared__ double s[TV_THREADS];
+	double e = 0.0;
+	if (state[pr] < 0) {
+		const size_t off = (size_t)pr * stride;
+		for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
+			const int i = p / nx, j = p - i * nx;
+			const float wx = I1wx[off + p], wy = I1wy[off + p], g = grad[off + p];
+			const float u1k = u1[off + p], u2k = u2[off + p];
+			const float rho = rho_c[off + p] + (wx * u1k + wy * u2k);
+			float d1, d2;
+			if (rho < -l_t * g) { d1 = l_t * wx; d2 = l_t * wy; }
+			else if (rho > l_t * g) { d1 = -l_t * wx; d2 = -l_t * wy; }
+			else if (g < TV_GRAD_IS_ZERO) d1 = d2 = 0;
+			else { const float fi = -rho / g; d1 = fi * wx; d2 = fi * wy; }
+			const float v1 = u1k + d1, v2 = u2k + d2;
+			const float dp1 = div_at(p11 + off, p12 + off, i, j, nx, ny);
+			const float dp2 = div_at(p21 + off, p22 + off, i, j, nx, ny);
+			const float n1 = v1 + theta * dp1;
+			const float n2 = v2 + theta * dp2;
+			u1[off + p] = n1;
+			u2[off + p] = n2;
+			e += (double)((n1 - u1k) * (n1 - u1k) + (n2 - u2k) * (n2 - u2k));
+		}
+	}
+	s[threadIdx.x] = e;
+	__syncthreads();
+	for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
+		if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) part[(size_t)pr * gridDim.x + blockIdx.x] = s[0];
+}
+
+// Stopping test of iteration k (tvl1flow_lib.c:165, 225): state becomes k
+// when error <= epsilon^2 or k reaches MAX_ITERATIONS.
+__global__ void tv_decide_kernel(const double *__restrict__ part, int nblocks, int size, float eps2,
+                                 int k, int *__restrict__ state, int *__restrict__ n_active)
+{
+	const int pr = blockIdx.x;
+	if (threadIdx.x != 0 || state[pr] >= 0) return;
+	double sum = 0.0;
+	for (int b = 0; b < nblocks; b++) sum += part[(size_t)pr * nblocks + b];
+	const float error = (float)sum / (float)size;
+	if (!(error > eps2) || k >= TV_MAX_ITERATIONS) {
+		state[pr] = k;
+		atomicSub(n_active, 1);
+	}
+}
+
+// Dual update with the forward-difference gradient of the new u
+// (tvl1flow_lib.c:228-245, mask.c:101-144); runs for pairs that were active
+// at the start of iteration k.
+__global__ void tv_p_kernel(const float *__restrict__ u1, const float *__restrict__ u2,
+                            float *__restrict__ p11, float *__restrict__ p12,
+                            float *__restrict__ p21, float *__restrict__ p22, int nx, int ny,
+                            int stride, float taut, int k, const int *__restrict__ state)
+{
+	const int pr = blockIdx.y;
+	if (!(state[pr] < 0 || state[pr] == k)) return;
+	const size_t off = (size_t)pr * stride;
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
+		const int i = p / nx, j = p - i * nx;
+		const float a1 = u1[off + p], a2 = u2[off + p];
+		const float u1x = j < nx - 1 ? u1[off + p + 1] - a1 : 0.f;
+		const float u1y = i < ny - 1 ? u1[off + p + nx] - a1 : 0.f;
+		const float u2x = j < nx - 1 ? u2[off + p + 1] - a2 : 0.f;
+		const float u2y = i < ny - 1 ? u2[off + p + nx] - a2 : 0.f;
+		const float g1 = (float)sqrt((double)u1x * u1x + (double)u1y * u1y);
+		const float g2 = (float)sqrt((double)u2x * u2x + (double)u2y * u2y);
+		const float ng1 = 1.0 + taut * g1;
+		const float ng2 = 1.0 + taut * g2;
+		p11[off + p] = (p11[off + p] + taut * u1x) / ng1;
+		p12[off + p] = (p12[off + p] + taut * u1y) / ng1;
+		p21[off + p] = (p21[off + p] + taut * u2x) / ng2;
+		p22[off + p] = (p22[off + p] + taut * u2y) / ng2;
+	}
+}
+
+__global__ void tv_fill_kernel(float *__restrict__ a, size_t n, float v)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		a[i] = v;
+}
+
+__global__ void tv_fill_int_kernel(int *__restrict__ a, int n, int v)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
+// Writes pair results into the sequence flow layout [F][2][H][W]:
+// dst frame `frame` gets (u, v) of pair pr.
+__global__ void tv_store_kernel(const float *__restrict__ u1, const float *__restrict__ u2, int size,
+                                const int *__restrict__ dst_frame, float *__restrict__ flow)
+{
+	const int pr = blockIdx.y;
+	const int f = dst_frame[pr];
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < size; p += gridDim.x * blockDim.x) {
+		flow[(size_t)f * 2 * size + p] = u1[(size_t)pr * size + p];
+		flow[(size_t)f * 2 * size + size + p] = u2[(size_t)pr * size + p];
+	}
+}
+
+__global__ void tv_copy_frame_kernel(float *__restrict__ flow, int size, int from, int to)
+{
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < 2 * size; p += gridDim.x * blockDim.x)
+		flow[(size_t)to * 2 * size + p] = flow[(size_t)from * 2 * size + p];
+}
+
+} // namespace bm4dof
+
+// ====================================================================== host
+
+using namespace bm4dof;
+
+namespace {
+
+inline int tv_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TV_THREADS - 1) / TV_THREADS, 4096)); }
+
+// Normalised 1-D Gaussian taps (mask.c:227-249), computed on the host with
+// the same libm calls as the reference.
+int gauss_taps(double sigma, std::vector<double> &B)
+{
+	const double den = 2 * sigma * sigma;
+	const int size = (int)(5 * sigma) + 1;
+	B.assign(size, 0.0);
+	for (int i = 0; i < size; i++) B[i] = 1 / (sigma * sqrt(2.0 * 3.1415926)) * exp(-i * i / den);
+	double norm = 0;
+	for (int i = 0; i < size; i++) norm += B[i];
+	norm *= 2;
+	norm -= B[0];
+	for (int i = 0; i < size; i++) B[i] /= norm;
+	return size;
+}
+
+struct TVPlan {
+	int P = 0;                       // number of pairs
+	int nscales = 0;
+	std::vector<int> nx, ny;         // per scale
+	std::vector<size_t> off;         // per-scale offset (floats) in a pyramid of P images
+	size_t pyr = 0;                  // floats per pyramid
+};
+
+TVPlan tv_plan(int P, int nxx, int nyy, int nscales, float zfactor)
+{
+	TVPlan pl;
+	pl.P = P;
+	pl.nscales = nscales;
+	pl.nx.resize(nscales);
+	pl.ny.resize(nscales);
+	pl.off.resize(nscales);
+	pl.nx[0] = nxx;
+	pl.ny[0] = nyy;
+	for (int s = 1; s < nscales; s++) { // zoom_size, zoom.c:24-36
+		pl.nx[s] = (int)((float)pl.nx[s - 1] * zfactor + 0.5);
+		pl.ny[s] = (int)((float)pl.ny[s - 1] * zfactor + 0.5);
+	}
+	size_t o = 0;
+	for (int s = 0; s < nscales; s++) {
+		pl.off[s] = o;
+		o += (size_t)P * pl.nx[s] * pl.ny[s];
+	}
+	pl.pyr = o;
+	return pl;
+}
+
+// Gaussian smoothing of P images (in -> out, via tmp).
+void tv_gauss(bm4dof_ctx *ctx, const float *in, float *tmp, float *out, int nx, int ny, int P,
+              double sigma)
+{
+	std::vector<double> B;
+	const int size = gauss_taps(sigma, B);
+	if (size > nx) throw ArgError("GaussianSmooth: sigma too large");
+	if (size >= ny) throw ArgError("GaussianSmooth: sigma too large for the image height");
+	double *dB = ctx->tv_taps.as<double>(64);
+	if (size > 64) throw ArgError("GaussianSmooth: sigma too large");
+	HIPCHK(hipMemcpyAsync(dB, B.data(), size * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+	const int n = nx * ny;
+	hipLaunchKernelGGL(tv_gauss_kernel, dim3(tv_grid(n), P), dim3(TV_THREADS), 0, ctx->stream, in, tmp,
+	                   nx, ny, n, dB, size, 1);
+	hipLaunchKernelGGL(tv_gauss_kernel, dim3(tv_grid(n), P), dim3(TV_THREADS), 0, ctx->stream, tmp, out,
+	                   nx, ny, n, dB, size, 0);
+	HIPCHK(hipGetLastError());
+	// the taps buffer is reused by the next call: keep ordering on the stream
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+// Multiscale TV-L1 of P pairs.  frames: device, nfr frames of nxx*nyy;
+// pair pr uses frames i0[pr] -> i1[pr].  Result in u1/u2 (P*nxx*nyy each,
+// device).  iters (host, may be NULL): P*nscales*warps ints.
+void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *frames, int nxx,
+                   int nyy, const std::vector<int> &i0, const std::vector<int> &i1, float *u1out,
+                   float *u2out, int *iters)
+{
+	const int P = (int)i0.size();
+	if (P == 0) return;
+	if (prm.nscales < 1 || prm.warps < 0 || prm.fscale < 0 || !(prm.zfactor > 0 && prm.zfactor < 1))
+		throw ArgError("invalid TV-L1 parameters");
+	const int nscales = prm.nscales;
+	TVPlan pl = tv_plan(P, nxx, nyy, nscales, prm.zfactor);
+	for (int s = 0; s < nscales; s++)
+		if (pl.nx[s] < 3 || pl.ny[s] < 3)
+			throw ArgError("TV-L1 pyramid level " + std::to_string(s) + " is smaller than 3x3; lower nscales");
+	const size_t size0 = (size_t)nxx * nyy;
+	hipStream_t st = ctx->stream;
+
+	int *d_pairs = ctx->tv_pairs.as<int>(2 * P);
+	HIPCHK(hipMemcpyAsync(d_pairs, i0.data(), P * sizeof(int), hipMemcpyHostToDevice, st));
+	HIPCHK(hipMemcpyAsync(d_pairs + P, i1.data(), P * sizeof(int), hipMemcpyHostToDevice, st));
+	// pyramids: I0s, I1s, u1s, u2s
+	float *pyr = ctx->tv_pyr.as<float>(pl.pyr * 4);
+	float *I0s = pyr, *I1s = pyr + pl.pyr, *U1 = pyr + 2 * pl.pyr, *U2 = pyr + 3 * pl.pyr;
+	// working arrays at the finest size: tmp, tmp2, I1x, I1y, I1wx, I1wy, grad, rho_c, p11..p22
+	const size_t W = (size_t)P * size0;
+	float *work = ctx->tv_work.as<float>(W * 14);
+	float *tmp = work, *tmp2 = work + W, *I1x = work + 2 * W, *I1y = work + 3 * W,
+	      *I1wx = work + 4 * W, *I1wy = work + 5 * W, *grad = work + 6 * W, *rho_c = work + 7 * W,
+	      *p11 = work + 8 * W, *p12 = work + 9 * W, *p21 = work + 10 * W, *p22 = work + 11 * W;
+	const int mmb = 64;
+	float *d_mm = ctx->tv_mm.as<float>((size_t)P * mmb * 4);
+	const int nb_max = tv_grid((int64_t)size0);
+	double *d_part = ctx->tv_part.as<double>((size_t)P * nb_max);
+	int *d_state = ctx->tv_state.as<int>(P + 1);
+	int *d_nact = d_state + P;
+
+	// normalisation + presmoothing (tvl1flow_lib.c:381-386)
+	hipLaunchKernelGGL(tv_minmax_kernel, dim3(mmb, P), dim3(TV_THREADS), 0, st, frames, (int)size0,
+	                   d_pairs, d_pairs + P, d_mm);
+	hipLaunchKernelGGL(tv_normalize_kernel, dim3(tv_grid(size0), P), dim3(TV_THREADS), 0, st, frames,
+	                   (int)size0, d_pairs, d_pairs + P, d_mm, mmb, tmp2, tmp2 + W);
+	HIPCHK(hipGetLastError());
+	tv_gauss(ctx, tmp2, tmp, I0s + pl.off[0], nxx, nyy, P, TV_PRESMOOTHING_SIGMA);
+	tv_gauss(ctx, tmp2 + W, tmp, I1s + pl.off[0], nxx, nyy, P, TV_PRESMOOTHING_SIGMA);
+	// pyramid (tvl1flow_lib.c:389-403; zoom_out, zoom.c:43-79)
+	const float zsig = TV_ZOOM_SIGMA_ZERO * sqrt(1.0 / (prm.zfactor * prm.zfactor) - 1.0);
+	for (int s = 1; s < nscales; s++) {
+		const int nx = pl.nx[s - 1], ny = pl.ny[s - 1], n = nx * ny;
+		for (int im = 0; im < 2; im++) {
+			float *src = (im ? I1s : I0s) + pl.off[s - 1];
+			float *dst = (im ? I1s : I0s) + pl.off[s];
+			tv_gauss(ctx, src, tmp, tmp2, nx, ny, P, zsig);
+			hipLaunchKernelGGL(tv_zoom_out_kernel, dim3(tv_grid(pl.nx[s] * pl.ny[s]), P), dim3(TV_THREADS), 0,
+			                   st, tmp2, nx, ny, n, dst, pl.nx[s], pl.ny[s], pl.nx[s] * pl.ny[s], prm.zfactor);
+			HIPCHK(hipGetLastError());
+		}
+	}
+	{
+		const int s = nscales - 1;
+		const size_t n = (size_t)P * pl.nx[s] * pl.ny[s];
+		hipLaunchKernelGGL(tv_fill_kernel, dim3(tv_grid(n)), dim3(TV_THREADS), 0, st, U1 + pl.off[s], n, 0.f);
+		hipLaunchKernelGGL(tv_fill_kernel, dim3(tv_grid(n)), dim3(TV_THREADS), 0, st, U2 + pl.off[s], n, 0.f);
+	}
+	if (iters) std::fill(iters, iters + (size_t)P * nscales * prm.warps, -1);
+	const float l_t = prm.lambda * prm.theta;
+	const float taut = prm.tau / prm.theta;
+	const float eps2 = prm.epsilon * prm.epsilon;
+	const float scale = (float)1.0 / prm.zfactor;
+	std::vector<int> h_state(P);
+
+	for (int s = nscales - 1; s >= prm.fscale; s--) {
+		// Dual_TVL1_optic_flow, tvl1flow_lib.c:93-275
+		const int nx = pl.nx[s], ny = pl.ny[s], n = nx * ny;
+		float *u1 = U1 + pl.off[s], *u2 = U2 + pl.off[s];
+		const float *I0 = I0s + pl.off[s], *I1 = I1s + pl.off[s];
+		const int g = tv_grid(n);
+		hipLaunchKernelGGL(tv_cgrad_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1, nx, ny, n, I1x, I1y);
+		const size_t nP = (size_t)P * n;
+		for (float *q : {p11, p12, p21, p22})
+			hipLaunchKernelGGL(tv_fill_kernel, dim3(tv_grid(nP)), dim3(TV_THREADS), 0, st, q, nP, 0.f);
+		for (int w = 0; w < prm.warps; w++) {
+			hipLaunchKernelGGL(tv_warp_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I0, I1, I1x, I1y, u1, u2,
+			                   nx, ny, n, I1wx, I1wy, grad, rho_c);
+			hipLaunchKernelGGL(tv_fill_int_kernel, dim3(1), dim3(TV_THREADS), 0, st, d_state, P, -1);
+			hipLaunchKernelGGL(tv_fill_int_kernel, dim3(1), dim3(1), 0, st, d_nact, 1, P);
+			HIPCHK(hipGetLastError());
+			for (int k = 1; k <= TV_MAX_ITERATIONS; k++) {
+				hipLaunchKernelGGL(tv_u_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1wx, I1wy, grad, rho_c,
+				                   u1, u2, p11, p12, p21, p22, nx, ny, n, l_t, prm.theta, d_state, d_part);
+				hipLaunchKernelGGL(tv_decide_kernel, dim3(P), dim3(64), 0, st, d_part, g, n, eps2, k,
+				                   d_state, d_nact);
+				hipLaunchKernelGGL(tv_p_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, u1, u2, p11, p12, p21,
+				                   p22, nx, ny, n, taut, k, d_state);
+				if (k % 8 == 0 || k == TV_MAX_ITERATIONS) {
+					HIPCHK(hipGetLastError());
+					int nact = 0;
+					HIPCHK(hipMemcpyAsync(&nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
+					HIPCHK(hipStreamSynchronize(st));
+					if (nact == 0) break;
+				}
+			}
+			HIPCHK(hipMemcpyAsync(h_state.data(), d_state, P * sizeof(int), hipMemcpyDeviceToHost, st));
+			HIPCHK(hipStreamSynchronize(st));
+			if (iters)
+				for (int pr = 0; pr < P; pr++) iters[((size_t)pr * nscales + s) * prm.warps + w] = h_state[pr];
+			if (prm.verbose)
+				for (int pr = 0; pr < P; pr++)
+					fprintf(stderr, "Pair %d scale %d warping %d: iterations %d\n", pr, s, w, h_state[pr]);
+		}
+		if (!s) break;
+		hipLaunchKernelGGL(tv_zoom_in_kernel, dim3(tv_grid(pl.nx[s - 1] * pl.ny[s - 1]), P), dim3(TV_THREADS), 0, st,
+		                   u1, nx, ny, n, U1 + pl.off[s - 1], pl.nx[s - 1], pl.ny[s - 1], pl.nx[s - 1] * pl.ny[s - 1], scale);
+		hipLaunchKernelGGL(tv_zoom_in_kernel, dim3(tv_grid(pl.nx[s - 1] * pl.ny[s - 1]), P), dim3(TV_THREADS), 0, st,
+		                   u2, nx, ny, n, U2 + pl.off[s - 1], pl.nx[s - 1], pl.ny[s - 1], pl.nx[s - 1] * pl.ny[s - 1], scale);
+		HIPCHK(hipGetLastError());
+	}
+	for (int s = prm.fscale - 1; s >= 0; s--) { // tvl1flow_lib.c:438-455
+		if (!s) break;
+		const int n = pl.nx[s] * pl.ny[s], nn = pl.nx[s - 1] * pl.ny[s - 1];
+		hipLaunchKernelGGL(tv_zoom_in_kernel, dim3(tv_grid(nn), P), dim3(TV_THREADS), 0, st, U1 + pl.off[s],
+		                   pl.nx[s], pl.ny[s], n, U1 + pl.off[s - 1], pl.nx[s - 1], pl.ny[s - 1], nn, scale);
+		hipLaunchKernelGGL(tv_zoom_in_kernel, dim3(tv_grid(nn), P), dim3(TV_THREADS), 0, st, U2 + pl.off[s],
+		                   pl.nx[s], pl.ny[s], n, U2 + pl.off[s - 1], pl.nx[s - 1], pl.ny[s - 1], nn, scale);
+		HIPCHK(hipGetLastError());
+	}
+	HIPCHK(hipMemcpyAsync(u1out, U1 + pl.off[0], (size_t)P * size0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+	HIPCHK(hipMemcpyAsync(u2out, U2 + pl.off[0], (size_t)P * size0 * sizeof(float), hipMemcpyDeviceToDevice, st));
+	HIPCHK(hipStreamSynchronize(st));
+}
+
+bm4dof_ctx *g_tv_ctx = nullptr;
+
+} // namespace
+
+extern "C" {
+
+__attribute__((visibility("default")))
+int bm4dof_tvl1(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm, const float *I0, const float *I1,
+                float *u1, float *u2, int nx, int ny, int on_device, int *iters)
+{
+	if (!ctx || !prm || !I0 || !I1 || !u1 || !u2) { set_error(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		if (nx < 3 || ny < 3) throw ArgError("image smaller than 3x3");
+		const size_t n = (size_t)nx * ny;
+		float *fr = ctx->tv_frames.as<float>(4 * n);
+		float *du1 = fr + 2 * n, *du2 = fr + 3 * n;
+		const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+		HIPCHK(hipMemcpyAsync(fr, I0, n * 4, k, ctx->stream));
+		HIPCHK(hipMemcpyAsync(fr + n, I1, n * 4, k, ctx->stream));
+		tv_flow_pairs(ctx, *prm, fr, nx, ny, {0}, {1}, du1, du2, iters);
+		const hipMemcpyKind kb = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+		HIPCHK(hipMemcpyAsync(u1, du1, n * 4, kb, ctx->stream));
+		HIPCHK(hipMemcpyAsync(u2, du2, n * 4, kb, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_tvl1_sequence(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm, const float *frames, int nx,
+                         int ny, int F, float *fflow, float *bflow, int on_device)
+{
+	if (!ctx || !prm || !frames || !fflow || !bflow) { set_error(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		if (F < 1 || nx < 3 || ny < 3) throw ArgError("empty sequence or frames smaller than 3x3");
+		const size_t n = (size_t)nx * ny;
+		float *fr = ctx->tv_frames.as<float>(n * F + (size_t)4 * n * F);
+		float *ff = fr + n * F, *bf = ff + 2 * n * F;
+		hipStream_t st = ctx->stream;
+		HIPCHK(hipMemcpyAsync(fr, frames, n * F * 4, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemsetAsync(ff, 0, 4 * n * F * 4, st));
+		if (F > 1) {
+			// scripts/tvl1flow-seq.sh:31-61: bwd(i) = I_i -> I_{i-1}, fwd(i) = I_i -> I_{i+1}
+			std::vector<int> i0, i1, dst;
+			for (int i = 1; i < F; i++) { i0.push_back(i); i1.push_back(i - 1); dst.push_back(i); }
+			for (int i = 0; i < F - 1; i++) { i0.push_back(i); i1.push_back(i + 1); dst.push_back(i); }
+			const int P = (int)i0.size();
+			DevBuf ub;
+			float *u = ub.as<float>((size_t)2 * P * n);
+			tv_flow_pairs(ctx, *prm, fr, nx, ny, i0, i1, u, u + (size_t)P * n, nullptr);
+			int *d_dst = ctx->tv_pairs.as<int>(P);
+			HIPCHK(hipMemcpyAsync(d_dst, dst.data(), P * sizeof(int), hipMemcpyHostToDevice, st));
+			const int half = F - 1;
+			hipLaunchKernelGGL(tv_store_kernel, dim3(tv_grid(n), half), dim3(TV_THREADS), 0, st, u,
+			                   u + (size_t)P * n, (int)n, d_dst, bf);
+			hipLaunchKernelGGL(tv_store_kernel, dim3(tv_grid(n), half), dim3(TV_THREADS), 0, st, u + half * n,
+			                   u + (size_t)P * n + half * n, (int)n, d_dst + half, ff);
+			hipLaunchKernelGGL(tv_copy_frame_kernel, dim3(tv_grid(2 * n)), dim3(TV_THREADS), 0, st, bf, (int)n, 1, 0);
+			hipLaunchKernelGGL(tv_copy_frame_kernel, dim3(tv_grid(2 * n)), dim3(TV_THREADS), 0, st, ff, (int)n, F - 2, F - 1);
+			HIPCHK(hipGetLastError());
+			HIPCHK(hipStreamSynchronize(st));
+			ub.release();
+		}
+		const hipMemcpyKind kb = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+		HIPCHK(hipMemcpyAsync(fflow, ff, 2 * n * F * 4, kb, st));
+		HIPCHK(hipMemcpyAsync(bflow, bf, 2 * n * F * 4, kb, st));
+		HIPCHK(hipStreamSynchronize(st));
+	});
+}
+
+__attribute__((visibility("default")))
+void Dual_TVL1_optic_flow_multiscale(float *I0, float *I1, float *u1, float *u2, int nxx, int nyy,
+                                     float tau, float lambda, float theta, int nscales, int fscale,
+                                     float zfactor, int warps, float epsilon, bool verbose)
+{
+	if (!g_tv_ctx && bm4dof_ctx_create(&g_tv_ctx, 0) != BM4DOF_OK) {
+		fprintf(stderr, "Dual_TVL1_optic_flow_multiscale: %s\n", bm4dof_global_error());
+		abort();
+	}
+	bm4dof_tvl1_params prm = {tau, lambda, theta, nscales, fscale, zfactor, warps, epsilon, verbose ? 1 : 0};
+	if (bm4dof_tvl1(g_tv_ctx, &prm, I0, I1, u1, u2, nxx, nyy, 0, nullptr) != BM4DOF_OK) {
+		fprintf(stderr, "Dual_TVL1_optic_flow_multiscale: %s\n", bm4dof_last_error(g_tv_ctx));
+		abort();
+	}
+}
+
+} // extern "C"

@@ -1,0 +1,224 @@
+/*
+ * bm4dof.h -- C-ABI of the MI355X BM4D-OF / NL-DCT video denoiser and its
+ * TV-L1 optical-flow input.
+ *
+ * Every entry point takes plain pointers and sizes.  Videos use the
+ * reference's planar layout  data[t*w*h*c + ch*w*h + y*w + x]
+ * (src/utils/lib_videot.hpp:88-92); flow videos have c = 2 (u, v).
+ *
+ * Entry points that take `const float*` video arguments without a `_dev`
+ * suffix take HOST memory; the `_dev` variants take device memory already
+ * resident in HBM on the context's device.
+ *
+ * Errors: functions return 0 on success and a negative BM4DOF_E* code on
+ * failure; bm4dof_last_error() returns the message, which for the checks the
+ * reference performs is the reference's own std::runtime_error text.
+ */
+#ifndef BM4DOF_H_INCLUDED
+#define BM4DOF_H_INCLUDED
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+#define BM4DOF_BOOL bool
+extern "C" {
+#else
+#define BM4DOF_BOOL _Bool
+#endif
+
+/* enum ColorSpace {RGB, YUV}       src/core/nldct.h:33 */
+enum { BM4DOF_RGB = 0, BM4DOF_YUV = 1 };
+/* enum Transform {dct, bior1_5}    src/core/nldct.h:34 */
+enum { BM4DOF_DCT = 0, BM4DOF_BIOR1_5 = 1 };
+
+/* Error codes. */
+enum {
+	BM4DOF_OK         =  0,
+	BM4DOF_EINVAL     = -1, /* argument check failed (reference runtime_error) */
+	BM4DOF_EHIP       = -2, /* HIP runtime error                               */
+	BM4DOF_ENOMEM     = -3, /* device allocation failed                        */
+	BM4DOF_EUNSUPP    = -4  /* configuration outside what the kernels handle   */
+};
+
+/*
+ * Fixed-width POD mirror of `struct nlbParams` (src/core/nldct.h:55-82),
+ * field for field and in the same order.  bool fields are int32 (0/1), the
+ * two enums are int32 with the reference's values.
+ */
+typedef struct bm4dof_params {
+	float    sigma;
+	uint32_t sizePatch;
+	uint32_t sizePatchTime;
+	uint32_t nSimilarPatches;
+	uint32_t sizeSearchWindow;
+	uint32_t sizeSearchTimeRangeFwd;
+	uint32_t sizeSearchTimeRangeBwd;
+	uint32_t boundary;
+	uint32_t offSet;
+	uint32_t offSetTime;
+	float    beta;
+	float    betaMean;
+	float    tau;
+	int32_t  orderInvariance;
+	int32_t  isFirstStep;
+	int32_t  doPasteBoost;
+	int32_t  verbose;
+	int32_t  colorSpace;            /* BM4DOF_RGB / BM4DOF_YUV          */
+	int32_t  transform;             /* BM4DOF_DCT / BM4DOF_BIOR1_5      */
+	uint32_t sizeSearchWindowPred;  /* VBM3D predictive search (unused) */
+	uint32_t nSimilarPatchesPred;   /* VBM3D predictive search (unused) */
+	float    dsub;                  /* VBM3D predictive search (unused) */
+	int32_t  agg_window;
+} bm4dof_params;
+
+/* ---- parameter set-up: same semantics as the reference, bit for bit ---- */
+
+/* initializeNlbParameters  src/core/nldct.cpp:81-148 (step = 1 or 2) */
+void bm4dof_init_params(bm4dof_params *p, unsigned step, float sigma,
+                        unsigned width, unsigned height, unsigned frames,
+                        unsigned channels, int verbose);
+/* setSizeSearchWindow      src/core/nldct.cpp:160-168 */
+void bm4dof_set_search_window(bm4dof_params *p, unsigned sizeSearchWindow);
+/* setSizePatch             src/core/nldct.cpp:179-188 */
+void bm4dof_set_patch_size(bm4dof_params *p, unsigned frames, unsigned sizePatch);
+/* setNSimilarPatches       src/core/nldct.cpp:199-210 */
+void bm4dof_set_n_similar(bm4dof_params *p, unsigned nSimilarPatches);
+
+/* ---- denoiser context ---- */
+
+typedef struct bm4dof_ctx bm4dof_ctx;
+
+/* Create a context bound to HIP device `device`.  One context per host
+ * thread; calls on one context are not re-entrant (the reference's
+ * runNlBayes is single-caller, src/core/nldct.cpp:267). */
+int  bm4dof_ctx_create(bm4dof_ctx **out, int device);
+void bm4dof_ctx_destroy(bm4dof_ctx *ctx);
+/* Message of the last failed call on ctx (never NULL). */
+const char *bm4dof_last_error(const bm4dof_ctx *ctx);
+/* Message of the last failed call that had no context (ctx_create). */
+const char *bm4dof_global_error(void);
+
+/*
+ * Whole two-step denoiser.  Replaces
+ *   std::vector<float> VideoNLB::runNlBayes(noisy, fflow, bflow, basic,
+ *                                           final, prms1, prms2)
+ *   src/core/nldct.h:239-247, src/core/nldct.cpp:267-478.
+ *
+ * Step selection is the reference's: p1->sizePatch == 0 skips step 1 and
+ * then `basic` is an INPUT (DENO_ONLY), p2->sizePatch == 0 skips step 2.
+ * fflow/bflow may be NULL (no flow: zero motion), else w*h*f*2 floats each.
+ * `nparts` is the number of spatial tiles the reference would use
+ * (2 * omp_get_max_threads(), src/core/nldct.cpp:318-323); the output
+ * depends on it and parity with a CPU run needs the same value.
+ * groups_ratio (may be NULL) receives the two percentages the reference
+ * returns (src/core/nldct.cpp:403-404, 471-472).
+ * Host buffers.  `final` may be NULL when step 2 is skipped.
+ */
+int bm4dof_run(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+               const float *noisy, unsigned width, unsigned height,
+               unsigned frames, unsigned channels,
+               const float *fflow, const float *bflow,
+               float *basic, float *final, int nparts, float groups_ratio[2]);
+
+/* Same as bm4dof_run on device buffers already resident in HBM on the
+ * context's device.  Runs on the context's stream and returns when the
+ * result is complete. */
+int bm4dof_run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+                   const float *noisy_dev, unsigned width, unsigned height,
+                   unsigned frames, unsigned channels,
+                   const float *fflow_dev, const float *bflow_dev,
+                   float *basic_dev, float *final_dev, int nparts,
+                   float groups_ratio[2]);
+
+/*
+ * Block matching of one step only, for parity checks of the neighbour sets.
+ * Mirrors estimateSimilarPatchesStep1 (src/core/nldct.cpp:1143-1348) /
+ * estimateSimilarPatchesStep2 (src/core/nldct.cpp:1607-1789) applied at every
+ * reference position of every tile, in the reference's processing order.
+ *   step        1 or 2 (selects which image the distance is taken on)
+ *   image       step 1: noisy video (already in the working colour space);
+ *               step 2: basic video
+ * Output arrays are sized by bm4dof_bm_count(): out_index[g*np + n] holds the
+ * reference's patch index  t*w*h*c + y*w + x  (global coordinates) of the
+ * n-th neighbour of group g, out_nsim[g] its group size, out_ref[g] the
+ * reference position (same encoding) and out_tile[g] the tile number.
+ * Host buffers.
+ */
+int bm4dof_bm_count(bm4dof_ctx *ctx, const bm4dof_params *p, unsigned width,
+                    unsigned height, unsigned frames, unsigned channels,
+                    int nparts, int64_t *n_groups);
+int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image,
+              unsigned width, unsigned height, unsigned frames, unsigned channels,
+              const float *fflow, const float *bflow, int nparts,
+              uint32_t *out_index, uint32_t *out_nsim, uint32_t *out_ref,
+              uint32_t *out_tile);
+
+/* Per-stage timings of the last bm4dof_run / bm4dof_run_dev call, in ms,
+ * measured with HIP events on the context's stream.  `names` receives a
+ * static string naming each slot.  Returns the number of slots filled. */
+int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, int max_slots);
+
+/* ---- TV-L1 optical flow ---- */
+
+/*
+ * Drop-in for the reference's flow library entry point, same signature and
+ * argument meaning:
+ *   void Dual_TVL1_optic_flow_multiscale(float *I0, float *I1, float *u1,
+ *        float *u2, int nx, int ny, float tau, float lambda, float theta,
+ *        int nscales, int fscale, float zfactor, int warps, float epsilon,
+ *        bool verbose)   lib/tvl1flow/tvl1flow_lib.c:345-361
+ * Host buffers; the caller allocates u1, u2 (nx*ny floats each), as
+ * lib/tvl1flow/main.c:173-174 does.  Runs on device 0 through a process-wide
+ * context.  Like the reference it has no error return; failures abort with a
+ * message on stderr.
+ */
+void Dual_TVL1_optic_flow_multiscale(float *I0, float *I1, float *u1, float *u2,
+                                     int nx, int ny, float tau, float lambda,
+                                     float theta, int nscales, int fscale,
+                                     float zfactor, int warps, float epsilon,
+                                     BM4DOF_BOOL verbose);
+
+/* TV-L1 parameters, in the argument order of the tvl1flow command line
+ * (lib/tvl1flow/main.c:86-99). */
+typedef struct bm4dof_tvl1_params {
+	float tau;
+	float lambda;
+	float theta;
+	int   nscales;
+	int   fscale;
+	float zfactor;
+	int   warps;
+	float epsilon;
+	int   verbose;
+} bm4dof_tvl1_params;
+
+/*
+ * Flow of one frame pair with an explicit context (device memory or host
+ * memory, see `on_device`).  Same computation as
+ * Dual_TVL1_optic_flow_multiscale.  iters (may be NULL) receives the
+ * iteration count of every (scale, warp) solve, nscales*warps ints,
+ * scale-major from the finest scale; unused slots are -1.
+ */
+int bm4dof_tvl1(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm,
+                const float *I0, const float *I1, float *u1, float *u2,
+                int nx, int ny, int on_device, int *iters);
+
+/*
+ * Forward and backward flow of a whole sequence, replacing the per-pair
+ * process loop of scripts/tvl1flow-seq.sh:27-61 in-process:
+ *   fwd(i) = flow(I_i -> I_{i+1}),  bwd(i) = flow(I_i -> I_{i-1}),
+ *   bwd(0) = bwd(1),  fwd(F-1) = fwd(F-2).
+ * frames: F gray frames (w*h*F floats).  fflow/bflow: w*h*F*2 floats in the
+ * denoiser's flow layout (u plane, then v plane, per frame).  F == 1 gives
+ * zero flow.  All pairs of the sequence are solved together.
+ */
+int bm4dof_tvl1_sequence(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm,
+                         const float *frames, int nx, int ny, int nframes,
+                         float *fflow, float *bflow, int on_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BM4DOF_H_INCLUDED */

@@ -1,0 +1,190 @@
+"""GPU parity of the denoiser (libbm4dof.so through its C-ABI) against the
+CPU oracle, on seeded synthetic inputs the oracle finishes in seconds.
+
+Bars (the north star's): block-matching neighbour lists identical (inputs
+have distinct distances, so the GPU's total order and the reference's
+std::partial_sort order agree); basic and final estimates within 1e-3
+max-abs on the 0-255 scale; end-to-end PSNR within 0.01 dB.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import bm4dof_amd as A, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+
+
+def _video(w, h, f, seed, sigma=20.0, channels=1):
+    clean = synth.video(w, h, f, channels=channels, seed=seed)
+    return clean, synth.add_noise(clean, sigma, seed + 1000)
+
+
+def _smooth_flow(F, H, W, seed, amp=2.5):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H, 0:W]
+    fl = np.zeros((F, 2, H, W), np.float32)
+    for t in range(F):
+        for c in range(2):
+            a, b, ph = rng.uniform(-1, 1, 3)
+            fl[t, c] = amp * np.sin(2 * np.pi * (a * xx / W + b * yy / H) + ph)
+    return fl
+
+
+def _params(shape, step, sigma=20.0, **kw):
+    F = shape[0]
+    return (A.cli_params(step, sigma, shape, **kw), O.cli_params(step, sigma, F, **kw))
+
+
+VIDEO_CASES = [
+    # (w, h, f, nparts, flow)
+    (64, 48, 6, 2, False),
+    (72, 40, 8, 1, True),
+    (96, 64, 8, 4, True),
+    (50, 38, 7, 6, True),
+]
+
+
+@pytest.mark.parametrize("w,h,f,nparts,flow", VIDEO_CASES)
+def test_block_matching_neighbour_lists(ctx, w, h, f, nparts, flow):
+    clean, noisy = _video(w, h, f, seed=w + f)
+    ff = bf = None
+    if flow:
+        ff, bf = _smooth_flow(f, h, w, 1), _smooth_flow(f, h, w, 2)
+    a1, o1 = _params(noisy.shape, 1, px=4, pt=4, wx=11, np_=32)
+    gi, gn, gr, gt = ctx.bm(a1, noisy, ff, bf, nparts=nparts)
+    oi, on, orf, ot = O.bm(o1, noisy, ff, bf, nparts=nparts, tie_mode=1)
+    assert np.array_equal(gr, orf) and np.array_equal(gt, ot)
+    assert np.array_equal(gn, on)
+    assert np.array_equal(gi, oi)
+    # step 2 distances are taken on the basic image (all channels)
+    basic = (clean + np.random.default_rng(0).normal(0, 3, clean.shape)).astype(np.float32)
+    a2, o2 = _params(noisy.shape, 2, px=4, pt=4, wx=11, np_=32)
+    gi, gn, _, _ = ctx.bm(a2, basic, ff, bf, nparts=nparts)
+    oi, on, _, _ = O.bm(o2, basic, ff, bf, nparts=nparts, tie_mode=1)
+    assert np.array_equal(gn, on) and np.array_equal(gi, oi)
+
+
+@pytest.mark.parametrize("w,h,f,nparts,flow", VIDEO_CASES)
+def test_video_two_step_parity(ctx, w, h, f, nparts, flow):
+    clean, noisy = _video(w, h, f, seed=3 * w + f)
+    ff = bf = None
+    if flow:
+        ff, bf = _smooth_flow(f, h, w, 3), _smooth_flow(f, h, w, 4)
+    a1, o1 = _params(noisy.shape, 1, px=4, pt=4, wx=11, np_=32)
+    a2, o2 = _params(noisy.shape, 2, px=4, pt=4, wx=11, np_=32)
+    ob, of_, ogr = O.run(o1, o2, noisy, ff, bf, nparts=nparts, tie_mode=1)
+    gb, gf, ggr = ctx.run(a1, a2, noisy, ff, bf, nparts=nparts)
+    assert np.abs(gb - ob).max() <= TOL
+    assert ggr == pytest.approx(ogr, abs=0)
+    # step 2 alone from the CPU basic estimate (DENO_ONLY, -px1 0 -b)
+    a0 = A.cli_params(1, 20, noisy.shape, px=0)
+    _, gf2, _ = ctx.run(a0, a2, noisy, ff, bf, basic=ob, nparts=nparts)
+    assert np.abs(gf2 - of_).max() <= TOL
+    # end to end
+    dpsnr = O.psnr(gf, clean)[0] - O.psnr(of_, clean)[0]
+    assert abs(dpsnr) <= 0.01
+
+
+def test_image_config0_bior_and_kaiser(ctx):
+    """Single grayscale image, px=8 (bior1.5 + Kaiser window in step 1, DCT in
+    step 2), np=16, 21x21 search, beta1 = 3."""
+    clean = synth.image(96, 80, seed=100)
+    noisy = synth.add_noise(clean, 20, 200)
+    a1, o1 = _params(noisy.shape, 1, px=8, pt=1, wx=21, np_=16, beta=3.0)
+    a2, o2 = _params(noisy.shape, 2, px=8, pt=1, wx=21, np_=16)
+    ob, of_, _ = O.run(o1, o2, noisy, nparts=2, tie_mode=1)
+    gb, gf, _ = ctx.run(a1, a2, noisy, nparts=2)
+    assert np.abs(gb - ob).max() <= TOL
+    _, gf2, _ = ctx.run(A.cli_params(1, 20, noisy.shape, px=0), a2, noisy, basic=ob, nparts=2)
+    assert np.abs(gf2 - of_).max() <= TOL
+    assert O.psnr(gf, clean)[0] > O.psnr(noisy, clean)[0] + 5
+
+
+def test_color_image_parity(ctx):
+    clean = synth.video(48, 40, 1, channels=3, seed=21)
+    noisy = synth.add_noise(clean, 20, 22)
+    a1, o1 = _params(noisy.shape, 1, px=8, pt=1, wx=11, np_=16)
+    a2, o2 = _params(noisy.shape, 2, px=8, pt=1, wx=11, np_=16)
+    ob, of_, _ = O.run(o1, o2, noisy, nparts=2, tie_mode=1)
+    gb, gf, _ = ctx.run(a1, a2, noisy, nparts=2)
+    assert np.abs(gb - ob).max() <= TOL
+    _, gf2, _ = ctx.run(A.cli_params(1, 20, noisy.shape, px=0), a2, noisy, basic=ob, nparts=2)
+    assert np.abs(gf2 - of_).max() <= TOL
+
+
+def test_order_invariance_and_np1(ctx):
+    clean, noisy = _video(40, 32, 6, seed=77)
+    for kw in (dict(order_inv=True), dict(np_=1)):
+        a1, o1 = _params(noisy.shape, 1, px=4, pt=4, wx=7, **dict(dict(np_=16), **kw))
+        a2, o2 = _params(noisy.shape, 2, px=4, pt=4, wx=7, **dict(dict(np_=16), **kw))
+        ob, of_, _ = O.run(o1, o2, noisy, nparts=2, tie_mode=1)
+        gb, gf, _ = ctx.run(a1, a2, noisy, nparts=2)
+        assert np.abs(gb - ob).max() <= TOL, kw
+        _, gf2, _ = ctx.run(A.cli_params(1, 20, noisy.shape, px=0), a2, noisy, basic=ob, nparts=2)
+        assert np.abs(gf2 - of_).max() <= TOL, kw
+
+
+def test_short_video_and_constant(ctx):
+    # fewer frames than the temporal patch: no reference patches, all-zero
+    # output (nldct.cpp:537, 2446)
+    _, noisy = _video(32, 24, 3, seed=5)
+    a1, o1 = _params(noisy.shape, 1, px=4, pt=4, wx=7, np_=8)
+    a2, o2 = _params(noisy.shape, 2, px=4, pt=4, wx=7, np_=8)
+    gb, gf, ggr = ctx.run(a1, a2, noisy, nparts=2)
+    ob, of_, ogr = O.run(o1, o2, noisy, nparts=2)
+    assert np.array_equal(gb, ob) and np.array_equal(gf, of_) and not gb.any()
+    assert ggr == (0.0, 0.0)
+    # constant video: every group takes the equality early-out with the
+    # stale weights of its predecessor; the output equals the input
+    v = np.full((6, 1, 24, 32), 42.0, np.float32)
+    gb, gf, _ = ctx.run(a1, a2, v, nparts=2)
+    assert np.abs(gb - 42).max() < 1e-4 and np.abs(gf - 42).max() < 1e-4
+
+
+def test_partly_flat_video_stale_weights(ctx):
+    """A video whose left part is noise-free: groups of identical patches take
+    the early-out and reuse the previous group's weights (nldct.cpp:1998)."""
+    clean, noisy = _video(48, 32, 6, seed=31)
+    noisy[:, :, :, :20] = 60.0
+    a1, o1 = _params(noisy.shape, 1, px=4, pt=4, wx=7, np_=8)
+    a2, o2 = _params(noisy.shape, 2, px=4, pt=4, wx=7, np_=8)
+    ob, of_, _ = O.run(o1, o2, noisy, nparts=2, tie_mode=1)
+    gb, gf, _ = ctx.run(a1, a2, noisy, nparts=2)
+    assert np.abs(gb - ob).max() <= TOL
+    _, gf2, _ = ctx.run(A.cli_params(1, 20, noisy.shape, px=0), a2, noisy, basic=ob, nparts=2)
+    assert np.abs(gf2 - of_).max() <= TOL
+
+
+def test_errors_match_reference_messages(ctx):
+    _, noisy = _video(32, 24, 6, seed=5)
+    a1 = A.cli_params(1, 20, noisy.shape, px=0)
+    a2 = A.cli_params(2, 20, noisy.shape, px=0)
+    with pytest.raises(A.BM4DOFError, match="Both patch sizes are zero"):
+        ctx.run(a1, a2, noisy)
+    two = np.zeros((6, 2, 24, 32), np.float32)
+    b1 = A.cli_params(1, 20, two.shape, px=4, pt=4)
+    with pytest.raises(A.BM4DOFError, match="Wrong number of channels"):
+        ctx.run(b1, a2, two)
+    bior = A.cli_params(1, 20, (1, 1, 24, 32), px=8)
+    bior.sizePatch = 7
+    with pytest.raises(A.BM4DOFError, match="bior1_5"):
+        ctx.run(bior, a2, noisy[:1])
+
+
+def test_deterministic_and_full_size_properties(ctx):
+    """960x540x20, sigma 20 (the benchmark workload): two runs are bitwise
+    equal, the output is finite and the PSNR gain is large."""
+    clean = synth.video(960, 540, 20, seed=101)
+    noisy = synth.add_noise(clean, 20, 201)
+    a1 = A.cli_params(1, 20, noisy.shape, px=4, pt=4, wx=11, np_=32)
+    a2 = A.cli_params(2, 20, noisy.shape, px=4, pt=4, wx=11, np_=32)
+    b1, f1, gr = ctx.run(a1, a2, noisy, nparts=2)
+    b2, f2, _ = ctx.run(a1, a2, noisy, nparts=2)
+    assert np.array_equal(b1, b2) and np.array_equal(f1, f2)
+    assert np.isfinite(f1).all()
+    assert O.psnr(f1, clean)[0] > O.psnr(noisy, clean)[0] + 15
+    assert O.psnr(f1, clean)[0] > O.psnr(b1, clean)[0]
+    assert gr[0] > 9 and gr[1] > 21
