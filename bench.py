@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the plain two-step BM4D-OF denoiser on one MI355X.
+
+Workload: grayscale video 960x540x20 float32, sigma 20, seeded procedural
+scene (20 textured shapes under random affine motion over a smooth
+gradient), px=4, pt=4, np1=np2=32, 11x11 spatial search, temporal +-4 frames
+along TV-L1 flow (5 scales, 10 warps, lambda 0.15).  The command line it
+replaces is `bm4d-of -px1 4 -pt1 4 -px2 4 -pt2 4 -np1 32 -np2 32 -wx1 11
+-wx2 11` with flows from `tvl1flow ... 0.25 0.15 0.3 5 0 0.5 10 0.01`.
+
+One step = step 1 + step 2 (runNlBayes, src/core/nldct.cpp:267-478) over the
+whole video, inputs and flows already resident in HBM (the flows are
+computed once beforehand by the GPU TV-L1; its time is reported apart).
+`value` is Mpix/s = 960*540*20 / seconds per step.
+
+The roofline entry is for the kernel with the largest share of the step,
+from the library's own HIP events on its stream.  The CPU baseline times the
+C restatement of the reference (oracle/) on a bounded crop of the same
+video with OpenMP over tiles (nParts = 2 x threads, as the reference).
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+W, H, F, SIGMA = 960, 540, 20, 20.0
+METRIC = "denoised Mpix/s at 960x540x20 σ=20, plain vs learned; ΔPSNR vs CPU ref"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def _load(name, rel):
+    if name in sys.modules:
+        return sys.modules[name]
+    path = os.path.join(REPO, rel)
+    kw = {}
+    if rel.endswith("__init__.py"):
+        kw["submodule_search_locations"] = [os.path.dirname(path)]
+    spec = importlib.util.spec_from_file_location(name, path, **kw)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _psnr(a, b):
+    import numpy as np
+    d = a.astype(np.float64) - b.astype(np.float64)
+    return float(10 * np.log10(255.0 ** 2 / np.mean(d * d)))
+
+
+def _stage_flops(stats, P=64, C=1):
+    """Algorithmic FLOP of the dense patch transforms, per step (one
+    multiply + one add per basis term): step 1 = forward + inverse
+    (4 N P^2), step 2 = two forward + one inverse (6 N P^2), N = group rows."""
+    return {"s1_filter": 4.0 * stats["rows1"] * P * P * C,
+            "s2_filter": 6.0 * stats["rows2"] * P * P * C}
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of `kernel`, as committed from rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs scaled by a copy-kernel calibration
+    (profiles/traffic.txt: lines `kernel fetch_bytes write_bytes`)."""
+    path = os.path.join(REPO, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == kernel:
+            return float(f[1]) + float(f[2])
+    return None
+
+
+def cpu_baseline(noisy, ff, bf, crop, threads):
+    """Time the oracle (C restatement of nldct.cpp, scalar per thread) on a
+    crop of the workload; returns the bench's cpu_baseline object."""
+    import numpy as np
+    sys.path.insert(0, REPO)
+    import oracle as O
+    cw, ch = crop
+    x0, y0 = (W - cw) // 2, (H - ch) // 2
+    sl = (slice(None), slice(None), slice(y0, y0 + ch), slice(x0, x0 + cw))
+    n = np.ascontiguousarray(noisy[sl])
+    f1 = np.ascontiguousarray(ff[sl])
+    b1 = np.ascontiguousarray(bf[sl])
+    o1 = O.cli_params(1, SIGMA, F, px=4, pt=4, wx=11, np_=32)
+    o2 = O.cli_params(2, SIGMA, F, px=4, pt=4, wx=11, np_=32)
+    O._omp_threads(threads)
+    t0 = time.perf_counter()
+    O.run(o1, o2, n, f1, b1, nparts=2 * threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(cw * ch * F / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{cw}x{ch}x{F} centre crop of the same video and flows, both steps, "
+                      f"{threads} OpenMP threads, nParts={2 * threads}, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nparts", type=int, default=2,
+                    help="spatial tiles (the reference uses 2 x OpenMP threads)")
+    ap.add_argument("--cpu-crop", default="480x270", help="crop WxH of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the denoiser "
+              "has no multi-GPU path in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    A = _load("bm4dof_amd", os.path.join("bm4d-of_amd", "__init__.py"))
+    S = _load("bm4dof_amd.synth", os.path.join("bm4d-of_amd", "synth.py"))
+    A.lib()
+    print(f"[bench] workload {W}x{H}x{F} sigma {SIGMA}", file=sys.stderr, flush=True)
+
+    clean = S.video(W, H, F, seed=101)
+    noisy = S.add_noise(clean, SIGMA, 201)
+    dev = torch.device("cuda:0")
+    d_noisy = torch.from_numpy(noisy).to(dev)
+    d_ff = torch.empty((F, 2, H, W), dtype=torch.float32, device=dev)
+    d_bf = torch.empty_like(d_ff)
+    d_basic = torch.empty_like(d_noisy)
+    d_final = torch.empty_like(d_noisy)
+    torch.cuda.synchronize()
+    ctx = A.Context(0)
+
+    t0 = time.perf_counter()
+    ctx.tvl1_sequence_dev(d_noisy.data_ptr(), W, H, F, d_ff.data_ptr(), d_bf.data_ptr())
+    torch.cuda.synchronize()
+    tvl1_s = time.perf_counter() - t0
+    print(f"[bench] TV-L1 flows of {2 * (F - 1)} pairs: {tvl1_s:.3f} s", file=sys.stderr, flush=True)
+
+    p1 = A.cli_params(1, SIGMA, noisy.shape, px=4, pt=4, wx=11, np_=32)
+    p2 = A.cli_params(2, SIGMA, noisy.shape, px=4, pt=4, wx=11, np_=32)
+
+    def step():
+        ctx.run_dev(p1, p2, d_noisy.data_ptr(), noisy.shape, d_ff.data_ptr(), d_bf.data_ptr(),
+                    d_basic.data_ptr(), d_final.data_ptr(), nparts=args.nparts)
+
+    for i in range(args.warmup):
+        step()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+
+    stage_ms, launches = {}, {}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        for name, ms in ctx.timings():
+            stage_ms[name] = stage_ms.get(name, 0.0) + ms
+            launches[name] = launches.get(name, 0) + 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = ctx.stats()
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    ms_per_step = elapsed / args.steps * 1e3
+    value = W * H * F / (elapsed / args.steps) / 1e6
+    basic = d_basic.cpu().numpy()
+    final = d_final.cpu().numpy()
+
+    # dominant kernel among the transform/filter, matching and aggregation stages
+    kern = {"s1_bm": "bm_kernel", "s2_bm": "bm_kernel", "s1_filter": "filter1_kernel",
+            "s2_filter": "filter2_kernel", "s1_aggregate": "agg_exact_kernel",
+            "s2_aggregate": "agg_exact_kernel"}
+    dom = max((k for k in stage_ms if k in kern), key=lambda k: stage_ms[k])
+    flops = _stage_flops(stats)
+    avg_ms = stage_ms[dom] / launches[dom]
+    roofline = None
+    if dom in flops:
+        per_launch = flops[dom] / (launches[dom] / args.steps)
+        ach = per_launch / (avg_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                    "traffic": _traffic(kern[dom]), "kernel": kern[dom],
+                    "avg_launch_ms": round(avg_ms, 3),
+                    "flop_per_launch": per_launch,
+                    "flop_basis": "dense fp32 patch transforms, 6 N P^2 per step-2 group "
+                                  "(4 N P^2 step 1), N = group rows, P = 64"}
+    else:
+        roofline = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": None, "traffic": _traffic(kern[dom]), "kernel": kern[dom],
+                    "avg_launch_ms": round(avg_ms, 3)}
+
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural video, i.i.d. Gaussian noise)",
+        "config": {"workload": f"plain two-step BM4D-OF, gray video {W}x{H}x{F}, sigma 20, "
+                               "px 4, pt 4, np1=np2=32, wx 11, temporal +-4 along TV-L1 flow",
+                   "nparts": args.nparts, "path": "plain"},
+        "roofline": roofline,
+        "stages_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
+        "tvl1_sequence_s": round(tvl1_s, 3),
+        "e2e_mpix_s": round(W * H * F / (elapsed / args.steps + tvl1_s) / 1e6, 4),
+        "psnr_db": {"noisy": round(_psnr(noisy, clean), 3), "basic": round(_psnr(basic, clean), 3),
+                    "final": round(_psnr(final, clean), 3)},
+        "groups": {"step1": stats["groups1"], "step2": stats["groups2"]},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        cw, ch = (int(v) for v in args.cpu_crop.lower().split("x"))
+        thr = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        print(f"[bench] CPU baseline: oracle on {cw}x{ch}x{F}, {thr} threads", file=sys.stderr,
+              flush=True)
+        out["cpu_baseline"] = cpu_baseline(noisy, d_ff.cpu().numpy(), d_bf.cpu().numpy(),
+                                           (cw, ch), thr)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

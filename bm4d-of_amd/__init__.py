@@ -66,7 +66,7 @@ EXPORTS = [
     "bm4dof_init_params", "bm4dof_set_search_window", "bm4dof_set_patch_size",
     "bm4dof_set_n_similar", "bm4dof_ctx_create", "bm4dof_ctx_destroy", "bm4dof_last_error",
     "bm4dof_global_error", "bm4dof_run", "bm4dof_run_dev", "bm4dof_bm_count", "bm4dof_bm",
-    "bm4dof_last_timings", "Dual_TVL1_optic_flow_multiscale", "bm4dof_tvl1",
+    "bm4dof_last_timings", "bm4dof_last_stats", "Dual_TVL1_optic_flow_multiscale", "bm4dof_tvl1",
     "bm4dof_tvl1_sequence",
 ]
 
@@ -114,6 +114,7 @@ def lib():
     L.bm4dof_bm.argtypes = [vp, P, vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, vp, vp, C.c_int,
                             vp, vp, vp, vp]
     L.bm4dof_last_timings.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_char_p), C.c_int]
+    L.bm4dof_last_stats.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
     if not hasattr(L, "bm4dof_tvl1"):
         _LIB = L
         return L
@@ -226,6 +227,16 @@ class Context:
         n = self._L.bm4dof_last_timings(self.h, ms, names, 64)
         return [(names[i].decode(), ms[i]) for i in range(n)]
 
+    def stats(self):
+        """Statistics of the last run: dict with the ordered aggregation's
+        pixel blocks / banded blocks / block shape and the groups and group
+        rows of each step."""
+        v = (C.c_int64 * 9)()
+        n = self._L.bm4dof_last_stats(self.h, v, 9)
+        keys = ("agg_blocks", "agg_banded_blocks", "agg_bx", "agg_by", "agg_bt",
+                "groups1", "rows1", "groups2", "rows2")
+        return {keys[i]: int(v[i]) for i in range(n)}
+
     def bm(self, p, image, fflow=None, bflow=None, nparts=2):
         """Neighbour lists at every reference position (bm4dof_bm)."""
         image = np.ascontiguousarray(image, np.float32)
@@ -277,3 +288,11 @@ class Context:
         self._check(self._L.bm4dof_tvl1_sequence(self.h, C.byref(prm), frames.ctypes.data, W, H,
                                                  F, ff.ctypes.data, bf.ctypes.data, 0))
         return ff, bf
+
+    def tvl1_sequence_dev(self, frames_ptr, W, H, F, fflow_ptr, bflow_ptr, tau=0.25, lam=0.15,
+                          theta=0.3, nscales=5, fscale=0, zfactor=0.5, warps=10, eps=0.01):
+        """bm4dof_tvl1_sequence on device pointers (frames: F*H*W floats;
+        flows: F*2*H*W floats each)."""
+        prm = TVL1Params(tau, lam, theta, nscales, fscale, zfactor, warps, eps, 0)
+        self._check(self._L.bm4dof_tvl1_sequence(self.h, C.byref(prm), frames_ptr, W, H, F,
+                                                 fflow_ptr, bflow_ptr, 1))

@@ -30,6 +30,10 @@ struct StepConst {
 	int nW, wTmp, hTmp;   // tile grid: columns, interior width / height
 };
 
+// Pixel block of the ordered aggregation: RX x RY x RT pixels, frame blocks
+// counted from tb0.
+struct AggRegion { int RX, RY, RT, tb0; };
+
 // Decode a 3-index position t*WH + y*W + x.
 __host__ __device__ inline void pos_decode(uint32_t pos, int W, int WH, int &x, int &y, int &t)
 {

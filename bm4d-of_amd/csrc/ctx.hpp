@@ -68,6 +68,10 @@ struct bm4dof_ctx {
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
 	size_t est_budget = (size_t)24 << 30;
+	// ordered-aggregation statistics of the last run
+	uint64_t agg_blocks = 0, agg_overflow_blocks = 0;
+	int agg_region[3] = {0, 0, 0};
+	uint64_t step_groups[2] = {0, 0}, step_rows[2] = {0, 0};   // groups / group rows per step
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

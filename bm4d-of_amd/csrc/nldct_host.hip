@@ -20,25 +20,24 @@ __global__ void bm_kernel(const float *, const float *, const float *, const uin
                           const uint32_t *, const TileBox *, StepConst, int, int, uint32_t *,
                           uint32_t *, int);
 __global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
-                               StepConst, int, int, const float *, const float *, const float *,
-                               float *, float *, uint8_t *, float *, uint32_t *);
+                               const uint32_t *, StepConst, int, int, const float *, const float *,
+                               const float *, float *, float *, uint32_t *, uint8_t *, float *,
+                               uint32_t *);
 __global__ void filter2_kernel(const float *, const float *, const uint32_t *, const uint32_t *,
-                               const uint32_t *, StepConst, int, int, const float *,
-                               const float *, const float *, float *, float *, uint8_t *,
-                               float *, uint32_t *);
+                               const uint32_t *, const uint32_t *, StepConst, int, int,
+                               const float *, const float *, const float *, float *, float *,
+                               uint32_t *, uint8_t *, float *, uint32_t *);
 __global__ void emit_entries_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int,
                                     int, uint32_t *, uint32_t *);
-__global__ void mark_ranges_kernel(const uint32_t *, const uint32_t *, int64_t, int,
-                                   const uint32_t *, uint32_t *, uint32_t *, uint32_t *,
-                                   uint32_t *);
-__global__ void clear_ranges_kernel(const uint32_t *, int64_t, uint32_t *, uint32_t *);
-__global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, const uint32_t *,
-                                   const float *, const float *, int, int, int, float *,
-                                   float *);
-__global__ void gather_kernel(const uint32_t *, const uint32_t *, const uint32_t *,
-                              const float *, const float *, const float *, StepConst, int, int,
-                              int, int, int, int, double *, double *);
-__global__ void normalize_kernel(const double *, const double *, const float *, StepConst,
+__global__ void count_positions_kernel(const uint32_t *, int64_t, uint32_t *);
+__global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, const uint32_t *, int,
+                                   const float *, const float *, int, int, int, float *, float *);
+#define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
+	float *, uint32_t *
+template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
+__global__ void band_bounds_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
+__global__ void normalize_kernel(const float *, const float *, const float *, StepConst,
                                  float *);
 
 } // namespace bm4dof
@@ -337,7 +336,8 @@ void check_step_params(const bm4dof_params &p, unsigned C)
 struct StepGeom {
 	Tiling til;
 	std::vector<uint32_t> gpos, gtile;
-	std::vector<uint32_t> tile_first; // first group of each tile
+	std::vector<uint32_t> tile_first;       // first group of each tile
+	std::vector<uint32_t> frame_first;      // first group of every (tile, reference frame)
 };
 
 StepGeom make_geometry(const bm4dof_params &p, int W, int H, int F, int nparts)
@@ -350,12 +350,14 @@ StepGeom make_geometry(const bm4dof_params &p, int W, int H, int F, int nparts)
 		const TileBox &c = g.til.crop[n];
 		g.tile_first.push_back((uint32_t)g.gpos.size());
 		tile_grid(p, c.ex - c.ox, c.ey - c.oy, F, c.ox > 0, c.oy > 0, c.ex < W, c.ey < H, xs, ys, ts);
-		for (int t : ts)
+		for (int t : ts) {
+			if (!xs.empty() && !ys.empty()) g.frame_first.push_back((uint32_t)g.gpos.size());
 			for (int y : ys)
 				for (int x : xs) {
 					g.gpos.push_back((uint32_t)t * WH + (uint32_t)(y + c.oy) * (uint32_t)W + (uint32_t)(x + c.ox));
 					g.gtile.push_back((uint32_t)n);
 				}
+		}
 	}
 	return g;
 }
@@ -425,6 +427,50 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	HIPCHK(hipGetLastError());
 }
 
+// Adds a chunk's rows, in row order, into the running float sums.  Bands
+// (row ranges processed one after the other) start at the first group of
+// every reference frame of every tile inside the chunk.
+void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uint32_t> &band_groups,
+                     int g0, int64_t M, int t0c, int t1c, const uint32_t *d_base,
+                     const uint32_t *kstart, const uint32_t *svals, const uint32_t *epos,
+                     const uint32_t *etile, const float *ew, const float *est, const float *win,
+                     float *num, float *wsum, uint32_t *d_nsplit)
+{
+	const int nb = (int)band_groups.size() + 1;
+	uint32_t *d_bands = ctx->elist.as<uint32_t>(2 * (size_t)nb + 2);
+	uint32_t *d_gfirst = d_bands + nb + 1;
+	const uint32_t zero = 0, mend = (uint32_t)M;
+	HIPCHK(hipMemcpyAsync(d_bands, &zero, 4, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(hipMemcpyAsync(d_bands + nb, &mend, 4, hipMemcpyHostToDevice, ctx->stream));
+	if (nb > 1) {
+		HIPCHK(hipMemcpyAsync(d_gfirst, band_groups.data(), band_groups.size() * 4, hipMemcpyHostToDevice,
+		                      ctx->stream));
+		hipLaunchKernelGGL(band_bounds_kernel, dim3(nblk(nb - 1, 256)), dim3(256), 0, ctx->stream, d_base,
+		                   d_gfirst, nb - 1, g0, d_bands + 1);
+		HIPCHK(hipGetLastError());
+	}
+	HIPCHK(hipMemsetAsync(d_nsplit, 0, 4, ctx->stream));
+	const dim3 grid((sc.W + 7) / 8, (sc.H + 7) / 8, t1c - t0c);
+	if (sc.C == 1)
+		hipLaunchKernelGGL(agg_exact_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
+		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
+	else if (sc.C == 3)
+		hipLaunchKernelGGL(agg_exact_kernel<3>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
+		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
+	else
+		hipLaunchKernelGGL(agg_exact_kernel<4>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
+		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
+	HIPCHK(hipGetLastError());
+	uint32_t ns = 0;
+	HIPCHK(hipMemcpyAsync(&ns, d_nsplit, 4, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	ctx->agg_blocks += (uint64_t)grid.x * grid.y * grid.z;
+	ctx->agg_overflow_blocks += ns;
+	ctx->agg_region[0] = 8;
+	ctx->agg_region[1] = 8;
+	ctx->agg_region[2] = nb;
+}
+
 // One denoising step on device buffers in the working colour space.
 //   step1: noisy_w -> out_w (basic)
 //   step2: noisy_w, basic_w -> out_w (final)
@@ -461,31 +507,27 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 
 	uint32_t *d_idx = ctx->idx.as<uint32_t>((size_t)std::max(G, 1) * np);
 	uint32_t *d_nsim = ctx->nsim.as<uint32_t>(std::max(G, 1));
-	uint32_t *d_rank = ctx->rank_of.as<uint32_t>((size_t)std::max(G, 1) * np);
 	uint8_t *d_early = ctx->early.as<uint8_t>(std::max(G, 1));
 	float *d_gw = ctx->gweight.as<float>(std::max(G, 1));
 	uint32_t *d_cnt = ctx->counters.as<uint32_t>(16);
-	double *d_num = ctx->num.as<double>(N_all);
-	double *d_wsum = ctx->wsum.as<double>(WHF);
-	uint32_t *d_lstart = ctx->lstart.as<uint32_t>(WHF);
-	uint32_t *d_lend = ctx->lend.as<uint32_t>(WHF);
+	float *d_num = ctx->num.as<float>(N_all);
+	float *d_wsum = ctx->wsum.as<float>(WHF);
+	uint32_t *d_kstart = ctx->lstart.as<uint32_t>(WHF + 1);
+	uint32_t *d_pcnt = ctx->lend.as<uint32_t>(WHF + 1);
 	HIPCHK(hipMemsetAsync(d_cnt, 0, 64, ctx->stream));
-	HIPCHK(hipMemsetAsync(d_num, 0, N_all * 8, ctx->stream));
-	HIPCHK(hipMemsetAsync(d_wsum, 0, WHF * 8, ctx->stream));
-	HIPCHK(hipMemsetAsync(d_lstart, 0, WHF * 4, ctx->stream));
-	HIPCHK(hipMemsetAsync(d_lend, 0, WHF * 4, ctx->stream));
+	HIPCHK(hipMemsetAsync(d_num, 0, N_all * 4, ctx->stream));
+	HIPCHK(hipMemsetAsync(d_wsum, 0, WHF * 4, ctx->stream));
 	tm.mark(step1 ? "s1_setup" : "s2_setup");
 
 	run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, d_gpos, d_gtile, d_tiles, G,
 	       d_idx, d_nsim);
 	tm.mark(step1 ? "s1_bm" : "s2_bm");
 
-	// Chunks bounded by the estimate buffer budget.
+	// Chunks of consecutive groups bounded by the estimate buffer budget.
 	const size_t row_bytes = (size_t)C * sc.P * 4;
 	const size_t max_rows = std::max<size_t>(np, ctx->est_budget / row_bytes);
 	const int chunk_groups = (int)std::min<size_t>((size_t)std::max(G, 1), max_rows / np);
 	uint32_t n_early_seen = 0;
-	std::vector<float> slot_state; // host-side stale-weight resolution (rare path)
 
 	const size_t filt_lds = step1 ? ((size_t)C * sc.P * np + (size_t)sc.P * np) * 4
 	                              : ((size_t)C * sc.P * np + 2 * (size_t)sc.P * np) * 4;
@@ -493,7 +535,7 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 
 	for (int g0 = 0; g0 < G; g0 += chunk_groups) {
 		const int Gc = std::min(chunk_groups, G - g0);
-		// entry offsets of the chunk
+		// row offsets of the chunk's groups
 		uint32_t *d_base = ctx->base.as<uint32_t>(Gc + 1);
 		size_t tmp_bytes = 0;
 		HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_nsim + g0, d_base, Gc, ctx->stream));
@@ -504,8 +546,10 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 		HIPCHK(hipMemcpyAsync(&last_n, d_nsim + g0 + Gc - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
 		HIPCHK(hipStreamSynchronize(ctx->stream));
 		const int64_t M = (int64_t)last_base + last_n;
+		ctx->step_groups[step1 ? 0 : 1] += (uint64_t)Gc;
+		ctx->step_rows[step1 ? 0 : 1] += (uint64_t)M;
 
-		uint32_t *d_keys = ctx->keys.as<uint32_t>(M);
+		uint32_t *d_epos = ctx->keys.as<uint32_t>(M);
 		uint32_t *d_vals = ctx->vals.as<uint32_t>(M);
 		uint32_t *d_skeys = ctx->skeys.as<uint32_t>(M);
 		uint32_t *d_svals = ctx->svals.as<uint32_t>(M);
@@ -513,29 +557,36 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 		float *d_ew = ctx->entry_w.as<float>(M);
 		float *d_est = ctx->est.as<float>((size_t)M * C * sc.P);
 		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk((int64_t)Gc * np, 256)), dim3(256), 0,
-		                   ctx->stream, d_idx, d_nsim, d_base, np, g0, Gc, d_keys, d_vals);
+		                   ctx->stream, d_idx, d_nsim, d_base, np, g0, Gc, d_epos, d_vals);
 		HIPCHK(hipGetLastError());
+		// rows sorted by patch position, stable (row order kept within a position)
 		int end_bit = 1;
 		while (end_bit < 32 && (1ull << end_bit) < WHF) end_bit++;
 		size_t sort_bytes = 0;
-		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, d_keys, d_skeys, d_vals, d_svals,
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, d_epos, d_skeys, d_vals, d_svals,
 		                                          (int)M, 0, end_bit, ctx->stream));
 		d_tmp = ctx->sort_tmp.get(sort_bytes);
-		HIPCHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, sort_bytes, d_keys, d_skeys, d_vals, d_svals,
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, sort_bytes, d_epos, d_skeys, d_vals, d_svals,
 		                                          (int)M, 0, end_bit, ctx->stream));
-		hipLaunchKernelGGL(mark_ranges_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream,
-		                   d_skeys, d_svals, M, np, d_gtile, d_lstart, d_lend, d_rank, d_etile);
+		// kstart[k] = number of rows at positions < k
+		HIPCHK(hipMemsetAsync(d_pcnt, 0, (WHF + 1) * 4, ctx->stream));
+		hipLaunchKernelGGL(count_positions_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, d_epos, M,
+		                   d_pcnt);
 		HIPCHK(hipGetLastError());
+		tmp_bytes = 0;
+		HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
+		d_tmp = ctx->sort_tmp.get(tmp_bytes);
+		HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
 		tm.mark(step1 ? "s1_sort" : "s2_sort");
 
 		if (step1)
 			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
-			                   d_idx, d_nsim, d_rank, sc, g0, Gc, d_basis, d_binv, d_win, d_est,
-			                   d_ew, d_early, d_gw, d_cnt);
+			                   d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc, d_basis, d_binv, d_win, d_est,
+			                   d_ew, d_etile, d_early, d_gw, d_cnt);
 		else
 			hipLaunchKernelGGL(filter2_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
-			                   basic_w, d_idx, d_nsim, d_rank, sc, g0, Gc, d_basis, d_binv, d_win,
-			                   d_est, d_ew, d_early, d_gw, d_cnt);
+			                   basic_w, d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc, d_basis, d_binv,
+			                   d_win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
 		HIPCHK(hipGetLastError());
 		tm.mark(step1 ? "s1_filter" : "s2_filter");
 
@@ -574,27 +625,25 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 				HIPCHK(hipMemcpyAsync(d_gl, glist.data(), glist.size() * 4, hipMemcpyHostToDevice, ctx->stream));
 				HIPCHK(hipMemcpyAsync(d_st, stale.data(), stale.size() * 4, hipMemcpyHostToDevice, ctx->stream));
 				hipLaunchKernelGGL(fixup_early_kernel, dim3((unsigned)glist.size()), dim3(256), 0, ctx->stream,
-				                   d_gl, (int)glist.size(), d_nsim, d_rank, d_st, d_win, np, C, sc.P,
+				                   d_gl, (int)glist.size(), d_nsim, d_base, g0, d_st, d_win, np, C, sc.P,
 				                   d_est, d_ew);
 				HIPCHK(hipGetLastError());
 				HIPCHK(hipStreamSynchronize(ctx->stream));
 			}
 		}
 
-		// Gather over the frames this chunk touches.
+		// Ordered aggregation over the frames this chunk touches.
 		uint32_t kfirst = 0, klast = 0;
 		HIPCHK(hipMemcpyAsync(&kfirst, d_skeys, 4, hipMemcpyDeviceToHost, ctx->stream));
 		HIPCHK(hipMemcpyAsync(&klast, d_skeys + M - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
 		HIPCHK(hipStreamSynchronize(ctx->stream));
-		const int t0 = (int)(kfirst / (uint32_t)sc.WH);
-		const int t1 = std::min(F, (int)(klast / (uint32_t)sc.WH) + sc.sPt);
-		const int64_t npx = (int64_t)W * H * (t1 - t0);
-		hipLaunchKernelGGL(gather_kernel, dim3(nblk(npx, 256)), dim3(256), 0, ctx->stream, d_lstart,
-		                   d_lend, d_etile, d_ew, d_est, d_win, sc, 0, 0, t0, W, H, t1 - t0, d_num, d_wsum);
-		HIPCHK(hipGetLastError());
-		hipLaunchKernelGGL(clear_ranges_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, d_skeys,
-		                   M, d_lstart, d_lend);
-		HIPCHK(hipGetLastError());
+		const int t0c = (int)(kfirst / (uint32_t)sc.WH);
+		const int t1c = std::min(F, (int)(klast / (uint32_t)sc.WH) + sc.sPt);
+		std::vector<uint32_t> band_groups;
+		for (uint32_t gf : geo.frame_first)
+			if ((int)gf > g0 && (int)gf < g0 + Gc) band_groups.push_back(gf);
+		aggregate_chunk(ctx, sc, band_groups, g0, M, t0c, t1c, d_base, d_kstart, d_svals, d_epos, d_etile,
+		                d_ew, d_est, d_win, d_num, d_wsum, d_cnt + 4);
 		tm.mark(step1 ? "s1_aggregate" : "s2_aggregate");
 	}
 	hipLaunchKernelGGL(normalize_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream,
@@ -614,6 +663,7 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 	const int step = p1->sizePatch ? (p2->sizePatch ? 0 : 1) : (p2->sizePatch ? 2 : -1);
 	if (step == -1) throw ArgError("VideoNLB::runNlBayes: Both patch sizes are zero.");
 	if (nparts < 1) throw ArgError("nparts must be >= 1");
+	if (nparts > 255) throw ArgError("nparts above 255 is not supported", BM4DOF_EUNSUPP);
 	if (W <= 0 || H <= 0 || F <= 0) throw ArgError("empty video");
 	if ((uint64_t)W * H * F * C >= (1ull << 32))
 		throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
@@ -621,6 +671,8 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 	HIPCHK(hipSetDevice(ctx->device));
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
 	ctx->ev.clear();
+	ctx->agg_blocks = ctx->agg_overflow_blocks = 0;
+	ctx->step_groups[0] = ctx->step_groups[1] = ctx->step_rows[0] = ctx->step_rows[1] = 0;
 	ctx->timings.clear();
 	Timer tm(ctx);
 	tm.mark("start");
@@ -882,6 +934,19 @@ int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image, unsig
 			out_tile[g] = geo.gtile[g];
 		}
 	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots)
+{
+	if (!ctx || !stats) return 0;
+	const int64_t v[9] = {(int64_t)ctx->agg_blocks, (int64_t)ctx->agg_overflow_blocks,
+	                      ctx->agg_region[0], ctx->agg_region[1], ctx->agg_region[2],
+	                      (int64_t)ctx->step_groups[0], (int64_t)ctx->step_rows[0],
+	                      (int64_t)ctx->step_groups[1], (int64_t)ctx->step_rows[1]};
+	int n = 0;
+	for (; n < 9 && n < max_slots; n++) stats[n] = v[n];
+	return n;
 }
 
 __attribute__((visibility("default")))

@@ -339,7 +339,8 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 //   step 2: computeBayesEstimateStep2 (:2196-2242) -> _vbm3d (:2015-2174)
 // Each neighbour's filtered patch, already multiplied by its aggregation
 // weight and window (the factor computeAggregationStep{1,2} applies,
-// nldct.cpp:2296-2299, 2385-2393), is written to row rank_of[g*np+n] of est.
+// nldct.cpp:2296-2299, 2385-2393), is written to row e = base[g-g0] + n of
+// est: rows follow the reference's processing order (group, then neighbour).
 // ---------------------------------------------------------------------------
 
 // Z[k*N+n] = sum_pix X[pix*N+n] * B[k*P+pix], sequential in pix.
@@ -443,12 +444,13 @@ __device__ void write_raw_group(const float *__restrict__ img, const uint32_t *_
 
 __global__ __launch_bounds__(FILTER_THREADS)
 void filter1_kernel(const float *__restrict__ img, const uint32_t *__restrict__ idx,
-                    const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ rank_of,
-                    StepConst sc, int g0, int Gc,
+                    const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ base,
+                    const uint32_t *__restrict__ gtile, StepConst sc, int g0, int Gc,
                     const float *__restrict__ basis, const float *__restrict__ binv,
                     const float *__restrict__ win, float *__restrict__ est,
-                    float *__restrict__ entry_w, uint8_t *__restrict__ early,
-                    float *__restrict__ gweight, uint32_t *__restrict__ n_early)
+                    float *__restrict__ entry_w, uint32_t *__restrict__ etile,
+                    uint8_t *__restrict__ early, float *__restrict__ gweight,
+                    uint32_t *__restrict__ n_early)
 {
 	extern __shared__ float fsm[];
 	__shared__ uint32_t s_rrow[64];
@@ -462,7 +464,10 @@ void filter1_kernel(const float *__restrict__ img, const uint32_t *__restrict__ 
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
 	float *X = fsm;                  // C*P*N (filtered estimates of all channels)
 	float *Z = X + (size_t)C * P * N; // P*N
-	if (threadIdx.x < N) s_rrow[threadIdx.x] = rank_of[(size_t)g * sc.npmax + threadIdx.x];
+	if (threadIdx.x < N) {
+		s_rrow[threadIdx.x] = base[gl] + threadIdx.x;
+		etile[base[gl] + threadIdx.x] = gtile[g];
+	}
 	if (threadIdx.x == 0) s_nnz_i = 0;
 	__syncthreads();
 
@@ -532,15 +537,16 @@ void filter1_kernel(const float *__restrict__ img, const uint32_t *__restrict__ 
 __global__ __launch_bounds__(FILTER_THREADS)
 void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ basic,
                     const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
-                    const uint32_t *__restrict__ rank_of, StepConst sc, int g0, int Gc,
+                    const uint32_t *__restrict__ base, const uint32_t *__restrict__ gtile,
+                    StepConst sc, int g0, int Gc,
                     const float *__restrict__ basis, const float *__restrict__ binv,
                     const float *__restrict__ win, float *__restrict__ est,
-                    float *__restrict__ entry_w, uint8_t *__restrict__ early,
-                    float *__restrict__ gweight, uint32_t *__restrict__ n_early)
+                    float *__restrict__ entry_w, uint32_t *__restrict__ etile,
+                    uint8_t *__restrict__ early, float *__restrict__ gweight,
+                    uint32_t *__restrict__ n_early)
 {
 	extern __shared__ float fsm[];
 	__shared__ uint32_t s_rrow[64];
-	__shared__ float s_part[FILTER_THREADS];
 	__shared__ float s_terms[256];
 	const int gl = blockIdx.x;
 	if (gl >= Gc) return;
@@ -551,7 +557,10 @@ void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ b
 	float *X = fsm;                   // C*P*N noisy group, then estimates
 	float *Zn = X + (size_t)C * P * N; // P*N
 	float *Yb = Zn + (size_t)P * N;   // P*N (basic group, then its coefficients)
-	if (threadIdx.x < N) s_rrow[threadIdx.x] = rank_of[(size_t)g * sc.npmax + threadIdx.x];
+	if (threadIdx.x < N) {
+		s_rrow[threadIdx.x] = base[gl] + threadIdx.x;
+		etile[base[gl] + threadIdx.x] = gtile[g];
+	}
 	__syncthreads();
 
 	if (group_all_equal(noisy, gidx, N, sc)) {
@@ -574,11 +583,19 @@ void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ b
 		haar(Zn, P, N, true);
 		haar(Yb, P, N, true);
 		if (!sc.order_inv) {
+			// w^2 terms are kept (in Yb) and summed in the reference's order
+			// below: the float sum sets the aggregation weight.
 			for (int o = threadIdx.x; o < P * N; o += FILTER_THREADS) {
 				const float y = Yb[o];
 				const float w = y * y / (y * y + sc.beta_sigma2);
 				Zn[o] = Zn[o] * w;
-				part = part + w * w;
+				Yb[o] = w * w;
+			}
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				float v = part;
+				for (int o = 0; o < P * N; o++) v = v + Yb[o];
+				part = v;
 			}
 		} else {
 			for (int i = threadIdx.x; i < P; i += FILTER_THREADS) {
@@ -603,16 +620,9 @@ void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ b
 		inv_transform(Zn, binv, Xc, P, N);
 		__syncthreads();
 	}
-	// Fixed-shape tree reduction of the Wiener energy.
-	s_part[threadIdx.x] = part;
-	__syncthreads();
-	for (int s = FILTER_THREADS / 2; s > 0; s >>= 1) {
-		if (threadIdx.x < s) s_part[threadIdx.x] = s_part[threadIdx.x] + s_part[threadIdx.x + s];
-		__syncthreads();
-	}
 	__shared__ float s_w;
 	if (threadIdx.x == 0) {
-		const float variance = sc.order_inv ? var_oi : s_part[0];
+		const float variance = sc.order_inv ? var_oi : part;
 		s_w = (float)(1. / (double)sc.sigma2 / (double)variance);
 		gweight[g] = s_w;
 		early[g] = 0;
@@ -629,11 +639,21 @@ void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ b
 }
 
 // ---------------------------------------------------------------------------
-// Aggregation support: per-position neighbour lists.
+// Aggregation in the reference's order.
+//
+// computeAggregationStep{1,2} (nldct.cpp:2259-2425) adds every neighbour
+// patch of every group into float numerator and weight images, group after
+// group in processing order and neighbour after neighbour, so each pixel's
+// float sums are folded in the order of the rows e of est.  The kernels below
+// reproduce exactly that fold: for a block of pixels they gather the rows
+// whose patches touch it (from per-position lists), sort them by e and add
+// them one by one into float accumulators.  Only rows of the pixel's owner
+// tile count (the tile whose interior holds it; subBuildTight copies only
+// interiors, lib_videot.hpp:1128-1160).
 // ---------------------------------------------------------------------------
 
-// keys[e] = position of neighbour n of group g, vals[e] = g*np + n, for the
-// groups [g0, g0+Gc) of a chunk; base = exclusive scan of nsim over the chunk.
+// keys[e] = patch position of row e, vals[e] = e, for the groups
+// [g0, g0+Gc) of a chunk; base = exclusive scan of nsim over the chunk.
 __global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
                                     const uint32_t *__restrict__ base, int npmax, int g0, int Gc,
                                     uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
@@ -645,42 +665,22 @@ __global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint
 	if (n >= (int)nsim[g]) return;
 	const uint32_t e = base[gl] + n;
 	keys[e] = idx[(size_t)g * npmax + n];
-	vals[e] = (uint32_t)g * npmax + n;
+	vals[e] = e;
 }
 
-// After the stable sort by position: list boundaries, inverse permutation
-// and per-row tile numbers.
-__global__ void mark_ranges_kernel(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ svals,
-                                   int64_t M, int npmax, const uint32_t *__restrict__ gtile,
-                                   uint32_t *__restrict__ lstart, uint32_t *__restrict__ lend,
-                                   uint32_t *__restrict__ rank_of, uint32_t *__restrict__ etile)
+// Number of rows per patch position.
+__global__ void count_positions_kernel(const uint32_t *__restrict__ keys, int64_t M,
+                                       uint32_t *__restrict__ cnt)
 {
 	const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (r >= M) return;
-	const uint32_t k = skeys[r];
-	if (r == 0 || skeys[r - 1] != k) lstart[k] = (uint32_t)r;
-	if (r == M - 1 || skeys[r + 1] != k) lend[k] = (uint32_t)(r + 1);
-	const uint32_t v = svals[r];
-	rank_of[v] = (uint32_t)r;
-	etile[r] = gtile[v / npmax];
-}
-
-// Clears the list boundaries touched by a chunk.
-__global__ void clear_ranges_kernel(const uint32_t *__restrict__ skeys, int64_t M,
-                                    uint32_t *__restrict__ lstart, uint32_t *__restrict__ lend)
-{
-	const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (r >= M) return;
-	const uint32_t k = skeys[r];
-	lstart[k] = 0;
-	lend[k] = 0;
+	if (r < M) atomicAdd(&cnt[keys[r]], 1u);
 }
 
 // Early-out groups: multiply the raw patch rows by the stale weight w_s and
 // the window (nldct.cpp:2296-2299).  stale[g*np+n] holds w_s per slot.
 __global__ void fixup_early_kernel(const uint32_t *__restrict__ glist, int ng,
-                                   const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ rank_of,
-                                   const float *__restrict__ stale, const float *__restrict__ win,
+                                   const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ base,
+                                   int g0, const float *__restrict__ stale, const float *__restrict__ win,
                                    int npmax, int C, int P, float *__restrict__ est,
                                    float *__restrict__ entry_w)
 {
@@ -691,70 +691,287 @@ __global__ void fixup_early_kernel(const uint32_t *__restrict__ glist, int ng,
 	for (int o = threadIdx.x; o < N * C * P; o += blockDim.x) {
 		const int n = o / (C * P), r = o - n * C * P;
 		const int pix = r % P;
-		const uint32_t row = rank_of[(size_t)g * npmax + n];
+		const uint32_t row = base[g - g0] + n;
 		const float ww = stale[(size_t)g * npmax + n] * win[pix];
 		est[(size_t)row * C * P + r] = ww * est[(size_t)row * C * P + r];
 		if (r == 0) entry_w[row] = stale[(size_t)g * npmax + n];
 	}
 }
 
-// ---------------------------------------------------------------------------
-// Aggregation: one thread per pixel of the chunk's bounding box; it sums, in
-// a fixed order, the contributions of the chunk's patches covering it that
-// belong to groups of the pixel's owner tile (the tile whose interior holds
-// it: only that tile's result is copied back by subBuildTight,
-// lib_videot.hpp:1128-1160).  num / wsum accumulate across chunks.
-// ---------------------------------------------------------------------------
-__global__ void gather_kernel(const uint32_t *__restrict__ lstart, const uint32_t *__restrict__ lend,
-                              const uint32_t *__restrict__ etile, const float *__restrict__ entry_w,
-                              const float *__restrict__ est, const float *__restrict__ win,
-                              StepConst sc, int bx0, int by0, int bt0, int bw, int bh, int bf,
-                              double *__restrict__ num, double *__restrict__ wsum)
+#define AGG_CAPB 1024   // rows of one band held in LDS
+#define AGG_MAXPOS 512  // patch positions overlapping an 8x8 pixel block
+#define AGG_STAGE 2048  // floats of staged row slices
+
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << (threadIdx.x & 63)) - 1ull; }
+
+// Stable LSD radix sort of K keys in LDS by one wave, 8-bit digits.
+// Returns the buffer that holds the result.
+__device__ uint32_t *wave_radix_sort(uint32_t *a, uint32_t *b, int K, int nbits, uint32_t *hist)
 {
-	const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (o >= (int64_t)bw * bh * bf) return;
-	const int lx = (int)(o % bw);
-	const int ly = (int)((o / bw) % bh);
-	const int lt = (int)(o / ((int64_t)bw * bh));
-	const int x = bx0 + lx, y = by0 + ly, t = bt0 + lt;
-	const int owner = (y / sc.hTmp) * sc.nW + (x / sc.wTmp);
-	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P, C = sc.C;
-	double acc[4] = {0., 0., 0., 0.};
-	double wacc = 0.;
-	bool any = false;
-	for (int ht = 0; ht < sPt; ht++) {
-		const int qt = t - ht;
-		if (qt < 0) break;
-		for (int hy = 0; hy < sPx; hy++) {
-			const int qy = y - hy;
-			if (qy < 0) break;
-			for (int hx = 0; hx < sPx; hx++) {
-				const int qx = x - hx;
-				if (qx < 0) break;
-				const uint32_t key = (uint32_t)qt * sc.WH + (uint32_t)qy * sc.W + qx;
-				const uint32_t r0 = lstart[key], r1 = lend[key];
-				const int i = (ht * sPx + hy) * sPx + hx;
-				for (uint32_t r = r0; r < r1; r++) {
-					if ((int)etile[r] != owner) continue;
-					any = true;
-					wacc = wacc + (double)(entry_w[r] * win[i]);
-					const float *e = est + (size_t)r * C * P + i;
-					for (int c = 0; c < C; c++) acc[c] = acc[c] + (double)e[(size_t)c * P];
+	const int lane = threadIdx.x & 63;
+	for (int shift = 0; shift < nbits; shift += 8) {
+		for (int d = lane; d < 256; d += 64) hist[d] = 0;
+		__syncthreads();
+		for (int k = lane; k < K; k += 64) atomicAdd(&hist[(a[k] >> shift) & 255u], 1u);
+		__syncthreads();
+		{
+			const uint32_t c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2], c3 = hist[4 * lane + 3];
+			const int local = (int)(c0 + c1 + c2 + c3);
+			const int incl = wave_incl_scan(local);
+			const uint32_t ex = (uint32_t)(incl - local);
+			__syncthreads();
+			hist[4 * lane] = ex;
+			hist[4 * lane + 1] = ex + c0;
+			hist[4 * lane + 2] = ex + c0 + c1;
+			hist[4 * lane + 3] = ex + c0 + c1 + c2;
+		}
+		__syncthreads();
+		for (int base = 0; base < K; base += 64) {
+			const int k = base + lane;
+			const bool valid = k < K;
+			const uint32_t key = valid ? a[k] : 0u;
+			const uint32_t d = (key >> shift) & 255u;
+			uint64_t m = __ballot(valid);
+#pragma unroll
+			for (int bit = 0; bit < 8; bit++) {
+				const bool bset = (d >> bit) & 1u;
+				const uint64_t bb = __ballot(valid && bset);
+				m &= bset ? bb : ~bb;
+			}
+			const int rank = __popcll(m & lanemask_lt());
+			const int cnt = __popcll(m);
+			const uint32_t pos = hist[d] + (uint32_t)rank;
+			__syncthreads();
+			if (valid) b[pos] = key;
+			if (valid && rank == cnt - 1) hist[d] = pos + 1;
+			__syncthreads();
+		}
+		uint32_t *t = a; a = b; b = t;
+	}
+	return a;
+}
+
+__device__ __forceinline__ int owner_tile(int x, int y, const StepConst &sc)
+{
+	return (y / sc.hTmp) * sc.nW + (x / sc.wTmp);
+}
+
+// First index in [lo, hi) of the sorted list v with v[i] >= key.
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__ v, uint32_t lo,
+                                                    uint32_t hi, uint32_t key)
+{
+	while (lo < hi) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (v[mid] < key) lo = mid + 1; else hi = mid;
+	}
+	return lo;
+}
+
+// Ordered aggregation of one 8 x 8 pixel block of one frame by one wave,
+// lane = pixel.  The rows touching the block are those of the patch
+// positions whose patch overlaps it; each position's rows are a sorted run of
+// svals.  Rows are taken band by band (bands = consecutive row ranges, one
+// per reference frame of a tile, so a band's rows are few), each band's rows
+// are sorted in LDS, their slices at this frame staged in LDS, then added one
+// by one into the lane's float accumulators: the order
+// computeAggregationStep{1,2} adds them in (nldct.cpp:2284-2300,
+// 2376-2393).  num/wsum hold the running sums and are updated in place.
+template <int CC>
+__global__ __launch_bounds__(64)
+void agg_exact_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__restrict__ svals,
+                      const uint32_t *__restrict__ epos, const uint32_t *__restrict__ etile,
+                      const float *__restrict__ entry_w, const float *__restrict__ est,
+                      const float *__restrict__ win, StepConst sc, const uint32_t *__restrict__ bands,
+                      int nbands, int tframe0, float *__restrict__ num, float *__restrict__ wsum,
+                      uint32_t *__restrict__ nsplit)
+{
+	__shared__ uint32_t keysA[AGG_CAPB];
+	__shared__ uint32_t keysB[AGG_CAPB];
+	__shared__ uint32_t hist[256];
+	__shared__ uint32_t pcur[AGG_MAXPOS], pend[AGG_MAXPOS], phead[AGG_MAXPOS], pub[AGG_MAXPOS];
+	__shared__ uint32_t poff[AGG_MAXPOS + 1];
+	__shared__ float stage[AGG_STAGE];
+	__shared__ float swin[256];
+	// per staged row: (qx - x0 + 16) | (qy - y0 + 16) << 8 | dt << 16 | tile << 24
+	__shared__ uint32_t m_pos[64];
+	__shared__ float m_w[64];
+	__shared__ uint32_t m_e[64];
+	const int lane = threadIdx.x;
+	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
+	const int SL = sPx * sPx;
+	const int x0 = (int)blockIdx.x * 8, y0 = (int)blockIdx.y * 8, t0 = tframe0 + (int)blockIdx.z;
+	const int qx0 = max(0, x0 - sPx + 1), qx1 = min(x0 + 7, sc.W - sPx);
+	const int qy0 = max(0, y0 - sPx + 1), qy1 = min(y0 + 7, sc.H - sPx);
+	const int qt0 = max(0, t0 - sPt + 1), qt1 = min(t0, sc.F - sPt);
+	if (qx1 < qx0 || qy1 < qy0 || qt1 < qt0 || t0 >= sc.F) return;
+	const int nqx = qx1 - qx0 + 1, nqy = qy1 - qy0 + 1;
+	const int npos = nqx * nqy * (qt1 - qt0 + 1);
+
+	for (int p = lane; p < npos; p += 64) {
+		const int qx = qx0 + p % nqx, qy = qy0 + (p / nqx) % nqy, qt = qt0 + p / (nqx * nqy);
+		const uint32_t key = (uint32_t)qt * sc.WH + (uint32_t)qy * sc.W + qx;
+		const uint32_t a = kstart[key], b = kstart[key + 1];
+		pcur[p] = a;
+		pend[p] = b;
+		phead[p] = a < b ? svals[a] : 0xffffffffu;
+	}
+	// this lane's pixel, its owner tile and running sums
+	const int x = x0 + (lane & 7), y = y0 + (lane >> 3);
+	const bool valid = x < sc.W && y < sc.H;
+	const int owner = valid ? owner_tile(x, y, sc) : -1;
+	const size_t pix = (size_t)t0 * sc.WH + (size_t)y * sc.W + x;
+	float aw = 0.f, an[CC];
+#pragma unroll
+	for (int c = 0; c < CC; c++) an[c] = 0.f;
+	if (valid) {
+		aw = wsum[pix];
+#pragma unroll
+		for (int c = 0; c < CC; c++) an[c] = num[(size_t)t0 * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x];
+	}
+	const int NB = min(64, AGG_STAGE / (SL * CC)); // rows per staged batch
+	for (int i = lane; i < P; i += 64) swin[i] = win[i];
+	// staging layout: a row slice is SL*CC floats; when it is at most 64
+	// floats, 64/(SL*CC) rows are staged per pass (lane -> row jl, float rl)
+	const int RS = SL * CC;
+	const int rpp = RS <= 64 ? 64 / RS : 1;
+	const int jl = RS <= 64 ? lane / RS : 0;
+	const int rl = RS <= 64 ? lane - jl * RS : lane;
+	const bool stage_lane = RS <= 64 ? jl < rpp : true;
+	const int nslot = RS <= 64 ? 1 : (RS + 63) / 64;
+	const int xo = x - x0 + 16, yo = y - y0 + 16;
+	__syncthreads();
+	for (int b = 0; b < nbands; b++) {
+		uint32_t lo = bands[b];
+		const uint32_t band_end = bands[b + 1];
+		while (lo < band_end) {
+			// runs of [lo, hi): one band, or a part of it when it is too large
+			uint32_t hi = band_end;
+			int K = 0;
+			for (int attempt = 0;; attempt++) {
+				// run of each position inside [lo, hi): [pcur, first row >= hi)
+				for (int p = lane; p < npos; p += 64) {
+					const uint32_t a = pcur[p];
+					uint32_t ub = a;
+					if (phead[p] < hi) ub = lower_bound_u32(svals, a + 1, pend[p], hi);
+					pub[p] = ub;
+				}
+				__syncthreads();
+				K = 0;
+				for (int p0 = 0; p0 < npos; p0 += 64) {
+					const int p = p0 + lane;
+					const int len = p < npos ? (int)(pub[p] - pcur[p]) : 0;
+					const int incl = wave_incl_scan(len);
+					if (p < npos) poff[p] = (uint32_t)(K + incl - len);
+					K += __shfl(incl, 63, 64);
+				}
+				if (K <= AGG_CAPB) break;
+				if (lane == 0 && attempt == 0) atomicAdd(nsplit, 1u);
+				hi = lo + (hi - lo + 1) / 2; // halve the row range and retry
+				__syncthreads();
+			}
+			if (lane == 0) poff[npos] = (uint32_t)K;
+			__syncthreads();
+			if (K > 0) {
+				// gather (entry k lies in the run p with poff[p] <= k < poff[p+1])
+				for (int k = lane; k < K; k += 64) {
+					int a = 0, bb = npos - 1;
+					while (a < bb) {
+						const int mid = (a + bb + 1) >> 1;
+						if (poff[mid] <= (uint32_t)k) a = mid; else bb = mid - 1;
+					}
+					keysA[k] = svals[pcur[a] + ((uint32_t)k - poff[a])] - lo;
+				}
+				__syncthreads();
+				int nbits = 1;
+				while (nbits < 32 && (1ull << nbits) < (uint64_t)(hi - lo)) nbits++;
+				const uint32_t *sorted = wave_radix_sort(keysA, keysB, K, nbits, hist);
+				__syncthreads();
+				for (int k0 = 0; k0 < K; k0 += NB) {
+					const int nb = min(NB, K - k0);
+					if (lane < nb) {
+						const uint32_t e = sorted[k0 + lane] + lo;
+						int qx, qy, qt;
+						pos_decode(epos[e], sc.W, sc.WH, qx, qy, qt);
+						m_pos[lane] = (uint32_t)(qx - x0 + 16) | ((uint32_t)(qy - y0 + 16) << 8) |
+						              ((uint32_t)(t0 - qt) << 16) | (etile[e] << 24);
+						m_w[lane] = entry_w[e];
+						m_e[lane] = e;
+					}
+					__syncthreads();
+					// stage the rows' slices at frame t0
+					if (RS <= 64) {
+						for (int j0 = 0; j0 < nb; j0 += rpp) {
+							const int j = j0 + jl;
+							if (stage_lane && j < nb) {
+								const int c = rl / SL, sidx = rl - c * SL;
+								const int dt = (int)((m_pos[j] >> 16) & 255u);
+								stage[j * RS + rl] = est[((size_t)m_e[j] * CC + c) * P + dt * SL + sidx];
+							}
+						}
+					} else {
+						for (int j = 0; j < nb; j++)
+							for (int k = 0; k < nslot; k++) {
+								const int r = lane + 64 * k;
+								if (r < RS) {
+									const int c = r / SL, sidx = r - c * SL;
+									const int dt = (int)((m_pos[j] >> 16) & 255u);
+									stage[j * RS + r] = est[((size_t)m_e[j] * CC + c) * P + dt * SL + sidx];
+								}
+							}
+					}
+					__syncthreads();
+					for (int j = 0; j < nb; j++) {
+						const uint32_t mp = m_pos[j];
+						const int dx = xo - (int)(mp & 255u);
+						const int dy = yo - (int)((mp >> 8) & 255u);
+						if ((unsigned)dx < (unsigned)sPx && (unsigned)dy < (unsigned)sPx && (int)(mp >> 24) == owner) {
+							const int dt = (int)((mp >> 16) & 255u);
+							const int sidx = dy * sPx + dx;
+							aw = aw + m_w[j] * swin[dt * SL + sidx];
+#pragma unroll
+							for (int c = 0; c < CC; c++) an[c] = an[c] + stage[j * RS + c * SL + sidx];
+						}
+					}
+					__syncthreads();
 				}
 			}
+			// advance the position cursors past [lo, hi)
+			if (K > 0)
+				for (int p = lane; p < npos; p += 64) {
+					const uint32_t ub = pub[p];
+					if (ub != pcur[p]) {
+						pcur[p] = ub;
+						phead[p] = ub < pend[p] ? svals[ub] : 0xffffffffu;
+					}
+				}
+			__syncthreads();
+			lo = hi;
 		}
 	}
-	if (!any) return;
-	const size_t pix = (size_t)t * sc.WH + (size_t)y * sc.W + x;
-	wsum[pix] = wsum[pix] + wacc;
-	for (int c = 0; c < C; c++) {
-		const size_t a = (size_t)t * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x;
-		num[a] = num[a] + acc[c];
+	if (valid) {
+		wsum[pix] = aw;
+#pragma unroll
+		for (int c = 0; c < CC; c++) num[(size_t)t0 * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x] = an[c];
 	}
 }
 
+#define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
+	float *, uint32_t *
+template __global__ void agg_exact_kernel<1>(AGG_ARGS);
+template __global__ void agg_exact_kernel<3>(AGG_ARGS);
+template __global__ void agg_exact_kernel<4>(AGG_ARGS);
+
+// Row index of the first row of each band: bands[b] = base[gfirst[b] - g0].
+__global__ void band_bounds_kernel(const uint32_t *__restrict__ base, const uint32_t *__restrict__ gfirst,
+                                   int nb, int g0, uint32_t *__restrict__ bands)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b < nb) bands[b] = base[gfirst[b] - g0];
+}
+
 // computeWeightedAggregation, nldct.cpp:2436-2448.
-__global__ void normalize_kernel(const double *__restrict__ num, const double *__restrict__ wsum,
+__global__ void normalize_kernel(const float *__restrict__ num, const float *__restrict__ wsum,
                                  const float *__restrict__ noisy, StepConst sc,
                                  float *__restrict__ out)
 {
@@ -763,9 +980,8 @@ __global__ void normalize_kernel(const double *__restrict__ num, const double *_
 	if (o >= n) return;
 	const int64_t t = o / sc.WHC, r = o - t * sc.WHC;
 	const int64_t pix = t * sc.WH + r % sc.WH;
-	const float w = (float)wsum[pix];
-	const float nm = (float)num[o];
-	out[o] = w > 0.f ? nm / w : nm / noisy[o];
+	const float w = wsum[pix];
+	out[o] = w > 0.f ? num[o] / w : num[o] / noisy[o];
 }
 
 } // namespace bm4dof

@@ -159,6 +159,14 @@ int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image,
  * static string naming each slot.  Returns the number of slots filled. */
 int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, int max_slots);
 
+/* Statistics of the last bm4dof_run / bm4dof_run_dev call: stats[0] =
+ * pixel blocks of the ordered aggregation, stats[1] = blocks whose rows
+ * did not fit the block capacity and were processed in several row bands,
+ * stats[2..4] = block shape (x, y, frames) of the last aggregation launch,
+ * stats[5..6] = groups and group rows (patches over all groups) of step 1,
+ * stats[7..8] = the same for step 2.  Returns the number of slots filled. */
+int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
+
 /* ---- TV-L1 optical flow ---- */
 
 /*
