@@ -67,7 +67,7 @@ EXPORTS = [
     "bm4dof_set_n_similar", "bm4dof_ctx_create", "bm4dof_ctx_destroy", "bm4dof_last_error",
     "bm4dof_global_error", "bm4dof_run", "bm4dof_run_dev", "bm4dof_bm_count", "bm4dof_bm",
     "bm4dof_last_timings", "bm4dof_last_stats", "Dual_TVL1_optic_flow_multiscale", "bm4dof_tvl1",
-    "bm4dof_tvl1_sequence",
+    "bm4dof_tvl1_sequence", "bm4dof_ctx_set_option",
 ]
 
 
@@ -115,6 +115,7 @@ def lib():
                             vp, vp, vp, vp]
     L.bm4dof_last_timings.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_char_p), C.c_int]
     L.bm4dof_last_stats.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
+    L.bm4dof_ctx_set_option.argtypes = [vp, C.c_char_p, C.c_longlong]
     if not hasattr(L, "bm4dof_tvl1"):
         _LIB = L
         return L
@@ -220,6 +221,10 @@ class Context:
                                            Cc, fflow_ptr, bflow_ptr, basic_ptr, final_ptr,
                                            nparts, gr))
         return gr[0], gr[1]
+
+    def set_option(self, key, value):
+        """bm4dof_ctx_set_option: select between kernels of equal output."""
+        self._check(self._L.bm4dof_ctx_set_option(self.h, key.encode(), int(value)))
 
     def timings(self):
         ms = (C.c_float * 64)()

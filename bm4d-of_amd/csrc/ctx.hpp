@@ -72,6 +72,7 @@ struct bm4dof_ctx {
 	uint64_t agg_blocks = 0, agg_overflow_blocks = 0;
 	int agg_region[3] = {0, 0, 0};
 	uint64_t step_groups[2] = {0, 0}, step_rows[2] = {0, 0};   // groups / group rows per step
+	bool fast_filter = true;   // register-resident filter where the group shape allows it
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

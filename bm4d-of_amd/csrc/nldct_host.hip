@@ -27,6 +27,14 @@ __global__ void filter2_kernel(const float *, const float *, const uint32_t *, c
                                const uint32_t *, const uint32_t *, StepConst, int, int,
                                const float *, const float *, const float *, float *, float *,
                                uint32_t *, uint8_t *, float *, uint32_t *);
+#define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	const uint32_t *, StepConst, int, int, const float *, const float *, const float *, float *, \
+	float *, uint32_t *, uint8_t *, float *, uint32_t *
+#define FILTER2_ARGS const float *, const float *, const uint32_t *, const uint32_t *, \
+	const uint32_t *, const uint32_t *, StepConst, int, int, const float *, const float *, \
+	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *
+template <int PX> __global__ void filter1_fast_kernel(FILTER1_ARGS);
+template <int PX> __global__ void filter2_fast_kernel(FILTER2_ARGS);
 __global__ void emit_entries_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int,
                                     int, uint32_t *, uint32_t *);
 __global__ void count_positions_kernel(const uint32_t *, int64_t, uint32_t *);
@@ -532,6 +540,9 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 	const size_t filt_lds = step1 ? ((size_t)C * sc.P * np + (size_t)sc.P * np) * 4
 	                              : ((size_t)C * sc.P * np + 2 * (size_t)sc.P * np) * 4;
 	if (filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
+	// register-resident filter: one channel, 64-sample patches, <= 32 neighbours
+	const bool fast = ctx->fast_filter && C == 1 && sc.P == 64 && np <= 32 && !sc.order_inv &&
+	                  ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1));
 
 	for (int g0 = 0; g0 < G; g0 += chunk_groups) {
 		const int Gc = std::min(chunk_groups, G - g0);
@@ -579,7 +590,16 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 		HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
 		tm.mark(step1 ? "s1_sort" : "s2_sort");
 
-		if (step1)
+		if (fast && step1)
+			hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
+			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, d_gtile, sc, g0,
+			                   Gc, d_basis, d_binv, d_win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
+		else if (fast)
+			hipLaunchKernelGGL(sc.sPx == 4 ? filter2_fast_kernel<4> : filter2_fast_kernel<8>, dim3(Gc),
+			                   dim3(64), 0, ctx->stream, noisy_w, basic_w, d_idx, d_nsim, d_base, d_gtile,
+			                   sc, g0, Gc, d_basis, d_binv, d_win, d_est, d_ew, d_etile, d_early, d_gw,
+			                   d_cnt);
+		else if (step1)
 			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
 			                   d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc, d_basis, d_binv, d_win, d_est,
 			                   d_ew, d_etile, d_early, d_gw, d_cnt);
@@ -825,6 +845,17 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 
 __attribute__((visibility("default")))
 const char *bm4dof_last_error(const bm4dof_ctx *ctx) { return ctx ? ctx->err.c_str() : g_global_err.c_str(); }
+
+__attribute__((visibility("default")))
+int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
+{
+	if (!ctx) return BM4DOF_EINVAL;
+	return guarded(ctx, [&] {
+		const std::string k = key ? key : "";
+		if (k == "fast_filter") ctx->fast_filter = value != 0;
+		else throw ArgError("unknown option '" + k + "'");
+	});
+}
 
 __attribute__((visibility("default")))
 int bm4dof_run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,

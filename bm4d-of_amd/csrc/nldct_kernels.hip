@@ -7,9 +7,13 @@
 //     1723-1737);
 //   * the forward patch transform, a sequential-K dot product per
 //     coefficient (nldct.cpp:1840-1844 via productMatrix/cblas_sgemm);
-//   * the Haar butterflies and the hard threshold (nldct.cpp:1847-1907).
-// Smooth quantities (inverse transform, Wiener weights, aggregation sums) use
-// a fixed but different order; they match the reference to rounding.
+//   * the Haar butterflies and the hard threshold (nldct.cpp:1847-1907);
+//   * the inverse transform, sequential in the coefficient index
+//     (nldct.cpp:1927-1931, 2155-2159);
+//   * the Wiener energy sum_w^2 in (coefficient, patch) order
+//     (nldct.cpp:2123-2135) and the aggregation sums in group order
+//     (nldct.cpp:2284-2300, 2376-2393).
+// The GPU output is therefore bitwise equal to the sequential reference.
 #include "common.hpp"
 
 namespace bm4dof {
@@ -637,6 +641,252 @@ void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ b
 	}
 	for (int n = threadIdx.x; n < N; n += FILTER_THREADS) entry_w[s_rrow[n]] = w;
 }
+
+// ---------------------------------------------------------------------------
+// Register-resident filter for one-channel groups of 64-sample patches
+// (px=4/pt=4 or px=8/pt=1) with at most 32 neighbours: one wave per group.
+//
+// Forward transform: lane k holds row k of the basis in registers; the
+// samples of patch n are wave-uniform and arrive through scalar loads, so
+// each coefficient z[k][n] is a register dot product in pixel order.
+// Haar along the group and the threshold / Wiener shrinkage then run inside
+// each lane (lane k owns coefficient k of every patch).  Inverse transform:
+// lane pix holds column pix of the inverse basis; the coefficients of
+// patch n are read from LDS as wave-uniform (broadcast) values, sequential
+// in k.  Same arithmetic, same order as filter1_kernel / filter2_kernel.
+// ---------------------------------------------------------------------------
+
+#define FAST_P 64
+#define FAST_NMAX 32
+
+__device__ __forceinline__ void haar_lane(float (&z)[FAST_NMAX], int N, bool forward)
+{
+	const float isqrt2 = 0.70710678118654752440f;
+#pragma unroll
+	for (int l = 0; l < 5; l++) {
+		const int s = forward ? (2 << l) : (32 >> l);   // 2..32 / 32..2
+		const int h = s >> 1;
+		if (s <= N) {
+#pragma unroll
+			for (int n = 0; n < FAST_NMAX; n += s) {
+				if (n < N) {
+					const float z0 = z[n] * isqrt2;
+					const float z1 = z[n + h] * isqrt2;
+					z[n] = z0 + z1;
+					z[n + h] = z0 - z1;
+				}
+			}
+		}
+	}
+}
+
+__device__ __forceinline__ int wave_sum_int(int v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+
+// Forward transform of the N patches at anchors gidx[0..N) of `img`
+// (one channel, anchor address = position) into zs[n*64 + lane].
+template <int PX>
+__device__ __forceinline__ void fast_forward(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                             int N, int W, int WH, const float (&brow)[FAST_P],
+                                             float *__restrict__ zs)
+{
+	constexpr int PT = FAST_P / (PX * PX);
+	for (int n = 0; n < N; n++) {
+		const float *src = img + gidx[n];
+		float acc = 0.f;
+#pragma unroll
+		for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+			for (int hy = 0; hy < PX; hy++) {
+				const float *r = src + ht * WH + hy * W;
+#pragma unroll
+				for (int hx = 0; hx < PX; hx++) acc = acc + r[hx] * brow[(ht * PX + hy) * PX + hx];
+			}
+		zs[n * FAST_P + threadIdx.x] = acc;
+	}
+}
+
+// Inverse transform of the coefficients zs[n*64 + k] and the weighted est
+// rows: est[(row0+n)*64 + pix] = (w*win[pix]) * sum_k z[k][n] Bi[k][pix].
+__device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const float *__restrict__ binv,
+                                             int N, float ww, uint32_t row0, float *__restrict__ est)
+{
+	float bcol[FAST_P];
+#pragma unroll
+	for (int k = 0; k < FAST_P; k++) bcol[k] = binv[k * FAST_P + threadIdx.x];
+	for (int n = 0; n < N; n++) {
+		const float4 *zr = reinterpret_cast<const float4 *>(zs + n * FAST_P);
+		float acc = 0.f;
+#pragma unroll
+		for (int q = 0; q < FAST_P / 4; q++) {
+			const float4 z = zr[q];
+			acc = acc + z.x * bcol[4 * q + 0];
+			acc = acc + z.y * bcol[4 * q + 1];
+			acc = acc + z.z * bcol[4 * q + 2];
+			acc = acc + z.w * bcol[4 * q + 3];
+		}
+		est[(size_t)(row0 + n) * FAST_P + threadIdx.x] = ww * acc;
+	}
+}
+
+// Equality early-out (nldct.cpp:1986-1999, 2222-2235) with lane = sample:
+// true when N > 1 and every patch equals patch 0; such a group is written
+// raw and flagged for the stale-weight fixup.
+template <int PX>
+__device__ __forceinline__ bool fast_early_out(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                               int N, int W, int WH, int g, uint32_t row0,
+                                               float *__restrict__ est, uint8_t *__restrict__ early,
+                                               float *__restrict__ gweight, uint32_t *__restrict__ n_early)
+{
+	if (N <= 1) return false;
+	const int l = threadIdx.x;
+	const int off = (l / (PX * PX)) * WH + ((l / PX) % PX) * W + (l % PX);
+	const float v0 = img[gidx[0] + off];
+	int differs = 0;
+	for (int n = 1; n < N; n++) differs |= (img[gidx[n] + off] != v0);
+	if (__any(differs)) return false;
+	for (int n = 0; n < N; n++) est[(size_t)(row0 + n) * FAST_P + l] = img[gidx[n] + off];
+	if (l == 0) { early[g] = 1; gweight[g] = 0.f; atomicAdd(n_early, 1u); }
+	return true;
+}
+
+template <int PX>
+__global__ __launch_bounds__(64)
+void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restrict__ idx,
+                         const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ base,
+                         const uint32_t *__restrict__ gtile, StepConst sc, int g0, int Gc,
+                         const float *__restrict__ basis, const float *__restrict__ binv,
+                         const float *__restrict__ win, float *__restrict__ est,
+                         float *__restrict__ entry_w, uint32_t *__restrict__ etile,
+                         uint8_t *__restrict__ early, float *__restrict__ gweight,
+                         uint32_t *__restrict__ n_early)
+{
+	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
+	const int gl = blockIdx.x;
+	if (gl >= Gc) return;
+	const int g = g0 + gl;
+	const int lane = threadIdx.x;
+	const int N = (int)nsim[g];
+	const uint32_t row0 = base[gl];
+	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	if (lane < N) etile[row0 + lane] = gtile[g];
+	if (fast_early_out<PX>(img, gidx, N, sc.W, sc.WH, g, row0, est, early, gweight, n_early)) return;
+
+	{
+		float brow[FAST_P];
+#pragma unroll
+		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
+		fast_forward<PX>(img, gidx, N, sc.W, sc.WH, brow, zs);
+	}
+	__syncthreads();
+	float z[FAST_NMAX];
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+	haar_lane(z, N, true);
+	int cnt = 0;
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) {
+		if (n < N) {
+			if (z[n] * z[n] > sc.beta_sigma2 || (lane == 0 && n == 0)) cnt++;
+			else z[n] = 0.f;
+		}
+	}
+	haar_lane(z, N, false);
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
+	const int nnz_i = wave_sum_int(cnt);
+	const float nnz = (float)nnz_i;
+	const float w = 1 / sc.sigma2 / nnz;
+	if (lane == 0) { gweight[g] = w; early[g] = 0; }
+	if (lane < N) entry_w[row0 + lane] = w;
+	__syncthreads();
+	fast_inverse(zs, binv, N, w * win[lane], row0, est);
+}
+
+template <int PX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restrict__ basic,
+                         const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
+                         const uint32_t *__restrict__ base, const uint32_t *__restrict__ gtile,
+                         StepConst sc, int g0, int Gc,
+                         const float *__restrict__ basis, const float *__restrict__ binv,
+                         const float *__restrict__ win, float *__restrict__ est,
+                         float *__restrict__ entry_w, uint32_t *__restrict__ etile,
+                         uint8_t *__restrict__ early, float *__restrict__ gweight,
+                         uint32_t *__restrict__ n_early)
+{
+	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
+	const int gl = blockIdx.x;
+	if (gl >= Gc) return;
+	const int g = g0 + gl;
+	const int lane = threadIdx.x;
+	const int N = (int)nsim[g];
+	const uint32_t row0 = base[gl];
+	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	if (lane < N) etile[row0 + lane] = gtile[g];
+	if (fast_early_out<PX>(noisy, gidx, N, sc.W, sc.WH, g, row0, est, early, gweight, n_early)) return;
+
+	float z[FAST_NMAX], y[FAST_NMAX];
+	{
+		float brow[FAST_P];
+#pragma unroll
+		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
+		fast_forward<PX>(basic, gidx, N, sc.W, sc.WH, brow, zs);
+		__syncthreads();
+#pragma unroll
+		for (int n = 0; n < FAST_NMAX; n++) y[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+		__syncthreads();
+		fast_forward<PX>(noisy, gidx, N, sc.W, sc.WH, brow, zs);
+	}
+	__syncthreads();
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+	haar_lane(z, N, true);
+	haar_lane(y, N, true);
+	// Wiener shrinkage; y[n] becomes the energy term w^2
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) {
+		if (n < N) {
+			const float w = y[n] * y[n] / (y[n] * y[n] + sc.beta_sigma2);
+			z[n] = z[n] * w;
+			y[n] = w * w;
+		}
+	}
+	// sum of w^2 in the reference's order, coefficient-major then patch:
+	// lane k folds its N terms into the running sum handed on by lane k-1.
+	float v = 0.f;
+	for (int k = 0; k < FAST_P; k++) {
+		float t = v;
+#pragma unroll
+		for (int n = 0; n < FAST_NMAX; n++) if (n < N) t = t + y[n];
+		v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t), k));
+	}
+	haar_lane(z, N, false);
+	__syncthreads();
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
+	const float w = (float)(1. / (double)sc.sigma2 / (double)v);
+	if (lane == 0) { gweight[g] = w; early[g] = 0; }
+	if (lane < N) entry_w[row0 + lane] = w;
+	__syncthreads();
+	fast_inverse(zs, binv, N, w * win[lane], row0, est);
+}
+
+
+#define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	const uint32_t *, StepConst, int, int, const float *, const float *, const float *, float *, \
+	float *, uint32_t *, uint8_t *, float *, uint32_t *
+#define FILTER2_ARGS const float *, const float *, const uint32_t *, const uint32_t *, \
+	const uint32_t *, const uint32_t *, StepConst, int, int, const float *, const float *, \
+	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *
+template __global__ void filter1_fast_kernel<4>(FILTER1_ARGS);
+template __global__ void filter1_fast_kernel<8>(FILTER1_ARGS);
+template __global__ void filter2_fast_kernel<4>(FILTER2_ARGS);
+template __global__ void filter2_fast_kernel<8>(FILTER2_ARGS);
 
 // ---------------------------------------------------------------------------
 // Aggregation in the reference's order.

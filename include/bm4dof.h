@@ -100,6 +100,18 @@ const char *bm4dof_last_error(const bm4dof_ctx *ctx);
 const char *bm4dof_global_error(void);
 
 /*
+ * Execution options of a context (no counterpart in the reference, whose
+ * execution is fixed).  They select between kernels that compute the same
+ * result; outputs do not depend on them.  Keys:
+ *   "fast_filter"  1 (default): register-resident group filter where the
+ *                  group shape allows it (one channel, 64-sample patches,
+ *                  at most 32 neighbours, no orderInvariance); 0: the
+ *                  general LDS filter for every group.
+ * Returns BM4DOF_EINVAL for an unknown key.
+ */
+int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value);
+
+/*
  * Whole two-step denoiser.  Replaces
  *   std::vector<float> VideoNLB::runNlBayes(noisy, fflow, bflow, basic,
  *                                           final, prms1, prms2)

@@ -98,6 +98,24 @@ def test_video_two_step_parity(ctx, w, h, f, nparts, flow):
     assert abs(dpsnr) <= 0.01
 
 
+@pytest.mark.parametrize("px,pt,np_,frames", [(4, 4, 32, 8), (8, 1, 16, 1), (4, 4, 8, 6), (8, 1, 32, 3)])
+def test_register_and_lds_filters_agree(ctx, px, pt, np_, frames):
+    """The register-resident filter (one channel, 64-sample patches) and the
+    general LDS filter give bitwise equal basic and final estimates."""
+    clean, noisy = _video(80, 56, frames, seed=10 * px + np_)
+    a1 = A.cli_params(1, 20, noisy.shape, px=px, pt=pt, wx=9, np_=np_)
+    a2 = A.cli_params(2, 20, noisy.shape, px=px, pt=pt, wx=9, np_=np_)
+    try:
+        ctx.set_option("fast_filter", 0)
+        b0, f0, _ = ctx.run(a1, a2, noisy, nparts=2)
+    finally:
+        ctx.set_option("fast_filter", 1)
+    b1, f1, _ = ctx.run(a1, a2, noisy, nparts=2)
+    assert np.array_equal(b0, b1) and np.array_equal(f0, f1)
+    with pytest.raises(A.BM4DOFError, match="unknown option"):
+        ctx.set_option("no_such_option", 1)
+
+
 def test_image_config0_bior_and_kaiser(ctx):
     """Single grayscale image, px=8 (bior1.5 + Kaiser window in step 1, DCT in
     step 2), np=16, 21x21 search, beta1 = 3."""
