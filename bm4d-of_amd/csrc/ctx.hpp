@@ -63,7 +63,7 @@ struct bm4dof_ctx {
 	bm4dof::DevBuf gpos, gtile, tiles, idx, nsim, base, keys, vals, skeys, svals, sort_tmp, lstart, lend,
 	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
 	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
-	    io_final, bm_image;
+	    io_final, bm_image, bm_wins, bm_wmeta;
 	// timings of the last run
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
@@ -73,6 +73,8 @@ struct bm4dof_ctx {
 	int agg_region[3] = {0, 0, 0};
 	uint64_t step_groups[2] = {0, 0}, step_rows[2] = {0, 0};   // groups / group rows per step
 	bool fast_filter = true;   // register-resident filter where the group shape allows it
+	bool fast_bm = true;       // staged one-wave block matching where the patch shape allows it
+	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

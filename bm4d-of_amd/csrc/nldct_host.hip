@@ -19,6 +19,11 @@ __global__ void color_kernel(float *, int, int, int, int, float, float, float);
 __global__ void bm_kernel(const float *, const float *, const float *, const uint32_t *,
                           const uint32_t *, const TileBox *, StepConst, int, int, uint32_t *,
                           uint32_t *, int);
+__global__ void bm_window_kernel(const float *, const float *, const uint32_t *, const uint32_t *,
+                                 const TileBox *, StepConst, int, int4 *, int2 *);
+#define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
+	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+template <int PX, int PT> __global__ void bm_fast_kernel(BMF_ARGS);
 __global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
                                const uint32_t *, StepConst, int, int, const float *, const float *,
                                const float *, float *, float *, uint32_t *, uint8_t *, float *,
@@ -44,6 +49,9 @@ __global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, cons
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
 	float *, uint32_t *
 template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
+#define AGG2_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	const float *, const float *, const float *, StepConst, int, float *, float *, uint32_t *
+template <int CC> __global__ void agg_chunk_kernel(AGG2_ARGS);
 __global__ void band_bounds_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
 __global__ void normalize_kernel(const float *, const float *, const float *, StepConst,
                                  float *);
@@ -338,6 +346,9 @@ void check_step_params(const bm4dof_params &p, unsigned C)
 		throw ArgError("temporal search range above 64 frames is not supported", BM4DOF_EUNSUPP);
 	if (p.offSet == 0 || p.offSetTime == 0) throw ArgError("offSet and offSetTime must be positive");
 	if (p.sizeSearchWindow == 0) throw ArgError("sizeSearchWindow must be positive");
+	// patch positions overlapping an 8 x 8 pixel block of one frame
+	if ((p.sizePatch + 7) * (p.sizePatch + 7) * p.sizePatchTime > 512)
+		throw ArgError("patch too large for the ordered aggregation ((px+7)^2*pt > 512)", BM4DOF_EUNSUPP);
 	(void)C;
 }
 
@@ -430,6 +441,26 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	const size_t lds = (size_t)maxCand * 12 + (size_t)sc.C * sc.P * 4;
 	if (lds > 60 * 1024) throw ArgError("search region too large for the block-matching kernel", BM4DOF_EUNSUPP);
 	if (G == 0) return;
+	// staged one-wave kernel: distances on one channel (step 1 always takes
+	// channel 0; step 2 sums all channels), patches 4x4x4 or 8x8x1
+	const int PX = sc.sPx, PT = sc.sPt;
+	const bool shape_ok = (PX == 4 && PT == 4) || (PX == 8 && PT == 1);
+	const size_t stage_floats = std::max<size_t>(2 * (size_t)(sc.sWx + PX - 1) * (sc.sWx + PX - 1) * PT,
+	                                             (size_t)maxCand);
+	const size_t lds_fast = ((size_t)maxCand + stage_floats) * 4;
+	if (ctx->fast_bm && shape_ok && (step == 1 || sc.C == 1) && lds_fast <= 56 * 1024) {
+		int4 *d_wins = ctx->bm_wins.as<int4>((size_t)G * sWt);
+		int2 *d_wmeta = ctx->bm_wmeta.as<int2>((size_t)G);
+		hipLaunchKernelGGL(bm_window_kernel, dim3(nblk(G, 256)), dim3(256), 0, ctx->stream, ff, bf, d_gpos,
+		                   d_gtile, d_tiles, sc, G, d_wins, d_wmeta);
+		HIPCHK(hipGetLastError());
+		auto kern = PX == 4 ? bm_fast_kernel<4, 4> : bm_fast_kernel<8, 1>;
+		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_fast,
+		                   ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile, d_tiles, sc, (int)stage_floats,
+		                   d_idx, d_nsim, G);
+		HIPCHK(hipGetLastError());
+		return;
+	}
 	hipLaunchKernelGGL(bm_kernel, dim3(G), dim3(256), lds, ctx->stream, img, ff, bf, d_gpos, d_gtile,
 	                   d_tiles, sc, step, maxCand, d_idx, d_nsim, G);
 	HIPCHK(hipGetLastError());
@@ -459,7 +490,19 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 	}
 	HIPCHK(hipMemsetAsync(d_nsplit, 0, 4, ctx->stream));
 	const dim3 grid((sc.W + 7) / 8, (sc.H + 7) / 8, t1c - t0c);
-	if (sc.C == 1)
+	// chunked kernel: a block then spans at most 2 x 2 tile interiors
+	const bool chunked = ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8;
+	if (chunked) {
+		if (sc.C == 1)
+			hipLaunchKernelGGL(agg_chunk_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile,
+			                   ew, est, win, sc, t0c, num, wsum, d_nsplit);
+		else if (sc.C == 3)
+			hipLaunchKernelGGL(agg_chunk_kernel<3>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile,
+			                   ew, est, win, sc, t0c, num, wsum, d_nsplit);
+		else
+			hipLaunchKernelGGL(agg_chunk_kernel<4>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile,
+			                   ew, est, win, sc, t0c, num, wsum, d_nsplit);
+	} else if (sc.C == 1)
 		hipLaunchKernelGGL(agg_exact_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
 		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
 	else if (sc.C == 3)
@@ -476,7 +519,7 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 	ctx->agg_overflow_blocks += ns;
 	ctx->agg_region[0] = 8;
 	ctx->agg_region[1] = 8;
-	ctx->agg_region[2] = nb;
+	ctx->agg_region[2] = chunked ? 1 : nb;
 }
 
 // One denoising step on device buffers in the working colour space.
@@ -834,7 +877,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->bm_image};
+	                  &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta};
 	for (DevBuf *b : bufs) b->release();
 	DevBuf *tvb[] = {&ctx->tv_frames, &ctx->tv_pairs, &ctx->tv_part, &ctx->tv_mm, &ctx->tv_state,
 	                 &ctx->tv_work, &ctx->tv_pyr, &ctx->tv_taps};
@@ -853,6 +896,8 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 	return guarded(ctx, [&] {
 		const std::string k = key ? key : "";
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
+		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
+		else if (k == "fast_bm") ctx->fast_bm = value != 0;
 		else throw ArgError("unknown option '" + k + "'");
 	});
 }

@@ -337,6 +337,281 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 }
 
 // ---------------------------------------------------------------------------
+// Block matching, staged form (one channel, patches of PX x PX x PT).
+//
+// bm_window_kernel: one thread per group computes the flow trajectory and
+// the shifted search window of every frame of the temporal range
+// (nldct.cpp:1173-1259), in the reference's frame schedule, as
+// (qt, rx0 | ry0 << 16, nx | ny << 16, first candidate index).
+// bm_fast_kernel: one wave per group; the search regions of two frames at a
+// time are staged in LDS, lane = candidate, and each distance is the
+// sequential (ht, hy, hx) sum of the reference (nldct.cpp:1262-1274,
+// 1723-1737).  Selection as bm_kernel.
+// ---------------------------------------------------------------------------
+__global__ void bm_window_kernel(const float *__restrict__ fflow, const float *__restrict__ bflow,
+                                 const uint32_t *__restrict__ gpos, const uint32_t *__restrict__ gtile,
+                                 const TileBox *__restrict__ tiles, StepConst sc, int G,
+                                 int4 *__restrict__ wins, int2 *__restrict__ wmeta)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= G) return;
+	const int sWt = sc.wtf + sc.wtb + 1;
+	int gx, gy, gt;
+	pos_decode(gpos[g], sc.W, sc.WH, gx, gy, gt);
+	const TileBox tb = tiles[gtile[g]];
+	const int Wt = tb.ex - tb.ox, Ht = tb.ey - tb.oy;
+	const int px = gx - tb.ox, py = gy - tb.oy, pt = gt;
+	const int F = sc.F, sPx = sc.sPx, sPt = sc.sPt;
+	const int shift_t = min(0, pt - sc.wtb) + max(0, pt + sc.wtf - F + sPt);
+	const int rt0 = max(0, pt - sc.wtb - shift_t);
+	const int rt1 = min(F - sPt, pt + sc.wtf - shift_t);
+	// The frame schedule is pt, pt+1..rt1, pt-1..rt0: the trajectory is
+	// walked in that same order and each frame's window emitted on the way.
+	const int half = (sc.sWx - 1) / 2;
+	int base = 0, ii = 0;
+	auto emit = [&](int qt, int cx, int cy) {
+		int shift_x = min(0, cx - half);
+		int shift_y = min(0, cy - half);
+		shift_x += max(0, cx + half - Wt + sPx);
+		shift_y += max(0, cy + half - Ht + sPx);
+		const int rx0 = max(0, cx - half - shift_x);
+		const int ry0 = max(0, cy - half - shift_y);
+		const int rx1 = min(Wt - sPx, cx + half - shift_x);
+		const int ry1 = min(Ht - sPx, cy + half - shift_y);
+		const int nx = max(0, rx1 - rx0 + 1), ny = max(0, ry1 - ry0 + 1);
+		wins[(size_t)g * sWt + ii] = make_int4(qt, rx0 | (ry0 << 16), nx | (ny << 16), base);
+		base += nx * ny;
+		ii++;
+	};
+	emit(pt, px, py);
+	for (int dir = 1; dir >= -1; dir -= 2) {
+		const float *fl = dir > 0 ? fflow : bflow;
+		int cx = px, cy = py, ct = pt;
+		for (int qt = pt + dir; dir > 0 ? qt <= rt1 : qt >= rt0; qt += dir) {
+			float fx = 0.f, fy = 0.f;
+			if (sc.use_flow) {
+				const size_t b = (size_t)ct * 2 * sc.WH + (size_t)(cy + tb.oy) * sc.W + (cx + tb.ox);
+				fx = fl[b];
+				fy = fl[b + sc.WH];
+			}
+			const float cx_f = (float)cx + fx;
+			const float cy_f = (float)cy + fy;
+			cx = (int)fmax_ref(0.f, fmin_ref((float)Wt - 1.f, roundf(cx_f)));
+			cy = (int)fmax_ref(0.f, fmin_ref((float)Ht - 1.f, roundf(cy_f)));
+			ct = qt;
+			emit(qt, cx, cy);
+		}
+	}
+	const int nf = ii;
+	wmeta[g] = make_int2(nf, base);
+}
+
+template <int PX, int PT>
+__global__ __launch_bounds__(64)
+void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
+                    const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
+                    const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
+                    StepConst sc, int stage_floats, uint32_t *__restrict__ out_idx,
+                    uint32_t *__restrict__ out_nsim, int G)
+{
+	constexpr int PP = PX * PX * PT;
+	extern __shared__ unsigned char smem[];
+	const int sWt = sc.wtf + sc.wtb + 1;
+	const int maxCand = sc.sWx * sc.sWx * sWt;
+	uint32_t *skey = (uint32_t *)smem;                    // maxCand
+	float *stage = (float *)(skey + maxCand);             // stage_floats (two frames' regions)
+	int *stie = (int *)stage;                             // ties, after the distances
+	__shared__ uint32_t hist[256];
+	__shared__ uint32_t s_bucket, s_below;
+	__shared__ uint32_t sel_key[64];
+	__shared__ int sel_rank[64];
+	__shared__ uint32_t out_key[64];
+	__shared__ int out_rank[64];
+	__shared__ int s_na, s_nb;
+
+	const int g = blockIdx.x;
+	if (g >= G) return;
+	const int lane = threadIdx.x;
+	const uint32_t pos = gpos[g];
+	uint32_t *oidx = out_idx + (size_t)g * sc.npmax;
+	if (sc.npmax <= 1) {
+		if (lane == 0) { oidx[0] = pos; out_nsim[g] = 1; }
+		return;
+	}
+	int gx, gy, gt;
+	pos_decode(pos, sc.W, sc.WH, gx, gy, gt);
+	const TileBox tb = tiles[gtile[g]];
+	const int px = gx - tb.ox, py = gy - tb.oy, pt = gt;
+	const int2 meta = wmeta[g];
+	const int nframes = meta.x, nsrch = meta.y;
+	const int4 *gw = wins + (size_t)g * sWt;
+
+	// reference patch (channel 0)
+	float ref[PP];
+#pragma unroll
+	for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+		for (int hy = 0; hy < PX; hy++)
+#pragma unroll
+			for (int hx = 0; hx < PX; hx++)
+				ref[(ht * PX + hy) * PX + hx] = img[(size_t)(gt + ht) * sc.WHC + (size_t)(gy + hy) * sc.W + gx + hx];
+
+	for (int ii = 0; ii < nframes; ii += 2) {
+		// stage frames ii and ii+1
+		int4 f[2];
+		int rw[2], rh[2], off[2], ncand[2];
+		int tot = 0;
+#pragma unroll
+		for (int j = 0; j < 2; j++) {
+			f[j] = ii + j < nframes ? gw[ii + j] : make_int4(0, 0, 0, 0);
+			const int nx = f[j].z & 0xffff, ny = f[j].z >> 16;
+			rw[j] = nx + PX - 1;
+			rh[j] = ny + PX - 1;
+			ncand[j] = (ii + j < nframes) ? nx * ny : 0;
+			off[j] = tot;
+			tot += ncand[j] ? rw[j] * rh[j] * PT : 0;
+		}
+		__syncthreads();
+		for (int i = lane; i < tot; i += 64) {
+			const int j = i >= off[1] && ncand[1] ? 1 : 0;
+			const int r = i - off[j];
+			const int ht = r / (rw[j] * rh[j]);
+			const int q = r - ht * rw[j] * rh[j];
+			const int yy = q / rw[j], xx = q - yy * rw[j];
+			const int ax = tb.ox + (f[j].y & 0xffff) + xx, ay = tb.oy + (f[j].y >> 16) + yy;
+			stage[i] = img[(size_t)(f[j].x + ht) * sc.WHC + (size_t)ay * sc.W + ax];
+		}
+		__syncthreads();
+		const int n01 = ncand[0] + ncand[1];
+		for (int o = lane; o < n01; o += 64) {
+			const int j = o >= ncand[0] ? 1 : 0;
+			const int lo = o - (j ? ncand[0] : 0);
+			const int nx = f[j].z & 0xffff;
+			const int cy = lo / nx, cx = lo - cy * nx;
+			const float *b = stage + off[j] + cy * rw[j] + cx;
+			float d = 0.f;
+#pragma unroll
+			for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+				for (int hy = 0; hy < PX; hy++) {
+					const float *r = b + (ht * rh[j] + hy) * rw[j];
+#pragma unroll
+					for (int hx = 0; hx < PX; hx++) {
+						const float dif = ref[(ht * PX + hy) * PX + hx] - r[hx];
+						d = d + dif * dif;
+					}
+				}
+			skey[f[j].w + lo] = __float_as_uint(d);
+		}
+	}
+	// index of the reference patch itself (frame pt is the first of the list)
+	int s_self;
+	{
+		const int4 f0 = gw[0];
+		s_self = f0.w + (py - (f0.y >> 16)) * (f0.z & 0xffff) + (px - (f0.y & 0xffff));
+	}
+	if (lane == 0) { s_na = 0; s_nb = 0; }
+	__syncthreads();
+
+	const int nSimP = min(sc.npmax, nsrch);
+	int kk = 1;
+	while (kk * 2 <= nSimP) kk *= 2;
+	uint32_t prefix = 0, pmask = 0;
+	uint32_t remaining = (uint32_t)kk;
+	for (int pass = 0; pass < 4; pass++) {
+		const int shift = 24 - 8 * pass;
+#pragma unroll
+		for (int b = 0; b < 4; b++) hist[4 * lane + b] = 0;
+		__syncthreads();
+		for (int s2 = lane; s2 < nsrch; s2 += 64) {
+			const uint32_t k = skey[s2];
+			if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+		}
+		__syncthreads();
+		{
+			const uint32_t c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2], c3 = hist[4 * lane + 3];
+			const int local = (int)(c0 + c1 + c2 + c3);
+			const int incl = wave_incl_scan(local);
+			const int excl = incl - local;
+			if ((uint32_t)excl < remaining && remaining <= (uint32_t)incl) {
+				uint32_t cum = (uint32_t)excl;
+				const uint32_t cs[4] = {c0, c1, c2, c3};
+				for (int b = 0; b < 4; b++) {
+					if (cum + cs[b] >= remaining) { s_bucket = 4 * lane + b; s_below = cum; break; }
+					cum += cs[b];
+				}
+			}
+		}
+		__syncthreads();
+		prefix |= s_bucket << shift;
+		pmask |= 255u << shift;
+		remaining -= s_below;
+		__syncthreads();
+	}
+	const uint32_t T = prefix;
+	const int na_expected = kk - (int)remaining;
+	for (int s2 = lane; s2 < nsrch; s2 += 64) {
+		const uint32_t k = skey[s2];
+		const int rank = s2 == s_self ? -1 : s2;
+		if (k < T) {
+			const int slot = atomicAdd(&s_na, 1);
+			sel_key[slot] = k;
+			sel_rank[slot] = rank;
+		} else if (k == T) {
+			const int slot = atomicAdd(&s_nb, 1);
+			stie[slot] = rank;
+		}
+	}
+	__syncthreads();
+	const int nb = s_nb;
+	for (int j = lane; j < nb; j += 64) {
+		const int r = stie[j];
+		int c = 0;
+		for (int i = 0; i < nb; i++) c += stie[i] < r;
+		if (c < (int)remaining) {
+			sel_key[na_expected + c] = T;
+			sel_rank[na_expected + c] = r;
+		}
+	}
+	__syncthreads();
+	if (lane < kk) {
+		const uint32_t k = sel_key[lane];
+		const int r = sel_rank[lane];
+		int p = 0;
+		for (int j = 0; j < kk; j++) {
+			const uint32_t kj = sel_key[j];
+			p += (kj < k) || (kj == k && sel_rank[j] < r);
+		}
+		out_key[p] = k;
+		out_rank[p] = r;
+	}
+	__syncthreads();
+	int n = 1;
+	while (2 * n <= nSimP && __uint_as_float(out_key[n]) < sc.tau2) n *= 2;
+	if (lane == 0) out_nsim[g] = n;
+	if (lane < n) {
+		const int r = out_rank[lane];
+		uint32_t q;
+		if (r < 0) q = pos;
+		else {
+			int ii = 0;
+			while (ii + 1 < nframes && gw[ii + 1].w <= r) ii++;
+			const int4 f = gw[ii];
+			const int o = r - f.w;
+			const int nx = f.z & 0xffff;
+			const int qy = (f.y >> 16) + o / nx, qx = (f.y & 0xffff) + o % nx;
+			q = (uint32_t)f.x * (uint32_t)sc.WH + (uint32_t)(qy + tb.oy) * (uint32_t)sc.W + (uint32_t)(qx + tb.ox);
+		}
+		oidx[lane] = q;
+	}
+}
+
+#define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
+	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+template __global__ void bm_fast_kernel<4, 4>(BMF_ARGS);
+template __global__ void bm_fast_kernel<8, 1>(BMF_ARGS);
+
+// ---------------------------------------------------------------------------
 // Group filtering, one workgroup per group.
 //   step 1: computeBayesEstimateStep1 (nldct.cpp:1962-2006) ->
 //           computeBayesEstimateStep1_vbm3d (:1799-1943)
@@ -1204,6 +1479,261 @@ void agg_exact_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 		for (int c = 0; c < CC; c++) num[(size_t)t0 * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x] = an[c];
 	}
 }
+
+// ---------------------------------------------------------------------------
+// Ordered aggregation, chunked form.  One wave per 8 x 8 pixel block of one
+// frame, lane = pixel.  All rows whose patch overlaps the block (the runs of
+// its patch positions in the position-sorted row list) are gathered into LDS
+// and radix-sorted by row index e, in e-ranges of at most AGG2_CAP rows.
+// Each chunk of up to 64 sorted rows then gets, per row, the 64-bit mask of
+// the block pixels it covers inside its own tile; a 64 x 64 bit transpose
+// across the wave turns the row masks into per-pixel masks, and every lane
+// folds only its own rows, in increasing e: the order of
+// computeAggregationStep{1,2} (nldct.cpp:2284-2300, 2376-2393).
+// ---------------------------------------------------------------------------
+
+#define AGG2_CAP 1536    // rows of one e-range held in LDS
+#define AGG2_STAGE 1024  // floats of staged row slices per chunk
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
+{
+	const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+	const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+	return ((uint64_t)hi << 32) | lo;
+}
+
+// Lane r holds row r of a 64 x 64 bit matrix (bit c = column c); returns
+// column `lane` (bit r = row r's bit `lane`).  Recursive block swap.
+__device__ __forceinline__ uint64_t transpose_bits64(uint64_t a)
+{
+	const int lane = threadIdx.x & 63;
+	const uint64_t L[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+	                       0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+	for (int i = 0; i < 6; i++) {
+		const int s = 32 >> i;
+		const uint64_t p = shfl_xor_u64(a, s);
+		a = (lane & s) ? ((a & ~L[i]) | ((p >> s) & L[i])) : ((a & L[i]) | ((p & L[i]) << s));
+	}
+	return a;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+	return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+	return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+	return v;
+}
+
+// bits [a, b) of an 8-bit lane row, a and b clamped to [0, 8]
+__device__ __forceinline__ uint32_t span8(int a, int b)
+{
+	a = max(a, 0); b = min(b, 8);
+	return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
+}
+
+template <int CC>
+__global__ __launch_bounds__(64)
+void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__restrict__ svals,
+                      const uint32_t *__restrict__ epos, const uint32_t *__restrict__ etile,
+                      const float *__restrict__ entry_w, const float *__restrict__ est,
+                      const float *__restrict__ win, StepConst sc, int tframe0,
+                      float *__restrict__ num, float *__restrict__ wsum, uint32_t *__restrict__ nsplit)
+{
+	__shared__ uint32_t keysA[AGG2_CAP];
+	__shared__ uint32_t keysB[AGG2_CAP];
+	__shared__ uint32_t hist[256];
+	__shared__ uint32_t pcur[AGG_MAXPOS], pend[AGG_MAXPOS], pub[AGG_MAXPOS];
+	__shared__ float stage[AGG2_STAGE];
+	__shared__ float swin[256];
+	__shared__ uint32_t m_pos[64];   // (dx0 + 16) | (dy0 + 16) << 8 | dt << 16
+	__shared__ uint32_t m_e[64];
+	__shared__ float m_w[64];
+	const int lane = threadIdx.x;
+	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
+	const int SL = sPx * sPx;
+	const int x0 = (int)blockIdx.x * 8, y0 = (int)blockIdx.y * 8, t0 = tframe0 + (int)blockIdx.z;
+	const int qx0 = max(0, x0 - sPx + 1), qx1 = min(x0 + 7, sc.W - sPx);
+	const int qy0 = max(0, y0 - sPx + 1), qy1 = min(y0 + 7, sc.H - sPx);
+	const int qt0 = max(0, t0 - sPt + 1), qt1 = min(t0, sc.F - sPt);
+	if (qx1 < qx0 || qy1 < qy0 || qt1 < qt0 || t0 >= sc.F) return;
+	const int nqx = qx1 - qx0 + 1, nqy = qy1 - qy0 + 1;
+	const int npos = nqx * nqy * (qt1 - qt0 + 1);
+
+	// position runs, total row count and e-range of the block
+	uint32_t ksum = 0, emin = 0xffffffffu, emax = 0u;
+	for (int p = lane; p < npos; p += 64) {
+		const int qx = qx0 + p % nqx, qy = qy0 + (p / nqx) % nqy, qt = qt0 + p / (nqx * nqy);
+		const uint32_t key = (uint32_t)qt * sc.WH + (uint32_t)qy * sc.W + qx;
+		const uint32_t a = kstart[key], b = kstart[key + 1];
+		pcur[p] = a;
+		pend[p] = b;
+		if (b > a) {
+			ksum += b - a;
+			emin = min(emin, svals[a]);
+			emax = max(emax, svals[b - 1]);
+		}
+	}
+	ksum = wave_sum_u32(ksum);
+	emin = wave_min_u32(emin);
+	emax = wave_max_u32(emax);
+
+	// this lane's pixel and the lanes owned by each tile present in the block
+	const int lx = lane & 7, ly = lane >> 3;
+	const int x = x0 + lx, y = y0 + ly;
+	const bool valid = x < sc.W && y < sc.H;
+	const int owner = valid ? owner_tile(x, y, sc) : -1;
+	int own_t[4] = {-1, -1, -1, -1};
+	uint64_t own_m[4] = {0, 0, 0, 0};
+	{
+		uint64_t rest = __ballot(valid);
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			if (rest) {
+				const int lead = __ffsll((unsigned long long)rest) - 1;
+				const int o = __shfl(owner, lead, 64);
+				const uint64_t m = __ballot(owner == o);
+				own_t[i] = o;
+				own_m[i] = m;
+				rest &= ~m;
+			}
+		}
+	}
+	const size_t pix = (size_t)t0 * sc.WH + (size_t)y * sc.W + x;
+	float aw = 0.f, an[CC];
+#pragma unroll
+	for (int c = 0; c < CC; c++) an[c] = 0.f;
+	if (valid) {
+		aw = wsum[pix];
+#pragma unroll
+		for (int c = 0; c < CC; c++) an[c] = num[(size_t)t0 * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x];
+	}
+	for (int i = lane; i < P; i += 64) swin[i] = win[i];
+	const int RS = SL * CC;                       // floats of one staged row slice
+	const int NBR = min(64, AGG2_STAGE / RS);     // rows per chunk
+	__syncthreads();
+
+	uint32_t left = ksum, lo = emin;
+	int nrange = 0;
+	while (left > 0) {
+		// e-range [lo, hi) with at most AGG2_CAP rows
+		uint64_t hi64 = (uint64_t)emax + 1;
+		if (left > AGG2_CAP)
+			hi64 = lo + max<uint64_t>(1, ((uint64_t)emax + 1 - lo) * (AGG2_CAP * 3 / 4) / left);
+		uint32_t hi = (uint32_t)hi64;
+		uint32_t K = 0;
+		for (;;) {
+			uint32_t kl = 0;
+			for (int p = lane; p < npos; p += 64) {
+				const uint32_t a = pcur[p], b = pend[p];
+				uint32_t l = a, r = b;   // first index in [a, b) with svals >= hi
+				while (l < r) {
+					const uint32_t mid = (l + r) >> 1;
+					if (svals[mid] < hi) l = mid + 1; else r = mid;
+				}
+				pub[p] = l;
+				kl += l - a;
+			}
+			K = wave_sum_u32(kl);
+			if (K <= AGG2_CAP) break;
+			hi = lo + (hi - lo + 1) / 2;
+		}
+		nrange++;
+		__syncthreads();
+		if (K > 0) {
+			// gather the range's rows (keys relative to lo)
+			uint32_t base = 0;
+			for (int p0 = 0; p0 < npos; p0 += 64) {
+				const int p = p0 + lane;
+				const uint32_t len = p < npos ? pub[p] - pcur[p] : 0u;
+				const int incl = wave_incl_scan((int)len);
+				uint32_t o = base + (uint32_t)incl - len;
+				if (p < npos)
+					for (uint32_t i = pcur[p]; i < pub[p]; i++) keysA[o++] = svals[i] - lo;
+				base += (uint32_t)__shfl(incl, 63, 64);
+			}
+			__syncthreads();
+			int nbits = 1;
+			while (nbits < 32 && (1ull << nbits) < (uint64_t)(hi - lo)) nbits++;
+			const uint32_t *sorted = wave_radix_sort(keysA, keysB, (int)K, nbits, hist);
+			__syncthreads();
+			for (uint32_t k0 = 0; k0 < K; k0 += NBR) {
+				const int nb = (int)min<uint32_t>(NBR, K - k0);
+				uint64_t mrow = 0;
+				if (lane < nb) {
+					const uint32_t e = sorted[k0 + lane] + lo;
+					int qx, qy, qt;
+					pos_decode(epos[e], sc.W, sc.WH, qx, qy, qt);
+					const int dx0 = qx - x0, dy0 = qy - y0, dt = t0 - qt;
+					const uint32_t cols = span8(dx0, dx0 + sPx), rows = span8(dy0, dy0 + sPx);
+					uint64_t rsel = 0;
+#pragma unroll
+					for (int r = 0; r < 8; r++) if ((rows >> r) & 1u) rsel |= 1ull << (8 * r);
+					mrow = (uint64_t)cols * rsel;
+					const int tl = (int)etile[e];
+					uint64_t om = 0;
+#pragma unroll
+					for (int i = 0; i < 4; i++) if (own_t[i] == tl) om = own_m[i];
+					mrow &= om;
+					m_pos[lane] = (uint32_t)(dx0 + 16) | ((uint32_t)(dy0 + 16) << 8) | ((uint32_t)dt << 16);
+					m_e[lane] = e;
+					m_w[lane] = entry_w[e];
+				}
+				__syncthreads();
+				// stage the rows' slices at frame t0
+				for (int i = lane; i < nb * RS; i += 64) {
+					const int j = i / RS, r = i - j * RS;
+					const int c = r / SL, sidx = r - c * SL;
+					const int dt = (int)((m_pos[j] >> 16) & 255u);
+					stage[i] = est[((size_t)m_e[j] * CC + c) * P + dt * SL + sidx];
+				}
+				__syncthreads();
+				uint64_t mine = transpose_bits64(mrow);
+				while (mine) {
+					const int j = __ffsll((unsigned long long)mine) - 1;
+					mine &= mine - 1;
+					const uint32_t mp = m_pos[j];
+					const int dx = lx + 16 - (int)(mp & 255u);
+					const int dy = ly + 16 - (int)((mp >> 8) & 255u);
+					const int dt = (int)((mp >> 16) & 255u);
+					const int sidx = dy * sPx + dx;
+					aw = aw + m_w[j] * swin[dt * SL + sidx];
+#pragma unroll
+					for (int c = 0; c < CC; c++) an[c] = an[c] + stage[j * RS + c * SL + sidx];
+				}
+				__syncthreads();
+			}
+		}
+		for (int p = lane; p < npos; p += 64) pcur[p] = pub[p];
+		__syncthreads();
+		left -= K;
+		lo = hi;
+	}
+	if (lane == 0 && nrange > 1) atomicAdd(nsplit, 1u);
+	if (valid) {
+		wsum[pix] = aw;
+#pragma unroll
+		for (int c = 0; c < CC; c++) num[(size_t)t0 * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x] = an[c];
+	}
+}
+
+#define AGG2_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	const float *, const float *, const float *, StepConst, int, float *, float *, uint32_t *
+template __global__ void agg_chunk_kernel<1>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<3>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<4>(AGG2_ARGS);
 
 #define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \

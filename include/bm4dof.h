@@ -107,6 +107,12 @@ const char *bm4dof_global_error(void);
  *                  group shape allows it (one channel, 64-sample patches,
  *                  at most 32 neighbours, no orderInvariance); 0: the
  *                  general LDS filter for every group.
+ *   "fast_bm"      1 (default): block matching by one wave per group on
+ *                  LDS-staged search windows (patches 4x4x4 or 8x8x1, one
+ *                  channel in the distance); 0: the general kernel.
+ *   "agg_chunked"  1 (default): ordered aggregation by sorted row chunks
+ *                  with per-pixel masks (tiles at least 8 pixels wide and
+ *                  high); 0: the banded aggregation for every block.
  * Returns BM4DOF_EINVAL for an unknown key.
  */
 int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value);

@@ -116,6 +116,48 @@ def test_register_and_lds_filters_agree(ctx, px, pt, np_, frames):
         ctx.set_option("no_such_option", 1)
 
 
+@pytest.mark.parametrize("w,h,f,nparts,px,pt", [(96, 64, 8, 4, 4, 4), (64, 48, 6, 2, 4, 4),
+                                                 (80, 72, 1, 4, 8, 1), (50, 38, 7, 6, 4, 4)])
+def test_chunked_and_banded_aggregation_agree(ctx, w, h, f, nparts, px, pt):
+    """The chunked aggregation (per-pixel masks of sorted row chunks) and the
+    banded one fold the same rows in the same order: bitwise equal output,
+    also where a pixel block straddles four tiles (nparts=4: 2 x 2 tiles)."""
+    clean, noisy = _video(w, h, f, seed=w * h + f)
+    ff = bf = None
+    if f > 1:
+        ff, bf = _smooth_flow(f, h, w, 5), _smooth_flow(f, h, w, 6)
+    a1 = A.cli_params(1, 20, noisy.shape, px=px, pt=pt, wx=9, np_=16)
+    a2 = A.cli_params(2, 20, noisy.shape, px=px, pt=pt, wx=9, np_=16)
+    try:
+        ctx.set_option("agg_chunked", 0)
+        b0, f0, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=nparts)
+    finally:
+        ctx.set_option("agg_chunked", 1)
+    b1, f1, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=nparts)
+    assert np.array_equal(b0, b1) and np.array_equal(f0, f1)
+
+
+@pytest.mark.parametrize("px,pt,flow,nparts", [(4, 4, True, 4), (8, 1, False, 2), (4, 4, False, 6)])
+def test_staged_and_general_block_matching_agree(ctx, px, pt, flow, nparts):
+    """Neighbour lists of the staged one-wave block matching equal those of
+    the general kernel, in both steps."""
+    f = 8 if pt > 1 else 1
+    clean, noisy = _video(72, 56, f, seed=px + 7 * nparts)
+    ff = bf = None
+    if flow:
+        ff, bf = _smooth_flow(f, 56, 72, 7), _smooth_flow(f, 56, 72, 8)
+    for step, img in ((1, noisy), (2, clean)):
+        a = A.cli_params(step, 20, noisy.shape, px=px, pt=pt, wx=11, np_=32)
+        try:
+            ctx.set_option("fast_bm", 0)
+            r0 = ctx.bm(a, img, ff, bf, nparts=nparts)
+        finally:
+            ctx.set_option("fast_bm", 1)
+        r1 = ctx.bm(a, img, ff, bf, nparts=nparts)
+        for u, v in zip(r0, r1):
+            assert np.array_equal(u, v), step
+
+
 def test_image_config0_bior_and_kaiser(ctx):
     """Single grayscale image, px=8 (bior1.5 + Kaiser window in step 1, DCT in
     step 2), np=16, 21x21 search, beta1 = 3."""
