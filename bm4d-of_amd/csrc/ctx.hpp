@@ -75,6 +75,7 @@ struct bm4dof_ctx {
 	bool fast_filter = true;   // register-resident filter where the group shape allows it
 	bool fast_bm = true;       // staged one-wave block matching where the patch shape allows it
 	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
+	int agg_cap = 1024;        // rows per e-range of the chunked aggregation (one channel)
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

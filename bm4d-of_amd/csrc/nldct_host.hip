@@ -51,7 +51,7 @@ __global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, cons
 template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
 #define AGG2_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, int, float *, float *, uint32_t *
-template <int CC> __global__ void agg_chunk_kernel(AGG2_ARGS);
+template <int CC, int CAP> __global__ void agg_chunk_kernel(AGG2_ARGS);
 __global__ void band_bounds_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
 __global__ void normalize_kernel(const float *, const float *, const float *, StepConst,
                                  float *);
@@ -493,15 +493,15 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 	// chunked kernel: a block then spans at most 2 x 2 tile interiors
 	const bool chunked = ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8;
 	if (chunked) {
-		if (sc.C == 1)
-			hipLaunchKernelGGL(agg_chunk_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile,
-			                   ew, est, win, sc, t0c, num, wsum, d_nsplit);
-		else if (sc.C == 3)
-			hipLaunchKernelGGL(agg_chunk_kernel<3>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile,
-			                   ew, est, win, sc, t0c, num, wsum, d_nsplit);
-		else
-			hipLaunchKernelGGL(agg_chunk_kernel<4>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile,
-			                   ew, est, win, sc, t0c, num, wsum, d_nsplit);
+		// rows held in LDS per e-range: 512 / 768 / 1024 / 1536 (one channel)
+		auto kern = agg_chunk_kernel<1, 1024>;
+		if (sc.C == 1 && ctx->agg_cap == 512) kern = agg_chunk_kernel<1, 512>;
+		else if (sc.C == 1 && ctx->agg_cap == 768) kern = agg_chunk_kernel<1, 768>;
+		else if (sc.C == 1 && ctx->agg_cap == 1536) kern = agg_chunk_kernel<1, 1536>;
+		else if (sc.C == 3) kern = agg_chunk_kernel<3, 1024>;
+		else if (sc.C == 4) kern = agg_chunk_kernel<4, 1024>;
+		hipLaunchKernelGGL(kern, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew, est, win, sc,
+		                   t0c, num, wsum, d_nsplit);
 	} else if (sc.C == 1)
 		hipLaunchKernelGGL(agg_exact_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
 		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
@@ -898,6 +898,11 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
 		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
 		else if (k == "fast_bm") ctx->fast_bm = value != 0;
+		else if (k == "agg_cap") {
+			if (value != 512 && value != 768 && value != 1024 && value != 1536)
+				throw ArgError("agg_cap must be 512, 768, 1024 or 1536");
+			ctx->agg_cap = (int)value;
+		}
 		else throw ArgError("unknown option '" + k + "'");
 	});
 }

@@ -1492,8 +1492,8 @@ void agg_exact_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 // computeAggregationStep{1,2} (nldct.cpp:2284-2300, 2376-2393).
 // ---------------------------------------------------------------------------
 
-#define AGG2_CAP 1536    // rows of one e-range held in LDS
 #define AGG2_STAGE 1024  // floats of staged row slices per chunk
+#define AGG2_SVN 16      // staged floats per lane in flight
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
@@ -1537,6 +1537,17 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 	return v;
 }
 
+// n / d for n, d < 2^32 from a float reciprocal estimate and exact
+// integer correction.
+__device__ __forceinline__ uint32_t udiv_rcp(uint32_t n, uint32_t d, float rcp)
+{
+	uint32_t q = (uint32_t)fmaxf(0.f, (float)n * rcp);
+	int64_t r = (int64_t)n - (int64_t)q * d;
+	while (r < 0) { q--; r += d; }
+	while (r >= (int64_t)d) { q++; r -= d; }
+	return q;
+}
+
 // bits [a, b) of an 8-bit lane row, a and b clamped to [0, 8]
 __device__ __forceinline__ uint32_t span8(int a, int b)
 {
@@ -1544,7 +1555,7 @@ __device__ __forceinline__ uint32_t span8(int a, int b)
 	return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
 }
 
-template <int CC>
+template <int CC, int CAP>
 __global__ __launch_bounds__(64)
 void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__restrict__ svals,
                       const uint32_t *__restrict__ epos, const uint32_t *__restrict__ etile,
@@ -1552,15 +1563,20 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
                       const float *__restrict__ win, StepConst sc, int tframe0,
                       float *__restrict__ num, float *__restrict__ wsum, uint32_t *__restrict__ nsplit)
 {
-	__shared__ uint32_t keysA[AGG2_CAP];
-	__shared__ uint32_t keysB[AGG2_CAP];
-	__shared__ uint32_t hist[256];
-	__shared__ uint32_t pcur[AGG_MAXPOS], pend[AGG_MAXPOS], pub[AGG_MAXPOS];
-	__shared__ float stage[AGG2_STAGE];
+	__shared__ uint32_t keysA[CAP];
+	// the sort's second buffer and histogram share space with the staged slices
+	constexpr int USZ = (CAP + 256) > AGG2_STAGE ? (CAP + 256) : AGG2_STAGE;
+	__shared__ __align__(16) uint32_t ubuf[USZ];
+	uint32_t *keysB = ubuf, *hist = ubuf + CAP;
+	float *stage = reinterpret_cast<float *>(ubuf);
+	__shared__ uint32_t pcur[AGG_MAXPOS], pend[AGG_MAXPOS];
+	__shared__ uint16_t pcnt[AGG_MAXPOS];   // rows of the current e-range (<= CAP)
 	__shared__ float swin[256];
-	__shared__ uint32_t m_pos[64];   // (dx0 + 16) | (dy0 + 16) << 8 | dt << 16
-	__shared__ uint32_t m_e[64];
-	__shared__ float m_w[64];
+	__shared__ uint32_t m_pos2[2][64];   // (dx0 + 16) | (dy0 + 16) << 8 | dt << 16
+	__shared__ uint32_t m_e2[2][64];
+	__shared__ float m_w2[2][64];
+	uint32_t *m_pos = m_pos2[0], *m_e = m_e2[0];
+	float *m_w = m_w2[0];
 	const int lane = threadIdx.x;
 	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
 	const int SL = sPx * sPx;
@@ -1622,16 +1638,21 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	}
 	for (int i = lane; i < P; i += 64) swin[i] = win[i];
 	const int RS = SL * CC;                       // floats of one staged row slice
-	const int NBR = min(64, AGG2_STAGE / RS);     // rows per chunk
+	const int rpp = RS <= 64 ? 64 / RS : 1;       // slices staged per pass
+	const int NBR = min(min(64, AGG2_STAGE / RS), AGG2_SVN * rpp);   // rows per chunk
+	const int jl = RS <= 64 ? lane / RS : 0, rl = RS <= 64 ? lane - jl * RS : 0;
+	const bool stage_lane = jl < rpp;
+	const int cl = rl / SL, sl = rl - cl * SL;
+	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
 	__syncthreads();
 
 	uint32_t left = ksum, lo = emin;
 	int nrange = 0;
 	while (left > 0) {
-		// e-range [lo, hi) with at most AGG2_CAP rows
+		// e-range [lo, hi) with at most CAP rows
 		uint64_t hi64 = (uint64_t)emax + 1;
-		if (left > AGG2_CAP)
-			hi64 = lo + max<uint64_t>(1, ((uint64_t)emax + 1 - lo) * (AGG2_CAP * 3 / 4) / left);
+		if (left > CAP)
+			hi64 = lo + max<uint64_t>(1, ((uint64_t)emax + 1 - lo) * (CAP * 3 / 4) / left);
 		uint32_t hi = (uint32_t)hi64;
 		uint32_t K = 0;
 		for (;;) {
@@ -1643,11 +1664,11 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 					const uint32_t mid = (l + r) >> 1;
 					if (svals[mid] < hi) l = mid + 1; else r = mid;
 				}
-				pub[p] = l;
+				pcnt[p] = (uint16_t)min<uint32_t>(l - a, 65535u);
 				kl += l - a;
 			}
 			K = wave_sum_u32(kl);
-			if (K <= AGG2_CAP) break;
+			if (K <= CAP) break;
 			hi = lo + (hi - lo + 1) / 2;
 		}
 		nrange++;
@@ -1657,11 +1678,11 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 			uint32_t base = 0;
 			for (int p0 = 0; p0 < npos; p0 += 64) {
 				const int p = p0 + lane;
-				const uint32_t len = p < npos ? pub[p] - pcur[p] : 0u;
+				const uint32_t len = p < npos ? (uint32_t)pcnt[p] : 0u;
 				const int incl = wave_incl_scan((int)len);
 				uint32_t o = base + (uint32_t)incl - len;
 				if (p < npos)
-					for (uint32_t i = pcur[p]; i < pub[p]; i++) keysA[o++] = svals[i] - lo;
+					for (uint32_t i = pcur[p], ie = pcur[p] + len; i < ie; i++) keysA[o++] = svals[i] - lo;
 				base += (uint32_t)__shfl(incl, 63, 64);
 			}
 			__syncthreads();
@@ -1669,13 +1690,115 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 			while (nbits < 32 && (1ull << nbits) < (uint64_t)(hi - lo)) nbits++;
 			const uint32_t *sorted = wave_radix_sort(keysA, keysB, (int)K, nbits, hist);
 			__syncthreads();
+			if (sorted != keysA) {
+				// keysB shares space with the staged slices: move the order back
+				for (uint32_t k = lane; k < K; k += 64) keysA[k] = keysB[k];
+				__syncthreads();
+				sorted = keysA;
+			}
+			if (RS <= 64) {
+				// Software-pipelined chunks: the row metadata of chunk c+2 and
+				// the slices of chunk c+1 are in flight while chunk c is folded.
+				const int nch = (int)((K + NBR - 1) / NBR);
+				uint32_t eN = 0, qN = 0, tN = 0;
+				float wN = 0.f;
+				auto meta_load = [&](int c) {
+					const int nbc = c < nch ? (int)min<uint32_t>(NBR, K - (uint32_t)c * NBR) : 0;
+					if (lane < nbc) {
+						eN = sorted[(uint32_t)c * NBR + lane] + lo;
+						qN = epos[eN];
+						tN = etile[eN];
+						wN = entry_w[eN];
+					}
+				};
+				auto meta_commit = [&](int c) -> uint64_t {
+					const int nbc = (int)min<uint32_t>(NBR, K - (uint32_t)c * NBR);
+					const int buf = c & 1;
+					uint64_t mrow = 0;
+					if (lane < nbc) {
+						const int qt = (int)udiv_rcp(qN, (uint32_t)sc.WH, rcpWH);
+						const uint32_t rr = qN - (uint32_t)qt * (uint32_t)sc.WH;
+						const int qy = (int)udiv_rcp(rr, (uint32_t)sc.W, rcpW);
+						const int qx = (int)(rr - (uint32_t)qy * (uint32_t)sc.W);
+						const int dx0 = qx - x0, dy0 = qy - y0, dt = t0 - qt;
+						const uint32_t cols = span8(dx0, dx0 + sPx), rows = span8(dy0, dy0 + sPx);
+						uint64_t rsel = 0;
+#pragma unroll
+						for (int r = 0; r < 8; r++) if ((rows >> r) & 1u) rsel |= 1ull << (8 * r);
+						mrow = (uint64_t)cols * rsel;
+						uint64_t om = 0;
+#pragma unroll
+						for (int i = 0; i < 4; i++) if (own_t[i] == (int)tN) om = own_m[i];
+						mrow &= om;
+						m_pos2[buf][lane] = (uint32_t)(dx0 + 16) | ((uint32_t)(dy0 + 16) << 8) | ((uint32_t)dt << 16);
+						m_e2[buf][lane] = eN;
+						m_w2[buf][lane] = wN;
+					}
+					return mrow;
+				};
+				float sv[AGG2_SVN];
+				auto stage_issue = [&](int c) {
+					const int nbc = (int)min<uint32_t>(NBR, K - (uint32_t)c * NBR);
+					const int buf = c & 1;
+#pragma unroll
+					for (int i = 0; i < AGG2_SVN; i++) {
+						const int j = jl + i * rpp;
+						if (stage_lane && j < nbc) {
+							const int dt = (int)((m_pos2[buf][j] >> 16) & 255u);
+							sv[i] = est[((size_t)m_e2[buf][j] * CC + cl) * P + dt * SL + sl];
+						}
+					}
+				};
+				auto stage_write = [&](int c) {
+					const int nbc = (int)min<uint32_t>(NBR, K - (uint32_t)c * NBR);
+#pragma unroll
+					for (int i = 0; i < AGG2_SVN; i++) {
+						const int j = jl + i * rpp;
+						if (stage_lane && j < nbc) stage[j * RS + rl] = sv[i];
+					}
+				};
+				meta_load(0);
+				uint64_t mcur = meta_commit(0);
+				meta_load(1);
+				__syncthreads();
+				stage_issue(0);
+				stage_write(0);
+				for (int c = 0; c < nch; c++) {
+					uint64_t mnext = 0;
+					if (c + 1 < nch) mnext = meta_commit(c + 1);
+					meta_load(c + 2);
+					__syncthreads();
+					if (c + 1 < nch) stage_issue(c + 1);
+					const int buf = c & 1;
+					uint64_t mine = transpose_bits64(mcur);
+					while (mine) {
+						const int j = __ffsll((unsigned long long)mine) - 1;
+						mine &= mine - 1;
+						const uint32_t mp = m_pos2[buf][j];
+						const int dx = lx + 16 - (int)(mp & 255u);
+						const int dy = ly + 16 - (int)((mp >> 8) & 255u);
+						const int dt = (int)((mp >> 16) & 255u);
+						const int sidx = dy * sPx + dx;
+						aw = aw + m_w2[buf][j] * swin[dt * SL + sidx];
+#pragma unroll
+						for (int cc = 0; cc < CC; cc++) an[cc] = an[cc] + stage[j * RS + cc * SL + sidx];
+					}
+					__syncthreads();
+					if (c + 1 < nch) stage_write(c + 1);
+					mcur = mnext;
+				}
+				__syncthreads();
+			} else
 			for (uint32_t k0 = 0; k0 < K; k0 += NBR) {
 				const int nb = (int)min<uint32_t>(NBR, K - k0);
 				uint64_t mrow = 0;
 				if (lane < nb) {
 					const uint32_t e = sorted[k0 + lane] + lo;
-					int qx, qy, qt;
-					pos_decode(epos[e], sc.W, sc.WH, qx, qy, qt);
+					const uint32_t q = epos[e];
+					const int qt = (int)udiv_rcp(q, (uint32_t)sc.WH, rcpWH);
+					const uint32_t rr = q - (uint32_t)qt * (uint32_t)sc.WH;
+					const int qy = (int)udiv_rcp(rr, (uint32_t)sc.W, rcpW);
+					const int qx = (int)(rr - (uint32_t)qy * (uint32_t)sc.W);
 					const int dx0 = qx - x0, dy0 = qy - y0, dt = t0 - qt;
 					const uint32_t cols = span8(dx0, dx0 + sPx), rows = span8(dy0, dy0 + sPx);
 					uint64_t rsel = 0;
@@ -1692,12 +1815,21 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 					m_w[lane] = entry_w[e];
 				}
 				__syncthreads();
-				// stage the rows' slices at frame t0
-				for (int i = lane; i < nb * RS; i += 64) {
-					const int j = i / RS, r = i - j * RS;
-					const int c = r / SL, sidx = r - c * SL;
-					const int dt = (int)((m_pos[j] >> 16) & 255u);
-					stage[i] = est[((size_t)m_e[j] * CC + c) * P + dt * SL + sidx];
+				// stage the rows' slices at frame t0 (lane -> fixed float of a
+				// slice when a slice fits the wave)
+				if (RS <= 64) {
+					for (int j = jl; j < nb; j += rpp)
+						if (stage_lane) {
+							const int dt = (int)((m_pos[j] >> 16) & 255u);
+							stage[j * RS + rl] = est[((size_t)m_e[j] * CC + cl) * P + dt * SL + sl];
+						}
+				} else {
+					for (int i = lane; i < nb * RS; i += 64) {
+						const int j = i / RS, r = i - j * RS;
+						const int c = r / SL, sidx = r - c * SL;
+						const int dt = (int)((m_pos[j] >> 16) & 255u);
+						stage[i] = est[((size_t)m_e[j] * CC + c) * P + dt * SL + sidx];
+					}
 				}
 				__syncthreads();
 				uint64_t mine = transpose_bits64(mrow);
@@ -1716,7 +1848,7 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 				__syncthreads();
 			}
 		}
-		for (int p = lane; p < npos; p += 64) pcur[p] = pub[p];
+		for (int p = lane; p < npos; p += 64) pcur[p] += pcnt[p];
 		__syncthreads();
 		left -= K;
 		lo = hi;
@@ -1731,9 +1863,12 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 
 #define AGG2_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, int, float *, float *, uint32_t *
-template __global__ void agg_chunk_kernel<1>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<3>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<4>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 512>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 768>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 1024>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 1536>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<3, 1024>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<4, 1024>(AGG2_ARGS);
 
 #define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \

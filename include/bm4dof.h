@@ -113,6 +113,8 @@ const char *bm4dof_global_error(void);
  *   "agg_chunked"  1 (default): ordered aggregation by sorted row chunks
  *                  with per-pixel masks (tiles at least 8 pixels wide and
  *                  high); 0: the banded aggregation for every block.
+ *   "agg_cap"      rows sorted per step of the chunked aggregation, for one
+ *                  channel: 512, 768, 1024 (default) or 1536.
  * Returns BM4DOF_EINVAL for an unknown key.
  */
 int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value);
