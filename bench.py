@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--cpu-crop", default="480x270", help="crop WxH of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the denoiser "
@@ -116,6 +118,17 @@ def main():
 
     import numpy as np
     import torch
+    if args.copy_calibration:
+        # a copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch scale
+        # the counters of the denoiser's kernels to bytes
+        n = 1 << 28
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
     A = _load("bm4dof_amd", os.path.join("bm4d-of_amd", "__init__.py"))
     S = _load("bm4dof_amd.synth", os.path.join("bm4d-of_amd", "synth.py"))
     A.lib()

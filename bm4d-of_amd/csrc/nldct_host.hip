@@ -493,6 +493,7 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 	// chunked kernel: a block then spans at most 2 x 2 tile interiors
 	const bool chunked = ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8;
 	if (chunked) {
+		const dim3 cgrid(t1c - t0c, (sc.W + 7) / 8, (sc.H + 7) / 8);
 		// rows held in LDS per e-range: 512 / 768 / 1024 / 1536 (one channel)
 		auto kern = agg_chunk_kernel<1, 1024>;
 		if (sc.C == 1 && ctx->agg_cap == 512) kern = agg_chunk_kernel<1, 512>;
@@ -500,7 +501,7 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 		else if (sc.C == 1 && ctx->agg_cap == 1536) kern = agg_chunk_kernel<1, 1536>;
 		else if (sc.C == 3) kern = agg_chunk_kernel<3, 1024>;
 		else if (sc.C == 4) kern = agg_chunk_kernel<4, 1024>;
-		hipLaunchKernelGGL(kern, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew, est, win, sc,
+		hipLaunchKernelGGL(kern, cgrid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew, est, win, sc,
 		                   t0c, num, wsum, d_nsplit);
 	} else if (sc.C == 1)
 		hipLaunchKernelGGL(agg_exact_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,

@@ -1580,7 +1580,9 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	const int lane = threadIdx.x;
 	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
 	const int SL = sPx * sPx;
-	const int x0 = (int)blockIdx.x * 8, y0 = (int)blockIdx.y * 8, t0 = tframe0 + (int)blockIdx.z;
+	// grid (frames, x blocks, y blocks): the blocks of consecutive frames at
+	// one place run together, so a row's frame slices are reused from L2
+	const int x0 = (int)blockIdx.y * 8, y0 = (int)blockIdx.z * 8, t0 = tframe0 + (int)blockIdx.x;
 	const int qx0 = max(0, x0 - sPx + 1), qx1 = min(x0 + 7, sc.W - sPx);
 	const int qy0 = max(0, y0 - sPx + 1), qy1 = min(y0 + 7, sc.H - sPx);
 	const int qt0 = max(0, t0 - sPt + 1), qt1 = min(t0, sc.F - sPt);
