@@ -63,12 +63,14 @@ def _stage_flops(stats, P=64, C=1):
 def _traffic(kernel):
     """HBM bytes per launch of `kernel`, as committed from rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE runs scaled by a copy-kernel calibration
-    (profiles/traffic.txt: lines `kernel fetch_bytes write_bytes`)."""
+    (profiles/traffic.txt: tab-separated lines `kernel fetch_bytes write_bytes`)."""
     path = os.path.join(REPO, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        f = line.split()
+        if line.startswith("#") or "\t" not in line:
+            continue
+        f = line.rstrip("\n").split("\t")
         if len(f) == 3 and f[0] == kernel:
             return float(f[1]) + float(f[2])
     return None
@@ -181,9 +183,10 @@ def main():
     final = d_final.cpu().numpy()
 
     # dominant kernel among the transform/filter, matching and aggregation stages
-    kern = {"s1_bm": "bm_kernel", "s2_bm": "bm_kernel", "s1_filter": "filter1_kernel",
-            "s2_filter": "filter2_kernel", "s1_aggregate": "agg_exact_kernel",
-            "s2_aggregate": "agg_exact_kernel"}
+    # kernels this workload runs (one channel, 4x4x4 patches, 32 neighbours)
+    kern = {"s1_bm": "bm_fast_kernel<4, 4>", "s2_bm": "bm_fast_kernel<4, 4>",
+            "s1_filter": "filter1_fast_kernel<4>", "s2_filter": "filter2_fast_kernel<4>",
+            "s1_aggregate": "agg_chunk_kernel<1, 1024>", "s2_aggregate": "agg_chunk_kernel<1, 1024>"}
     dom = max((k for k in stage_ms if k in kern), key=lambda k: stage_ms[k])
     flops = _stage_flops(stats)
     avg_ms = stage_ms[dom] / launches[dom]
