@@ -49,8 +49,9 @@ __global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, cons
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
 	float *, uint32_t *
 template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
-#define AGG2_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
-	const float *, const float *, const float *, StepConst, int, float *, float *, uint32_t *
+#define AGG2_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, \
+	StepConst, int, int, float *, float *, uint32_t *
+__global__ void pack_rows_kernel(const uint32_t *, const uint32_t *, const float *, int64_t, uint4 *);
 template <int CC, int CAP> __global__ void agg_chunk_kernel(AGG2_ARGS);
 __global__ void band_bounds_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
 __global__ void normalize_kernel(const float *, const float *, const float *, StepConst,
@@ -492,8 +493,14 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 	const dim3 grid((sc.W + 7) / 8, (sc.H + 7) / 8, t1c - t0c);
 	// chunked kernel: a block then spans at most 2 x 2 tile interiors
 	const bool chunked = ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8;
-	if (chunked) {
-		const dim3 cgrid(t1c - t0c, (sc.W + 7) / 8, (sc.H + 7) / 8);
+	if (M <= 0 || t1c <= t0c) {
+		// no rows in this chunk: nothing to add
+	} else if (chunked) {
+		uint4 *d_erec = ctx->erec.as<uint4>((size_t)std::max<int64_t>(M, 1));
+		hipLaunchKernelGGL(pack_rows_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, epos, etile, ew, M,
+		                   d_erec);
+		HIPCHK(hipGetLastError());
+		const dim3 cgrid((unsigned)(((size_t)(t1c - t0c) * ((sc.W + 7) / 8) * ((sc.H + 7) / 8) + 7) / 8 * 8));
 		// rows held in LDS per e-range: 512 / 768 / 1024 / 1536 (one channel)
 		auto kern = agg_chunk_kernel<1, 1024>;
 		if (sc.C == 1 && ctx->agg_cap == 512) kern = agg_chunk_kernel<1, 512>;
@@ -501,8 +508,8 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 		else if (sc.C == 1 && ctx->agg_cap == 1536) kern = agg_chunk_kernel<1, 1536>;
 		else if (sc.C == 3) kern = agg_chunk_kernel<3, 1024>;
 		else if (sc.C == 4) kern = agg_chunk_kernel<4, 1024>;
-		hipLaunchKernelGGL(kern, cgrid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew, est, win, sc,
-		                   t0c, num, wsum, d_nsplit);
+		hipLaunchKernelGGL(kern, cgrid, dim3(64), 0, ctx->stream, kstart, svals, d_erec, est, win, sc, t0c,
+		                   t1c - t0c, num, wsum, d_nsplit);
 	} else if (sc.C == 1)
 		hipLaunchKernelGGL(agg_exact_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
 		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
@@ -878,7 +885,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta};
+	                  &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec};
 	for (DevBuf *b : bufs) b->release();
 	DevBuf *tvb[] = {&ctx->tv_frames, &ctx->tv_pairs, &ctx->tv_part, &ctx->tv_mm, &ctx->tv_state,
 	                 &ctx->tv_work, &ctx->tv_pyr, &ctx->tv_taps};

@@ -1193,6 +1193,15 @@ __global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint
 	vals[e] = e;
 }
 
+// erec[e] = (position, tile, weight bits, 0): the row metadata the chunked
+// aggregation reads, one 16-byte record per row.
+__global__ void pack_rows_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ etile,
+                                 const float *__restrict__ entry_w, int64_t M, uint4 *__restrict__ erec)
+{
+	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (e < M) erec[e] = make_uint4(epos[e], etile[e], __float_as_uint(entry_w[e]), 0u);
+}
+
 // Number of rows per patch position.
 __global__ void count_positions_kernel(const uint32_t *__restrict__ keys, int64_t M,
                                        uint32_t *__restrict__ cnt)
@@ -1558,9 +1567,8 @@ __device__ __forceinline__ uint32_t span8(int a, int b)
 template <int CC, int CAP>
 __global__ __launch_bounds__(64)
 void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__restrict__ svals,
-                      const uint32_t *__restrict__ epos, const uint32_t *__restrict__ etile,
-                      const float *__restrict__ entry_w, const float *__restrict__ est,
-                      const float *__restrict__ win, StepConst sc, int tframe0,
+                      const uint4 *__restrict__ erec, const float *__restrict__ est,
+                      const float *__restrict__ win, StepConst sc, int tframe0, int nframes,
                       float *__restrict__ num, float *__restrict__ wsum, uint32_t *__restrict__ nsplit)
 {
 	__shared__ uint32_t keysA[CAP];
@@ -1580,9 +1588,18 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	const int lane = threadIdx.x;
 	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
 	const int SL = sPx * sPx;
-	// grid (frames, x blocks, y blocks): the blocks of consecutive frames at
-	// one place run together, so a row's frame slices are reused from L2
-	const int x0 = (int)blockIdx.y * 8, y0 = (int)blockIdx.z * 8, t0 = tframe0 + (int)blockIdx.x;
+	// One-dimensional grid.  Work items are ordered frame-fastest, then x,
+	// then y, and each XCD (workgroups are dealt to the 8 XCDs in turn) takes
+	// one contiguous eighth of them: the blocks that share rows (the frames
+	// of one place, horizontal neighbours) run together on one L2.
+	const int nbx = (sc.W + 7) / 8, nby = (sc.H + 7) / 8;
+	const uint32_t nwork = (uint32_t)nframes * nbx * nby;
+	const uint32_t per_xcd = (nwork + 7) / 8;
+	const uint32_t wk = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+	if (wk >= nwork) return;
+	const int bt = (int)(wk % (uint32_t)nframes);
+	const uint32_t wr = wk / (uint32_t)nframes;
+	const int x0 = (int)(wr % (uint32_t)nbx) * 8, y0 = (int)(wr / (uint32_t)nbx) * 8, t0 = tframe0 + bt;
 	const int qx0 = max(0, x0 - sPx + 1), qx1 = min(x0 + 7, sc.W - sPx);
 	const int qy0 = max(0, y0 - sPx + 1), qy1 = min(y0 + 7, sc.H - sPx);
 	const int qt0 = max(0, t0 - sPt + 1), qt1 = min(t0, sc.F - sPt);
@@ -1708,9 +1725,10 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 					const int nbc = c < nch ? (int)min<uint32_t>(NBR, K - (uint32_t)c * NBR) : 0;
 					if (lane < nbc) {
 						eN = sorted[(uint32_t)c * NBR + lane] + lo;
-						qN = epos[eN];
-						tN = etile[eN];
-						wN = entry_w[eN];
+						const uint4 rec = erec[eN];
+						qN = rec.x;
+						tN = rec.y;
+						wN = __uint_as_float(rec.z);
 					}
 				};
 				auto meta_commit = [&](int c) -> uint64_t {
@@ -1796,7 +1814,8 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 				uint64_t mrow = 0;
 				if (lane < nb) {
 					const uint32_t e = sorted[k0 + lane] + lo;
-					const uint32_t q = epos[e];
+					const uint4 rec = erec[e];
+					const uint32_t q = rec.x;
 					const int qt = (int)udiv_rcp(q, (uint32_t)sc.WH, rcpWH);
 					const uint32_t rr = q - (uint32_t)qt * (uint32_t)sc.WH;
 					const int qy = (int)udiv_rcp(rr, (uint32_t)sc.W, rcpW);
@@ -1807,14 +1826,14 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 #pragma unroll
 					for (int r = 0; r < 8; r++) if ((rows >> r) & 1u) rsel |= 1ull << (8 * r);
 					mrow = (uint64_t)cols * rsel;
-					const int tl = (int)etile[e];
+					const int tl = (int)rec.y;
 					uint64_t om = 0;
 #pragma unroll
 					for (int i = 0; i < 4; i++) if (own_t[i] == tl) om = own_m[i];
 					mrow &= om;
 					m_pos[lane] = (uint32_t)(dx0 + 16) | ((uint32_t)(dy0 + 16) << 8) | ((uint32_t)dt << 16);
 					m_e[lane] = e;
-					m_w[lane] = entry_w[e];
+					m_w[lane] = __uint_as_float(rec.z);
 				}
 				__syncthreads();
 				// stage the rows' slices at frame t0 (lane -> fixed float of a
@@ -1863,8 +1882,8 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	}
 }
 
-#define AGG2_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
-	const float *, const float *, const float *, StepConst, int, float *, float *, uint32_t *
+#define AGG2_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, \
+	StepConst, int, int, float *, float *, uint32_t *
 template __global__ void agg_chunk_kernel<1, 512>(AGG2_ARGS);
 template __global__ void agg_chunk_kernel<1, 768>(AGG2_ARGS);
 template __global__ void agg_chunk_kernel<1, 1024>(AGG2_ARGS);
