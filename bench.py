@@ -11,7 +11,9 @@ replaces is `bm4d-of -px1 4 -pt1 4 -px2 4 -pt2 4 -np1 32 -np2 32 -wx1 11
 One step = step 1 + step 2 (runNlBayes, src/core/nldct.cpp:267-478) over the
 whole video, inputs and flows already resident in HBM (the flows are
 computed once beforehand by the GPU TV-L1; its time is reported apart).
-`value` is Mpix/s = 960*540*20 / seconds per step.
+`value` is Mpix/s = 960*540*20 / seconds per step.  The `learned` entry
+times the learned single pass (shipped network weights) on the same video
+and flows and gives its PSNR against the plain two-step output.
 
 The roofline entry is for the kernel with the largest share of the step,
 from the library's own HIP events on its stream.  The CPU baseline times the
@@ -110,6 +112,7 @@ def main():
     ap.add_argument("--cpu-crop", default="480x270", help="crop WxH of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned single pass")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -222,6 +225,38 @@ def main():
                     "final": round(_psnr(final, clean), 3)},
         "groups": {"step1": stats["groups1"], "step2": stats["groups2"]},
     }
+    # learned single pass (shipped weights): the same video and flows
+    if not args.no_learned:
+        L = _load("bm4dof_amd.learned", os.path.join("bm4d-of_amd", "learned.py"))
+        ctx.mlp_set(L.load())
+        d_learned = torch.empty_like(d_noisy)
+
+        def lstep():
+            return ctx.run_learned_dev(p1, p2, d_noisy.data_ptr(), noisy.shape, d_ff.data_ptr(),
+                                       d_bf.data_ptr(), d_learned.data_ptr(), nparts=args.nparts)
+
+        for _ in range(args.warmup):
+            lstep()
+        torch.cuda.synchronize()
+        lstage = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lstep()
+            for name, ms in ctx.timings():
+                lstage[name] = lstage.get(name, 0.0) + ms
+        torch.cuda.synchronize()
+        lel = time.perf_counter() - t0
+        learned = d_learned.cpu().numpy()
+        lv = W * H * F / (lel / args.steps) / 1e6
+        out["learned"] = {
+            "value": round(lv, 4), "unit": "Mpix/s", "ms_per_step": round(lel / args.steps * 1e3, 3),
+            "speedup_vs_plain": round(lv / value, 3),
+            "psnr_db": round(_psnr(learned, clean), 3),
+            "dpsnr_vs_plain_db": round(_psnr(learned, clean) - out["psnr_db"]["final"], 3),
+            "stages_ms_per_step": {k: round(v / args.steps, 3) for k, v in lstage.items()},
+            "weights": "bm4d-of_amd/weights/gain_mlp_66_128_128_64.f32",
+        }
+        print(f"[bench] learned single pass: {lv:.2f} Mpix/s", file=sys.stderr, flush=True)
     ctx.close()
     if not args.no_cpu:
         cw, ch = (int(v) for v in args.cpu_crop.lower().split("x"))

@@ -67,8 +67,12 @@ EXPORTS = [
     "bm4dof_set_n_similar", "bm4dof_ctx_create", "bm4dof_ctx_destroy", "bm4dof_last_error",
     "bm4dof_global_error", "bm4dof_run", "bm4dof_run_dev", "bm4dof_bm_count", "bm4dof_bm",
     "bm4dof_last_timings", "bm4dof_last_stats", "Dual_TVL1_optic_flow_multiscale", "bm4dof_tvl1",
-    "bm4dof_tvl1_sequence", "bm4dof_ctx_set_option",
+    "bm4dof_tvl1_sequence", "bm4dof_ctx_set_option", "bm4dof_mlp_set", "bm4dof_mlp_get",
+    "bm4dof_mlp_forward", "bm4dof_mlp_grad", "bm4dof_mlp_train", "bm4dof_learned_pairs",
+    "bm4dof_run_learned", "bm4dof_run_learned_dev",
 ]
+
+MLP_NPARAMS = 33344
 
 
 class BM4DOFError(RuntimeError):
@@ -88,6 +92,15 @@ def lib():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}`")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, loaded under another file name).  Loading torch first
+    # lets the dynamic loader bind this library's libamdhip64.so.7 to that
+    # same runtime; the other order would load a second runtime, and torch's
+    # would then find no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.POINTER(Params)
     vp = C.c_void_p
@@ -116,6 +129,18 @@ def lib():
     L.bm4dof_last_timings.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_char_p), C.c_int]
     L.bm4dof_last_stats.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
     L.bm4dof_ctx_set_option.argtypes = [vp, C.c_char_p, C.c_longlong]
+    L.bm4dof_mlp_set.argtypes = [vp, vp]
+    L.bm4dof_mlp_get.argtypes = [vp, vp]
+    L.bm4dof_mlp_forward.argtypes = [vp, vp, C.c_int64, vp, C.c_int]
+    L.bm4dof_mlp_grad.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_float), vp]
+    L.bm4dof_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint, vp]
+    L.bm4dof_learned_pairs.argtypes = [vp, P, vp, vp, C.c_uint, C.c_uint, C.c_uint, vp, vp, C.c_int,
+                                       C.c_int, C.c_float, vp, vp, vp, C.c_int64,
+                                       C.POINTER(C.c_int64)]
+    learned = [vp, P, P, vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, vp, vp, vp, C.c_int, C.c_float,
+               C.POINTER(C.c_float)]
+    L.bm4dof_run_learned.argtypes = learned
+    L.bm4dof_run_learned_dev.argtypes = learned
     if not hasattr(L, "bm4dof_tvl1"):
         _LIB = L
         return L
@@ -172,6 +197,7 @@ class Context:
     """A bm4dof_ctx bound to one HIP device."""
 
     def __init__(self, device=0):
+        self.device = device
         self._L = lib()
         h = C.c_void_p()
         rc = self._L.bm4dof_ctx_create(C.byref(h), device)
@@ -264,6 +290,75 @@ class Context:
                                       idx.ctypes.data, nsim.ctypes.data, ref.ctypes.data,
                                       tile.ctypes.data))
         return idx.reshape(G, npm), nsim, ref, tile
+
+    # ------------------------------------------------------------- learned pass
+
+    def mlp_set(self, theta):
+        theta = np.ascontiguousarray(theta, np.float32)
+        assert theta.size == MLP_NPARAMS
+        self._check(self._L.bm4dof_mlp_set(self.h, theta.ctypes.data))
+
+    def mlp_get(self):
+        theta = np.zeros(MLP_NPARAMS, np.float32)
+        self._check(self._L.bm4dof_mlp_get(self.h, theta.ctypes.data))
+        return theta
+
+    def mlp_forward(self, x):
+        """Gains of feature rows x (B, 66) on the matrix cores -> (B, 64)."""
+        x = np.ascontiguousarray(x, np.float32)
+        g = np.zeros((x.shape[0], 64), np.float32)
+        self._check(self._L.bm4dof_mlp_forward(self.h, x.ctypes.data, x.shape[0], g.ctypes.data, 0))
+        return g
+
+    def mlp_grad(self, x, t):
+        """(loss, gradient) of one batch by the training kernels."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        loss = C.c_float()
+        self._check(self._L.bm4dof_mlp_grad(self.h, x.ctypes.data, t.ctypes.data, x.shape[0],
+                                            C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def mlp_train_dev(self, x_ptr, t_ptr, rows, batch, steps, lr, seed=1):
+        """Adam steps on a device data set; returns the per-step losses."""
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(self._L.bm4dof_mlp_train(self.h, x_ptr, t_ptr, rows, batch, steps, lr, seed,
+                                             losses.ctypes.data))
+        return losses[:steps]
+
+    def learned_pairs_dev(self, p1, noisy_ptr, basic_ptr, shape, ff_ptr, bf_ptr, nparts, stride, beta2,
+                          x_ptr, t_ptr, flag_ptr, max_rows):
+        F, Cc, H, W = shape
+        rows = C.c_int64()
+        self._check(self._L.bm4dof_learned_pairs(self.h, C.byref(p1), noisy_ptr, basic_ptr, W, H, F,
+                                                 ff_ptr, bf_ptr, nparts, stride, beta2, x_ptr, t_ptr,
+                                                 flag_ptr, max_rows, C.byref(rows)))
+        return rows.value
+
+    def run_learned(self, p1, p2, noisy, fflow=None, bflow=None, nparts=2, gvar_thr=-1.0):
+        """Learned single pass on host arrays -> (output, flagged fraction)."""
+        noisy = np.ascontiguousarray(noisy, np.float32)
+        F, Cc, H, W = noisy.shape
+        out = np.zeros_like(noisy)
+        ff = bf = None
+        if fflow is not None:
+            ff = np.ascontiguousarray(fflow, np.float32)
+            bf = np.ascontiguousarray(bflow, np.float32)
+        frac = C.c_float()
+        self._check(self._L.bm4dof_run_learned(
+            self.h, C.byref(p1), C.byref(p2) if p2 is not None else None, noisy.ctypes.data, W, H, F, Cc,
+            ff.ctypes.data if ff is not None else None, bf.ctypes.data if bf is not None else None,
+            out.ctypes.data, nparts, gvar_thr, C.byref(frac)))
+        return out, frac.value
+
+    def run_learned_dev(self, p1, p2, noisy_ptr, shape, ff_ptr, bf_ptr, out_ptr, nparts=2, gvar_thr=-1.0):
+        F, Cc, H, W = shape
+        frac = C.c_float()
+        self._check(self._L.bm4dof_run_learned_dev(
+            self.h, C.byref(p1), C.byref(p2) if p2 is not None else None, noisy_ptr, W, H, F, Cc, ff_ptr,
+            bf_ptr, out_ptr, nparts, gvar_thr, C.byref(frac)))
+        return frac.value
 
     # ------------------------------------------------------------- TV-L1
 

@@ -63,7 +63,7 @@ struct bm4dof_ctx {
 	bm4dof::DevBuf gpos, gtile, tiles, idx, nsim, base, keys, vals, skeys, svals, sort_tmp, lstart, lend,
 	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
 	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
-	    io_final, bm_image, bm_wins, bm_wmeta, erec;
+	    io_final, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx;
 	// timings of the last run
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
@@ -76,6 +76,10 @@ struct bm4dof_ctx {
 	bool fast_bm = true;       // staged one-wave block matching where the patch shape allows it
 	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
 	int agg_cap = 1024;        // rows per e-range of the chunked aggregation (one channel)
+	// learned gain network (mlp.hip)
+	bm4dof::DevBuf mlp_theta, mlp_m, mlp_v, mlp_wq, mlp_inf, mlp_ws, mlp_grad, mlp_xb, mlp_tb, mlp_sse;
+	int mlp_step = 0;
+	bool mlp_ready = false;
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

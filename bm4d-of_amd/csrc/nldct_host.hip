@@ -1,6 +1,7 @@
 // Host side of the MI355X BM4D-OF denoiser: parameters, tiling, reference
 // grid, bases, the per-step pipeline and the C-ABI of include/bm4dof.h.
 #include "ctx.hpp"
+#include "mlp_dev.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -40,6 +41,16 @@ __global__ void filter2_kernel(const float *, const float *, const uint32_t *, c
 	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *
 template <int PX> __global__ void filter1_fast_kernel(FILTER1_ARGS);
 template <int PX> __global__ void filter2_fast_kernel(FILTER2_ARGS);
+#define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	StepConst, int, int, const float *, const float *, const float *, MlpInfer, float, float, float *, \
+	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *
+#define PAIRS_ARGS const float *, const float *, const uint32_t *, const uint32_t *, StepConst, int, int, \
+	const float *, float, float, float *, float *, uint8_t *
+template <int PX> __global__ void filter_learned_kernel(LEARNED_ARGS);
+template <int PX> __global__ void learned_pairs_kernel(PAIRS_ARGS);
+__global__ void fallback_mark_kernel(const uint32_t *, const uint32_t *, const uint8_t *, StepConst, int,
+                                     uint8_t *);
+__global__ void fallback_blend_kernel(const uint8_t *, const float *, int64_t, int, float *);
 __global__ void emit_entries_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int,
                                     int, uint32_t *, uint32_t *);
 __global__ void count_positions_kernel(const uint32_t *, int64_t, uint32_t *);
@@ -534,9 +545,18 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 //   step1: noisy_w -> out_w (basic)
 //   step2: noisy_w, basic_w -> out_w (final)
 // Returns the number of groups processed.
+// The learned single pass runs step 1's grid and matching with the network's
+// gains in place of the threshold.
+struct LearnedArgs {
+	MlpInfer net;
+	float sigma, gvar_thr;
+	uint8_t *gflag;      // per group: gain variance above gvar_thr
+	uint32_t *nflag;     // count of flagged groups
+};
+
 uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
                   const float *noisy_w, const float *basic_w, const float *ff, const float *bf,
-                  float *out_w, int nparts, Timer &tm)
+                  float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr)
 {
 	const bool step1 = p.isFirstStep != 0;
 	const bool use_flow = ff != nullptr && bf != nullptr;
@@ -576,11 +596,11 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 	HIPCHK(hipMemsetAsync(d_cnt, 0, 64, ctx->stream));
 	HIPCHK(hipMemsetAsync(d_num, 0, N_all * 4, ctx->stream));
 	HIPCHK(hipMemsetAsync(d_wsum, 0, WHF * 4, ctx->stream));
-	tm.mark(step1 ? "s1_setup" : "s2_setup");
+	tm.mark(la ? "l_setup" : step1 ? "s1_setup" : "s2_setup");
 
 	run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, d_gpos, d_gtile, d_tiles, G,
 	       d_idx, d_nsim);
-	tm.mark(step1 ? "s1_bm" : "s2_bm");
+	tm.mark(la ? "l_bm" : step1 ? "s1_bm" : "s2_bm");
 
 	// Chunks of consecutive groups bounded by the estimate buffer budget.
 	const size_t row_bytes = (size_t)C * sc.P * 4;
@@ -592,8 +612,13 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 	                              : ((size_t)C * sc.P * np + 2 * (size_t)sc.P * np) * 4;
 	if (filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
 	// register-resident filter: one channel, 64-sample patches, <= 32 neighbours
-	const bool fast = ctx->fast_filter && C == 1 && sc.P == 64 && np <= 32 && !sc.order_inv &&
-	                  ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1));
+	const bool fast_shape = C == 1 && sc.P == 64 && np <= 32 &&
+	                        ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1));
+	const bool fast = ctx->fast_filter && fast_shape && !sc.order_inv;
+	if (la && !fast_shape)
+		throw ArgError("learned pass: one channel, 4x4x4 or 8x8x1 patches and at most 32 neighbours",
+		               BM4DOF_EUNSUPP);
+	uint8_t *d_gflag = la ? ctx->gflag.as<uint8_t>(std::max(G, 1)) : nullptr;
 
 	for (int g0 = 0; g0 < G; g0 += chunk_groups) {
 		const int Gc = std::min(chunk_groups, G - g0);
@@ -639,9 +664,14 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 		HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
 		d_tmp = ctx->sort_tmp.get(tmp_bytes);
 		HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
-		tm.mark(step1 ? "s1_sort" : "s2_sort");
+		tm.mark(la ? "l_sort" : step1 ? "s1_sort" : "s2_sort");
 
-		if (fast && step1)
+		if (la)
+			hipLaunchKernelGGL(sc.sPx == 4 ? filter_learned_kernel<4> : filter_learned_kernel<8>, dim3(Gc),
+			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc,
+			                   d_basis, d_binv, d_win, la->net, la->sigma, la->gvar_thr, d_est, d_ew, d_etile,
+			                   d_early, d_gw, d_gflag, la->nflag);
+		else if (fast && step1)
 			hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
 			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, d_gtile, sc, g0,
 			                   Gc, d_basis, d_binv, d_win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
@@ -659,7 +689,7 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 			                   basic_w, d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc, d_basis, d_binv,
 			                   d_win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
 		HIPCHK(hipGetLastError());
-		tm.mark(step1 ? "s1_filter" : "s2_filter");
+		tm.mark(la ? "l_filter" : step1 ? "s1_filter" : "s2_filter");
 
 		// Early-out groups reuse the previous group's weights (rare).
 		uint32_t n_early = 0;
@@ -715,12 +745,20 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
 			if ((int)gf > g0 && (int)gf < g0 + Gc) band_groups.push_back(gf);
 		aggregate_chunk(ctx, sc, band_groups, g0, M, t0c, t1c, d_base, d_kstart, d_svals, d_epos, d_etile,
 		                d_ew, d_est, d_win, d_num, d_wsum, d_cnt + 4);
-		tm.mark(step1 ? "s1_aggregate" : "s2_aggregate");
+		tm.mark(la ? "l_aggregate" : step1 ? "s1_aggregate" : "s2_aggregate");
 	}
 	hipLaunchKernelGGL(normalize_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream,
 	                   d_num, d_wsum, noisy_w, sc, out_w);
 	HIPCHK(hipGetLastError());
-	tm.mark(step1 ? "s1_normalize" : "s2_normalize");
+	tm.mark(la ? "l_normalize" : step1 ? "s1_normalize" : "s2_normalize");
+	if (la && G > 0) {
+		// pixels under flagged groups, for the plain two-step fallback
+		uint8_t *d_mask = ctx->fmask.as<uint8_t>(WHF);
+		HIPCHK(hipMemsetAsync(d_mask, 0, WHF, ctx->stream));
+		hipLaunchKernelGGL(fallback_mark_kernel, dim3(G), dim3(64), 0, ctx->stream, d_idx, d_nsim, d_gflag, sc,
+		                   G, d_mask);
+		HIPCHK(hipGetLastError());
+	}
 	return (uint64_t)G;
 }
 
@@ -802,6 +840,115 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 		HIPCHK(hipEventElapsedTime(&ms, ctx->ev[i - 1], ctx->ev[i]));
 		ctx->timings[i].second = ms;
 	}
+}
+
+// Learned single pass on device buffers: step-1 grid and matching on the
+// noisy video, network gains, step-2 style weights.  When gvar_thr >= 0,
+// pixels under groups whose gain variance exceeds it take the plain
+// two-step result (p1, p2) instead; *flag_frac receives the fraction of
+// flagged groups.
+void run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, const float *noisy,
+                     int W, int H, int F, int C, const float *ff, const float *bf, float *out, int nparts,
+                     float gvar_thr, float *flag_frac)
+{
+	if (C != 1) throw ArgError("the learned pass takes one-channel video", BM4DOF_EUNSUPP);
+	if (nparts < 1 || nparts > 255) throw ArgError("nparts must be in 1..255");
+	if (W <= 0 || H <= 0 || F <= 0) throw ArgError("empty video");
+	if ((uint64_t)W * H * F >= (1ull << 32)) throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
+	if (!p1->sizePatch) throw ArgError("learned pass: sizePatch of the first-step parameters is zero");
+	HIPCHK(hipSetDevice(ctx->device));
+	for (auto e : ctx->ev) (void)hipEventDestroy(e);
+	ctx->ev.clear();
+	ctx->agg_blocks = ctx->agg_overflow_blocks = 0;
+	ctx->step_groups[0] = ctx->step_groups[1] = ctx->step_rows[0] = ctx->step_rows[1] = 0;
+	ctx->timings.clear();
+	Timer tm(ctx);
+	tm.mark("start");
+	LearnedArgs la;
+	la.net = mlp_net(ctx);
+	la.sigma = p1->sigma;
+	la.gvar_thr = gvar_thr >= 0.f ? gvar_thr : 3.4e38f;
+	la.nflag = ctx->counters.as<uint32_t>(16) + 8;
+	la.gflag = nullptr;
+	HIPCHK(hipMemsetAsync(la.nflag, 0, 4, ctx->stream));
+	bm4dof_params pl = *p1;
+	pl.isFirstStep = 1;
+	const uint64_t G = run_step(ctx, pl, W, H, F, 1, noisy, nullptr, ff, bf, out, nparts, tm, &la);
+	uint32_t nflag = 0;
+	HIPCHK(hipMemcpyAsync(&nflag, la.nflag, 4, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	if (flag_frac) *flag_frac = G ? (float)nflag / (float)G : 0.f;
+	if (gvar_thr >= 0.f && nflag > 0) {
+		if (!p2 || !p2->sizePatch) throw ArgError("fallback needs second-step parameters");
+		const size_t n = (size_t)W * H * F;
+		float *pb = ctx->basic_w.as<float>(n);
+		float *pf = ctx->out_w.as<float>(n);
+		// keep the learned statistics; the plain run appends its own stages
+		const auto ev = ctx->ev;
+		const auto timings = ctx->timings;
+		run_step(ctx, *p1, W, H, F, 1, noisy, nullptr, ff, bf, pb, nparts, tm);
+		run_step(ctx, *p2, W, H, F, 1, noisy, pb, ff, bf, pf, nparts, tm);
+		hipLaunchKernelGGL(fallback_blend_kernel, dim3(nblk((int64_t)n, 256)), dim3(256), 0, ctx->stream,
+		                   ctx->fmask.as<uint8_t>(n), pf, (int64_t)n, 1, out);
+		HIPCHK(hipGetLastError());
+		tm.mark("fallback_blend");
+		(void)ev;
+		(void)timings;
+	}
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	for (size_t i = 1; i < ctx->ev.size(); i++) {
+		float ms = 0.f;
+		HIPCHK(hipEventElapsedTime(&ms, ctx->ev[i - 1], ctx->ev[i]));
+		ctx->timings[i].second = ms;
+	}
+}
+
+// Training pairs of the learned pass: for every `stride`-th group of the
+// step-1 grid (matching on the noisy video), 32 rows of features and of the
+// step-2 Wiener gains of the basic video at the same neighbours.  Device
+// outputs of max_rows rows; rows beyond a group's N are flagged 0.
+int64_t learned_pairs_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const float *noisy, const float *basic,
+                          int W, int H, int F, const float *ff, const float *bf, int nparts, int stride,
+                          float beta2, float *xout, float *tout, uint8_t *rflag, int64_t max_rows)
+{
+	if (stride < 1) throw ArgError("stride must be >= 1");
+	if (nparts < 1 || nparts > 255) throw ArgError("nparts must be in 1..255");
+	HIPCHK(hipSetDevice(ctx->device));
+	bm4dof_params p = *p1;
+	p.isFirstStep = 1;
+	check_step_params(p, 1);
+	const bool use_flow = ff && bf;
+	std::vector<float> hb, hbi, hw;
+	build_bases(p, hb, hbi, hw);
+	StepGeom geo = make_geometry(p, W, H, F, nparts);
+	StepConst sc = make_const(p, W, H, F, 1, use_flow, geo.til);
+	if (!(sc.P == 64 && sc.npmax <= 32 && ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1))))
+		throw ArgError("learned pass: 4x4x4 or 8x8x1 patches and at most 32 neighbours", BM4DOF_EUNSUPP);
+	const int G = (int)geo.gpos.size();
+	const int nslot = (G + stride - 1) / stride;
+	const int64_t rows = (int64_t)nslot * 32;
+	if (rows > max_rows) throw ArgError("output holds fewer rows than the sampled groups need");
+	if (G == 0) return 0;
+	float *d_basis = ctx->basis.as<float>(hb.size());
+	HIPCHK(hipMemcpyAsync(d_basis, hb.data(), hb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+	TileBox *d_tiles = ctx->tiles.as<TileBox>(geo.til.crop.size());
+	HIPCHK(hipMemcpyAsync(d_tiles, geo.til.crop.data(), geo.til.crop.size() * sizeof(TileBox),
+	                      hipMemcpyHostToDevice, ctx->stream));
+	uint32_t *d_gpos = ctx->gpos.as<uint32_t>(G);
+	uint32_t *d_gtile = ctx->gtile.as<uint32_t>(G);
+	HIPCHK(hipMemcpyAsync(d_gpos, geo.gpos.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(hipMemcpyAsync(d_gtile, geo.gtile.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+	uint32_t *d_idx = ctx->idx.as<uint32_t>((size_t)G * sc.npmax);
+	uint32_t *d_nsim = ctx->nsim.as<uint32_t>(G);
+	run_bm(ctx, sc, 1, noisy, use_flow ? ff : nullptr, use_flow ? bf : nullptr, d_gpos, d_gtile, d_tiles, G,
+	       d_idx, d_nsim);
+	const float bs = beta2 * p.sigma;
+	hipLaunchKernelGGL(sc.sPx == 4 ? learned_pairs_kernel<4> : learned_pairs_kernel<8>, dim3(nslot), dim3(64), 0,
+	                   ctx->stream, noisy, basic, d_idx, d_nsim, sc, G, stride, d_basis, p.sigma, bs * bs, xout,
+	                   tout, rflag);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	return rows;
 }
 
 } // namespace
@@ -925,6 +1072,65 @@ int bm4dof_run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params
 	return guarded(ctx, [&] {
 		run_dev(ctx, p1, p2, noisy, (int)w, (int)h, (int)f, (int)c, (ff && bf) ? ff : nullptr,
 		        (ff && bf) ? bf : nullptr, basic, final, nparts, groups_ratio);
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+                           const float *noisy, unsigned w, unsigned h, unsigned f, unsigned c,
+                           const float *ff, const float *bf, float *out, int nparts, float gvar_thr,
+                           float *flag_frac)
+{
+	if (!ctx || !p1 || !noisy || !out) { set_err(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		run_learned_dev(ctx, p1, p2, noisy, (int)w, (int)h, (int)f, (int)c, (ff && bf) ? ff : nullptr,
+		                (ff && bf) ? bf : nullptr, out, nparts, gvar_thr, flag_frac);
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_run_learned(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+                       const float *noisy, unsigned w, unsigned h, unsigned f, unsigned c,
+                       const float *ff, const float *bf, float *out, int nparts, float gvar_thr,
+                       float *flag_frac)
+{
+	if (!ctx || !p1 || !noisy || !out) { set_err(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		const size_t n = (size_t)w * h * f * c;
+		const size_t nf = (size_t)w * h * f * 2;
+		const bool flow = ff && bf;
+		float *dn = ctx->io_noisy.as<float>(n);
+		float *dout = ctx->io_final.as<float>(n);
+		float *dff = flow ? ctx->io_fflow.as<float>(nf) : nullptr;
+		float *dbf = flow ? ctx->io_bflow.as<float>(nf) : nullptr;
+		HIPCHK(hipMemcpyAsync(dn, noisy, n * 4, hipMemcpyHostToDevice, ctx->stream));
+		if (flow) {
+			HIPCHK(hipMemcpyAsync(dff, ff, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+			HIPCHK(hipMemcpyAsync(dbf, bf, nf * 4, hipMemcpyHostToDevice, ctx->stream));
+		}
+		run_learned_dev(ctx, p1, p2, dn, (int)w, (int)h, (int)f, (int)c, dff, dbf, dout, nparts, gvar_thr,
+		                flag_frac);
+		HIPCHK(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_learned_pairs(bm4dof_ctx *ctx, const bm4dof_params *p1, const float *noisy, const float *basic,
+                         unsigned w, unsigned h, unsigned f, const float *ff, const float *bf, int nparts,
+                         int stride, float beta2, float *x_out, float *t_out, unsigned char *row_flag,
+                         int64_t max_rows, int64_t *rows)
+{
+	if (!ctx || !p1 || !noisy || !basic || !x_out || !t_out || !row_flag) {
+		set_err(ctx, "NULL argument");
+		return BM4DOF_EINVAL;
+	}
+	return guarded(ctx, [&] {
+		const int64_t r = learned_pairs_dev(ctx, p1, noisy, basic, (int)w, (int)h, (int)f, (ff && bf) ? ff : nullptr,
+		                                    (ff && bf) ? bf : nullptr, nparts, stride, beta2, x_out, t_out,
+		                                    (uint8_t *)row_flag, max_rows);
+		if (rows) *rows = r;
 	});
 }
 

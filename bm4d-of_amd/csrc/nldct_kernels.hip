@@ -15,6 +15,7 @@
 //     (nldct.cpp:2284-2300, 2376-2393).
 // The GPU output is therefore bitwise equal to the sequential reference.
 #include "common.hpp"
+#include "mlp_dev.hpp"
 
 namespace bm4dof {
 
@@ -1150,6 +1151,207 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 	__syncthreads();
 	fast_inverse(zs, binv, N, w * win[lane], row0, est);
 }
+
+
+// ---------------------------------------------------------------------------
+// Learned single pass (no counterpart in the reference).  Groups and
+// neighbours as step 1 (noisy image); per group row n the network maps the
+// Haar-domain coefficients z[.][n] (features: z/sigma, log(sigma/20) and the
+// row statistic log1p(sum_k z[k][n]^2 / (64 sigma^2))) to 64 gains that replace the
+// step-2 Wiener gains; aggregation weight 1/sigma^2/sum g^2 as in step 2
+// (nldct.cpp:2167).  Groups whose gain variance exceeds `gvar_thr` are
+// flagged (gflag) for the plain two-step fallback.
+// ---------------------------------------------------------------------------
+
+// Haar-domain coefficients of a group into the lane registers z[n] (lane k),
+// and each row's distance statistic: log1p of its energy over 64 sigma^2
+// (for a detail row, the squared differences between the patch sub-means
+// it compares; for the first row, the group mean's energy) into rstat[n].
+template <int PX>
+__device__ __forceinline__ void learned_coeffs(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                               int N, const StepConst &sc, const float *__restrict__ basis,
+                                               float sigma, float *__restrict__ zs, float (&z)[FAST_NMAX],
+                                               float *__restrict__ rstat)
+{
+	const int lane = threadIdx.x;
+	{
+		float brow[FAST_P];
+#pragma unroll
+		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
+		fast_forward<PX>(img, gidx, N, sc.W, sc.WH, brow, zs);
+	}
+	__syncthreads();
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+	haar_lane(z, N, true);
+	__syncthreads();
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) zs[n * FAST_P + lane] = z[n] * z[n];
+	__syncthreads();
+	if (lane < FAST_NMAX) {
+		float e = 0.f;
+		for (int k = 0; k < FAST_P; k++) e += zs[lane * FAST_P + k];
+		rstat[lane] = lane < N ? log1pf(e / (64.f * sigma * sigma)) : 0.f;
+	}
+	__syncthreads();
+}
+
+// bf16 feature rows [32][MLP_XS] from the lane registers (rows >= N zero)
+__device__ __forceinline__ void learned_features_lds(const float (&z)[FAST_NMAX], int N, float sigma,
+                                                     const float *__restrict__ rstat, uint16_t *__restrict__ xb)
+{
+	const int lane = threadIdx.x;
+	const float is = 1.f / sigma;
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) xb[n * MLP_XS + lane] = f32_to_bf16(n < N ? z[n] * is : 0.f);
+	if (lane < 32) {
+		const bool live = lane < N;
+		xb[lane * MLP_XS + 64] = f32_to_bf16(live ? logf(sigma / 20.f) : 0.f);
+		xb[lane * MLP_XS + 65] = f32_to_bf16(rstat[lane]);
+		for (int k = 66; k < MLP_XS; k++) xb[lane * MLP_XS + k] = 0;
+	}
+}
+
+template <int PX>
+__global__ __launch_bounds__(64)
+void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__restrict__ idx,
+                           const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ base,
+                           const uint32_t *__restrict__ gtile, StepConst sc, int g0, int Gc,
+                           const float *__restrict__ basis, const float *__restrict__ binv,
+                           const float *__restrict__ win, MlpInfer net, float sigma, float gvar_thr,
+                           float *__restrict__ est, float *__restrict__ entry_w, uint32_t *__restrict__ etile,
+                           uint8_t *__restrict__ early, float *__restrict__ gweight,
+                           uint8_t *__restrict__ gflag, uint32_t *__restrict__ n_flag)
+{
+	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];   // coefficients, then gains
+	__shared__ __align__(16) uint16_t xb[32 * MLP_XS];
+	__shared__ __align__(16) uint16_t hb[32 * MLP_HS];
+	const int gl = blockIdx.x;
+	if (gl >= Gc) return;
+	const int g = g0 + gl;
+	const int lane = threadIdx.x;
+	const int N = (int)nsim[g];
+	const uint32_t row0 = base[gl];
+	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	if (lane < N) etile[row0 + lane] = gtile[g];
+
+	__shared__ float rstat[FAST_NMAX];
+	float z[FAST_NMAX];
+	learned_coeffs<PX>(img, gidx, N, sc, basis, sigma, zs, z, rstat);
+	learned_features_lds(z, N, sigma, rstat, xb);
+	__syncthreads();
+	mlp_rows32(net, xb, hb, zs, FAST_P);
+	float s2 = 0.f, s1 = 0.f;
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) {
+		if (n < N) {
+			const float gn = zs[n * FAST_P + lane];
+			z[n] = z[n] * gn;
+			s1 += gn;
+			s2 += gn * gn;
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		s1 += __shfl_xor(s1, o, 64);
+		s2 += __shfl_xor(s2, o, 64);
+	}
+	haar_lane(z, N, false);
+	__syncthreads();
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
+	const float cnt = (float)(64 * N);
+	const float gvar = s2 / cnt - (s1 / cnt) * (s1 / cnt);
+	const float w = (float)(1. / ((double)sigma * sigma) / (double)s2);
+	if (lane == 0) {
+		gweight[g] = w;
+		early[g] = 0;
+		const uint8_t f = gvar > gvar_thr ? 1 : 0;
+		gflag[g] = f;
+		if (f) atomicAdd(n_flag, 1u);
+	}
+	if (lane < N) entry_w[row0 + lane] = w;
+	__syncthreads();
+	fast_inverse(zs, binv, N, w * win[lane], row0, est);
+}
+
+// Training pairs: for every `stride`-th group, the features of its rows
+// (noisy image, as filter_learned_kernel) and the step-2 Wiener gains that
+// the basic estimate gives at the same neighbours:
+// y^2 / (y^2 + (beta sigma)^2) (nldct.cpp:2131).  Rows n >= N are flagged 0.
+template <int PX>
+__global__ __launch_bounds__(64)
+void learned_pairs_kernel(const float *__restrict__ noisy, const float *__restrict__ basic,
+                          const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim, StepConst sc,
+                          int G, int stride, const float *__restrict__ basis, float sigma, float beta_sigma2,
+                          float *__restrict__ xout, float *__restrict__ tout, uint8_t *__restrict__ rflag)
+{
+	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
+	const int slot = blockIdx.x;
+	const int g = slot * stride;
+	if (g >= G) return;
+	const int lane = threadIdx.x;
+	const int N = (int)nsim[g];
+	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	__shared__ float rstat[FAST_NMAX];
+	float z[FAST_NMAX];
+	learned_coeffs<PX>(noisy, gidx, N, sc, basis, sigma, zs, z, rstat);
+	const float is = 1.f / sigma, ls = logf(sigma / 20.f);
+	const size_t r0 = (size_t)slot * FAST_NMAX;
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) xout[(r0 + n) * MLP_IN + lane] = n < N ? z[n] * is : 0.f;
+	if (lane < 32) {
+		xout[(r0 + lane) * MLP_IN + 64] = lane < N ? ls : 0.f;
+		xout[(r0 + lane) * MLP_IN + 65] = rstat[lane];
+		rflag[r0 + lane] = lane < N ? 1 : 0;
+	}
+	__syncthreads();
+	float y[FAST_NMAX];
+	learned_coeffs<PX>(basic, gidx, N, sc, basis, sigma, zs, y, rstat);
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) {
+		const float wgt = y[n] * y[n] / (y[n] * y[n] + beta_sigma2);
+		tout[(r0 + n) * MLP_OUT + lane] = n < N ? wgt : 0.f;
+	}
+}
+
+// Pixels covered by the patches of flagged groups (plain two-step fallback
+// of the learned pass); one block per group.
+__global__ void fallback_mark_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
+                                     const uint8_t *__restrict__ gflag, StepConst sc, int G,
+                                     uint8_t *__restrict__ mask)
+{
+	const int g = blockIdx.x;
+	if (g >= G || !gflag[g]) return;
+	const int N = (int)nsim[g];
+	const int sPx = sc.sPx, SL = sPx * sPx;
+	for (int n = 0; n < N; n++) {
+		const uint32_t pos = idx[(size_t)g * sc.npmax + n];
+		for (int p = threadIdx.x; p < sc.P; p += blockDim.x) {
+			const int ht = p / SL, r = p - ht * SL, hy = r / sPx, hx = r - hy * sPx;
+			mask[pos + (uint32_t)ht * sc.WH + (uint32_t)hy * sc.W + hx] = 1;
+		}
+	}
+}
+
+// out = plain where the mask is set (one channel)
+__global__ void fallback_blend_kernel(const uint8_t *__restrict__ mask, const float *__restrict__ plain,
+                                      int64_t n, int unused, float *__restrict__ out)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	(void)unused;
+	if (i < n && mask[i]) out[i] = plain[i];
+}
+
+#define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
+	StepConst, int, int, const float *, const float *, const float *, MlpInfer, float, float, float *, \
+	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *
+#define PAIRS_ARGS const float *, const float *, const uint32_t *, const uint32_t *, StepConst, int, int, \
+	const float *, float, float, float *, float *, uint8_t *
+template __global__ void filter_learned_kernel<4>(LEARNED_ARGS);
+template __global__ void filter_learned_kernel<8>(LEARNED_ARGS);
+template __global__ void learned_pairs_kernel<4>(PAIRS_ARGS);
+template __global__ void learned_pairs_kernel<8>(PAIRS_ARGS);
 
 
 #define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \

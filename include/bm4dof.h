@@ -187,6 +187,61 @@ int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, in
  * stats[7..8] = the same for step 2.  Returns the number of slots filled. */
 int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
 
+/* ---- learned single pass ----
+ *
+ * No counterpart in the reference.  A network of 66 -> 128 -> 128 -> 64
+ * units (ReLU, sigmoid; bfloat16 weights, float32 accumulation on the
+ * matrix cores) maps, for every row of a group of the first step's grid,
+ * the Haar-domain coefficients of the noisy patches (features:
+ * coefficient / sigma, log(sigma / 20), log1p of the row's energy over
+ * 64 sigma^2) to 64 gains that stand in for the second step's
+ * Wiener gains y^2 / (y^2 + (beta sigma)^2) (src/core/nldct.cpp:2131), so
+ * one pass replaces the two.
+ *
+ * theta: BM4DOF_MLP_NPARAMS floats, W1[66][128], b1[128], W2[128][128],
+ * b2[128], W3[128][64], b3[64], each W input-major (W[k][j] at k*cols+j).
+ */
+#define BM4DOF_MLP_NPARAMS 33344
+
+/* Load a network (host theta); resets the Adam state. */
+int bm4dof_mlp_set(bm4dof_ctx *ctx, const float *theta);
+/* Copy the current master parameters to host theta. */
+int bm4dof_mlp_get(const bm4dof_ctx *ctx, float *theta);
+/* Gains of `rows` feature rows (x: rows*66, gains: rows*64), on the matrix
+ * cores; buffers on the host or (on_device != 0) on the device. */
+int bm4dof_mlp_forward(bm4dof_ctx *ctx, const float *x, int64_t rows, float *gains, int on_device);
+/* Mean squared error loss and its gradient (host buffers; grad may be
+ * NULL) of one batch, by the training kernels. */
+int bm4dof_mlp_grad(bm4dof_ctx *ctx, const float *x, const float *t, int rows, float *loss, float *grad);
+/* `steps` Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on batches of
+ * `batch` rows drawn from the device data set x (rows*66), t (rows*64);
+ * batch == rows takes the whole set in order.  losses (host, may be NULL)
+ * receives each step's batch loss before its update. */
+int bm4dof_mlp_train(bm4dof_ctx *ctx, const float *x, const float *t, int64_t rows, int batch, int steps,
+                     float lr, unsigned seed, float *losses);
+/* Training pairs: for every stride-th group of p1's grid (matching on the
+ * noisy video), 32 rows of features (x_out, rows*66) and of the Wiener gains
+ * of `basic` at the same neighbours with factor beta2 (t_out, rows*64);
+ * row_flag marks the rows of real neighbours.  Device buffers of max_rows
+ * rows; *rows receives 32 * (number of sampled groups).  One channel. */
+int bm4dof_learned_pairs(bm4dof_ctx *ctx, const bm4dof_params *p1, const float *noisy, const float *basic,
+                         unsigned w, unsigned h, unsigned f, const float *fflow, const float *bflow,
+                         int nparts, int stride, float beta2, float *x_out, float *t_out,
+                         unsigned char *row_flag, int64_t max_rows, int64_t *rows);
+/* The learned single pass (one channel): p1's grid, matching and patch
+ * size with the network's gains.  With gvar_thr >= 0, pixels under groups
+ * whose predicted gains have a variance above gvar_thr take the plain
+ * two-step result (p1, p2) instead; *flag_frac (may be NULL) receives the
+ * fraction of such groups.  Host buffers / device buffers (_dev). */
+int bm4dof_run_learned(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+                       const float *noisy, unsigned w, unsigned h, unsigned f, unsigned c,
+                       const float *fflow, const float *bflow, float *out, int nparts, float gvar_thr,
+                       float *flag_frac);
+int bm4dof_run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2,
+                           const float *noisy, unsigned w, unsigned h, unsigned f, unsigned c,
+                           const float *fflow, const float *bflow, float *out, int nparts, float gvar_thr,
+                           float *flag_frac);
+
 /* ---- TV-L1 optical flow ---- */
 
 /*
