@@ -3,7 +3,6 @@
 #include "ctx.hpp"
 #include "mlp_dev.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -52,10 +51,23 @@ __global__ void fallback_mark_kernel(const uint32_t *, const uint32_t *, const u
                                      uint8_t *);
 __global__ void fallback_blend_kernel(const uint8_t *, const float *, int64_t, int, float *);
 __global__ void emit_entries_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int,
-                                    int, uint32_t *, uint32_t *);
-__global__ void count_positions_kernel(const uint32_t *, int64_t, uint32_t *);
-__global__ void fixup_early_kernel(const uint32_t *, int, const uint32_t *, const uint32_t *, int,
-                                   const float *, const float *, int, int, int, float *, float *);
+                                    int, uint32_t *);
+// sortscan.hip
+__global__ void fill_u32_kernel(uint32_t *, uint32_t, int64_t);
+__global__ void fill_f32_kernel(float *, float, int64_t);
+__global__ void fill_u8_kernel(uint8_t *, uint8_t, int64_t);
+template <int OP> __global__ void scan_block_kernel(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
+template <int OP> __global__ void scan_top_kernel(uint32_t *, int64_t);
+template <int OP> __global__ void scan_add_kernel(uint32_t *, int64_t, const uint32_t *);
+__global__ void count_rows_kernel(const uint32_t *, const uint32_t *, int64_t, uint32_t *);
+__global__ void scatter_rows_kernel(const uint32_t *, const uint32_t *, int64_t, const uint32_t *, uint32_t *,
+                                    uint32_t *);
+__global__ void sort_runs_kernel(const uint32_t *, int64_t, uint32_t *);
+__global__ void add_rows_kernel(const uint32_t *, unsigned long long *);
+__global__ void early_fix_kernel(const uint8_t *, const uint32_t *, const float *, const uint32_t *,
+                                 const uint32_t *, const uint32_t *, const uint32_t *, int, int, const float *,
+                                 int, int, float *, float *);
+__global__ void filtered_index_kernel(const uint8_t *, int, uint32_t *);
 #define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
 	float *, uint32_t *
@@ -478,37 +490,56 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	HIPCHK(hipGetLastError());
 }
 
-// Adds a chunk's rows, in row order, into the running float sums.  Bands
-// (row ranges processed one after the other) start at the first group of
-// every reference frame of every tile inside the chunk.
+void fill_u32(bm4dof_ctx *ctx, uint32_t *p, uint32_t v, int64_t n)
+{
+	if (n <= 0) return;
+	hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, p, v, n);
+	HIPCHK(hipGetLastError());
+}
+void fill_f32(bm4dof_ctx *ctx, float *p, float v, int64_t n)
+{
+	if (n <= 0) return;
+	hipLaunchKernelGGL(fill_f32_kernel, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, p, v, n);
+	HIPCHK(hipGetLastError());
+}
+
+// out[0..n) = scan of in[0..n_in) (identity past n_in): op 0 exclusive sum,
+// op 1 inclusive maximum.  Needs ctx->scan_tmp reserved (reserve_scan).
+void reserve_scan(bm4dof_ctx *ctx, int64_t n) { (void)ctx->scan_tmp.as<uint32_t>((size_t)(n / 1024 + 2)); }
+void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n, uint32_t *out)
+{
+	if (n <= 0) return;
+	const int64_t nb = (n + 1023) / 1024;
+	uint32_t *bsum = ctx->scan_tmp.as<uint32_t>((size_t)nb + 2);
+	if (op == 0) {
+		hipLaunchKernelGGL(scan_block_kernel<0>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, in, n_in, n, out, bsum);
+		hipLaunchKernelGGL(scan_top_kernel<0>, dim3(1), dim3(256), 0, ctx->stream, bsum, nb);
+		hipLaunchKernelGGL(scan_add_kernel<0>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, out, n, bsum);
+	} else {
+		hipLaunchKernelGGL(scan_block_kernel<1>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, in, n_in, n, out, bsum);
+		hipLaunchKernelGGL(scan_top_kernel<1>, dim3(1), dim3(256), 0, ctx->stream, bsum, nb);
+		hipLaunchKernelGGL(scan_add_kernel<1>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, out, n, bsum);
+	}
+	HIPCHK(hipGetLastError());
+}
+
+// Adds a chunk's rows, in row order, into the running float sums.  The row
+// count is on the device (*Mp); Mub bounds it.  Bands (row ranges processed
+// one after the other, banded kernel only) start at the first group of every
+// reference frame of every tile inside the chunk.
 void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uint32_t> &band_groups,
-                     int g0, int64_t M, int t0c, int t1c, const uint32_t *d_base,
+                     int g0, int64_t Mub, const uint32_t *Mp, int t0c, int t1c, const uint32_t *d_base,
                      const uint32_t *kstart, const uint32_t *svals, const uint32_t *epos,
                      const uint32_t *etile, const float *ew, const float *est, const float *win,
                      float *num, float *wsum, uint32_t *d_nsplit)
 {
-	const int nb = (int)band_groups.size() + 1;
-	uint32_t *d_bands = ctx->elist.as<uint32_t>(2 * (size_t)nb + 2);
-	uint32_t *d_gfirst = d_bands + nb + 1;
-	const uint32_t zero = 0, mend = (uint32_t)M;
-	HIPCHK(hipMemcpyAsync(d_bands, &zero, 4, hipMemcpyHostToDevice, ctx->stream));
-	HIPCHK(hipMemcpyAsync(d_bands + nb, &mend, 4, hipMemcpyHostToDevice, ctx->stream));
-	if (nb > 1) {
-		HIPCHK(hipMemcpyAsync(d_gfirst, band_groups.data(), band_groups.size() * 4, hipMemcpyHostToDevice,
-		                      ctx->stream));
-		hipLaunchKernelGGL(band_bounds_kernel, dim3(nblk(nb - 1, 256)), dim3(256), 0, ctx->stream, d_base,
-		                   d_gfirst, nb - 1, g0, d_bands + 1);
-		HIPCHK(hipGetLastError());
-	}
-	HIPCHK(hipMemsetAsync(d_nsplit, 0, 4, ctx->stream));
+	if (Mub <= 0 || t1c <= t0c) return;
 	const dim3 grid((sc.W + 7) / 8, (sc.H + 7) / 8, t1c - t0c);
 	// chunked kernel: a block then spans at most 2 x 2 tile interiors
 	const bool chunked = ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8;
-	if (M <= 0 || t1c <= t0c) {
-		// no rows in this chunk: nothing to add
-	} else if (chunked) {
-		uint4 *d_erec = ctx->erec.as<uint4>((size_t)std::max<int64_t>(M, 1));
-		hipLaunchKernelGGL(pack_rows_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, epos, etile, ew, M,
+	if (chunked) {
+		uint4 *d_erec = ctx->erec.as<uint4>((size_t)Mub);
+		hipLaunchKernelGGL(pack_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, epos, etile, ew, Mub,
 		                   d_erec);
 		HIPCHK(hipGetLastError());
 		const dim3 cgrid((unsigned)(((size_t)(t1c - t0c) * ((sc.W + 7) / 8) * ((sc.H + 7) / 8) + 7) / 8 * 8));
@@ -521,7 +552,26 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 		else if (sc.C == 4) kern = agg_chunk_kernel<4, 1024>;
 		hipLaunchKernelGGL(kern, cgrid, dim3(64), 0, ctx->stream, kstart, svals, d_erec, est, win, sc, t0c,
 		                   t1c - t0c, num, wsum, d_nsplit);
-	} else if (sc.C == 1)
+		HIPCHK(hipGetLastError());
+		ctx->agg_blocks += (uint64_t)grid.x * grid.y * grid.z;
+		ctx->agg_region[0] = 8;
+		ctx->agg_region[1] = 8;
+		ctx->agg_region[2] = 1;
+		return;
+	}
+	const int nb = (int)band_groups.size() + 1;
+	uint32_t *d_bands = ctx->elist.as<uint32_t>(2 * (size_t)nb + 2);
+	uint32_t *d_gfirst = d_bands + nb + 1;
+	fill_u32(ctx, d_bands, 0u, 1);
+	HIPCHK(hipMemcpyAsync(d_bands + nb, Mp, 4, hipMemcpyDeviceToDevice, ctx->stream));
+	if (nb > 1) {
+		HIPCHK(hipMemcpyAsync(d_gfirst, band_groups.data(), band_groups.size() * 4, hipMemcpyHostToDevice,
+		                      ctx->stream));
+		hipLaunchKernelGGL(band_bounds_kernel, dim3(nblk(nb - 1, 256)), dim3(256), 0, ctx->stream, d_base,
+		                   d_gfirst, nb - 1, g0, d_bands + 1);
+		HIPCHK(hipGetLastError());
+	}
+	if (sc.C == 1)
 		hipLaunchKernelGGL(agg_exact_kernel<1>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
 		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
 	else if (sc.C == 3)
@@ -531,20 +581,14 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 		hipLaunchKernelGGL(agg_exact_kernel<4>, grid, dim3(64), 0, ctx->stream, kstart, svals, epos, etile, ew,
 		                   est, win, sc, d_bands, nb, t0c, num, wsum, d_nsplit);
 	HIPCHK(hipGetLastError());
-	uint32_t ns = 0;
-	HIPCHK(hipMemcpyAsync(&ns, d_nsplit, 4, hipMemcpyDeviceToHost, ctx->stream));
+	// the banded kernel needs the band list on the host side: keep it alive
 	HIPCHK(hipStreamSynchronize(ctx->stream));
 	ctx->agg_blocks += (uint64_t)grid.x * grid.y * grid.z;
-	ctx->agg_overflow_blocks += ns;
 	ctx->agg_region[0] = 8;
 	ctx->agg_region[1] = 8;
-	ctx->agg_region[2] = chunked ? 1 : nb;
+	ctx->agg_region[2] = nb;
 }
 
-// One denoising step on device buffers in the working colour space.
-//   step1: noisy_w -> out_w (basic)
-//   step2: noisy_w, basic_w -> out_w (final)
-// Returns the number of groups processed.
 // The learned single pass runs step 1's grid and matching with the network's
 // gains in place of the threshold.
 struct LearnedArgs {
@@ -554,212 +598,279 @@ struct LearnedArgs {
 	uint32_t *nflag;     // count of flagged groups
 };
 
-uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
-                  const float *noisy_w, const float *basic_w, const float *ff, const float *bf,
-                  float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr)
-{
-	const bool step1 = p.isFirstStep != 0;
-	const bool use_flow = ff != nullptr && bf != nullptr;
-	check_step_params(p, (unsigned)C);
+// Host-side plan of one step: parameters, geometry and derived sizes.
+struct StepPlan {
+	bm4dof_params p;
+	int W = 0, H = 0, F = 0, C = 0, nparts = 0;
+	bool step1 = true, use_flow = false, learned = false, fast = false, fast_shape = false;
 	std::vector<float> hb, hbi, hw;
-	build_bases(p, hb, hbi, hw);
-	StepGeom geo = make_geometry(p, W, H, F, nparts);
-	StepConst sc = make_const(p, W, H, F, C, use_flow, geo.til);
-	const int G = (int)geo.gpos.size();
-	const int np = sc.npmax;
-	const size_t WHF = (size_t)W * H * F;
-	const size_t N_all = (size_t)WHF * C;
+	StepGeom geo;
+	StepConst sc;
+	int G = 0, np = 0, chunk_groups = 1;
+	size_t WHF = 0, N_all = 0, filt_lds = 0;
+};
 
-	float *d_basis = ctx->basis.as<float>(hb.size());
-	float *d_binv = ctx->binv.as<float>(hbi.size());
-	float *d_win = ctx->win.as<float>(hw.size());
-	HIPCHK(hipMemcpyAsync(d_basis, hb.data(), hb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-	HIPCHK(hipMemcpyAsync(d_binv, hbi.data(), hbi.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-	HIPCHK(hipMemcpyAsync(d_win, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-	TileBox *d_tiles = ctx->tiles.as<TileBox>(geo.til.crop.size());
-	HIPCHK(hipMemcpyAsync(d_tiles, geo.til.crop.data(), geo.til.crop.size() * sizeof(TileBox),
-	                      hipMemcpyHostToDevice, ctx->stream));
-	uint32_t *d_gpos = ctx->gpos.as<uint32_t>(std::max(G, 1));
-	uint32_t *d_gtile = ctx->gtile.as<uint32_t>(std::max(G, 1));
-	HIPCHK(hipMemcpyAsync(d_gpos, geo.gpos.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
-	HIPCHK(hipMemcpyAsync(d_gtile, geo.gtile.data(), (size_t)G * 4, hipMemcpyHostToDevice, ctx->stream));
+StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C, bool use_flow, int nparts,
+                   bool learned)
+{
+	StepPlan pl;
+	pl.p = p;
+	pl.W = W; pl.H = H; pl.F = F; pl.C = C; pl.nparts = nparts;
+	pl.step1 = p.isFirstStep != 0;
+	pl.use_flow = use_flow;
+	pl.learned = learned;
+	check_step_params(p, (unsigned)C);
+	build_bases(p, pl.hb, pl.hbi, pl.hw);
+	pl.geo = make_geometry(p, W, H, F, nparts);
+	pl.sc = make_const(p, W, H, F, C, use_flow, pl.geo.til);
+	pl.G = (int)pl.geo.gpos.size();
+	pl.np = pl.sc.npmax;
+	pl.WHF = (size_t)W * H * F;
+	pl.N_all = pl.WHF * C;
+	// chunks of consecutive groups bounded by the estimate buffer budget
+	const size_t row_bytes = (size_t)C * pl.sc.P * 4;
+	const size_t max_rows = std::max<size_t>(pl.np, ctx->est_budget / row_bytes);
+	pl.chunk_groups = (int)std::min<size_t>((size_t)std::max(pl.G, 1), max_rows / pl.np);
+	pl.filt_lds = pl.step1 ? ((size_t)C * pl.sc.P * pl.np + (size_t)pl.sc.P * pl.np) * 4
+	                       : ((size_t)C * pl.sc.P * pl.np + 2 * (size_t)pl.sc.P * pl.np) * 4;
+	if (pl.filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
+	// register-resident filter: one channel, 64-sample patches, <= 32 neighbours
+	pl.fast_shape = C == 1 && pl.sc.P == 64 && pl.np <= 32 &&
+	                ((pl.sc.sPx == 4 && pl.sc.sPt == 4) || (pl.sc.sPx == 8 && pl.sc.sPt == 1));
+	pl.fast = ctx->fast_filter && pl.fast_shape && !pl.sc.order_inv;
+	if (learned && !pl.fast_shape)
+		throw ArgError("learned pass: one channel, 4x4x4 or 8x8x1 patches and at most 32 neighbours",
+		               BM4DOF_EUNSUPP);
+	return pl;
+}
 
+// Device geometry of a plan (bases, tiles, group positions and tiles, first
+// group per tile) in buffers that outlive the step's launches.
+struct StepDev {
+	float *basis = nullptr, *binv = nullptr, *win = nullptr;
+	TileBox *tiles = nullptr;
+	uint32_t *gpos = nullptr, *gtile = nullptr, *tile_first = nullptr;
+};
+
+// Uploads a plan's geometry into `geo_buf` (grown as needed).  Synchronous.
+StepDev upload_plan(bm4dof_ctx *ctx, const StepPlan &pl, DevBuf &geo_buf)
+{
+	const size_t nb = pl.hb.size(), nbi = pl.hbi.size(), nw = pl.hw.size(), nt = pl.geo.til.crop.size();
+	const size_t G = (size_t)std::max(pl.G, 1), ntf = pl.geo.tile_first.size();
+	auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+	const size_t o_basis = 0, o_binv = o_basis + al(nb * 4), o_win = o_binv + al(nbi * 4),
+	             o_tiles = o_win + al(nw * 4), o_gpos = o_tiles + al(nt * sizeof(TileBox)),
+	             o_gtile = o_gpos + al(G * 4), o_tf = o_gtile + al(G * 4), total = o_tf + al(std::max<size_t>(ntf, 1) * 4);
+	char *b = (char *)geo_buf.get(total);
+	StepDev d;
+	d.basis = (float *)(b + o_basis);
+	d.binv = (float *)(b + o_binv);
+	d.win = (float *)(b + o_win);
+	d.tiles = (TileBox *)(b + o_tiles);
+	d.gpos = (uint32_t *)(b + o_gpos);
+	d.gtile = (uint32_t *)(b + o_gtile);
+	d.tile_first = (uint32_t *)(b + o_tf);
+	HIPCHK(hipMemcpy(d.basis, pl.hb.data(), nb * 4, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d.binv, pl.hbi.data(), nbi * 4, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d.win, pl.hw.data(), nw * 4, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(d.tiles, pl.geo.til.crop.data(), nt * sizeof(TileBox), hipMemcpyHostToDevice));
+	if (pl.G > 0) {
+		HIPCHK(hipMemcpy(d.gpos, pl.geo.gpos.data(), (size_t)pl.G * 4, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(d.gtile, pl.geo.gtile.data(), (size_t)pl.G * 4, hipMemcpyHostToDevice));
+	}
+	if (ntf) HIPCHK(hipMemcpy(d.tile_first, pl.geo.tile_first.data(), ntf * 4, hipMemcpyHostToDevice));
+	return d;
+}
+
+// Grows every shared work buffer a plan's launches use, so that enqueueing
+// the step allocates nothing (required while a stream is being captured).
+void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
+{
+	const size_t G = (size_t)std::max(pl.G, 1), np = (size_t)pl.np;
+	const size_t Mub = (size_t)std::min(pl.chunk_groups, std::max(pl.G, 1)) * np;
+	const int sWt = pl.sc.wtf + pl.sc.wtb + 1;
+	(void)ctx->idx.as<uint32_t>(G * np);
+	(void)ctx->nsim.as<uint32_t>(G);
+	(void)ctx->early.as<uint8_t>(G);
+	(void)ctx->gweight.as<float>(G);
+	(void)ctx->counters.as<uint32_t>(16);
+	(void)ctx->stat64.as<unsigned long long>(4);
+	(void)ctx->num.as<float>(pl.N_all);
+	(void)ctx->wsum.as<float>(pl.WHF);
+	(void)ctx->lstart.as<uint32_t>(pl.WHF + 1);
+	(void)ctx->lend.as<uint32_t>(pl.WHF + 1);
+	(void)ctx->base.as<uint32_t>((size_t)pl.chunk_groups + 1);
+	(void)ctx->keys.as<uint32_t>(Mub);
+	(void)ctx->svals.as<uint32_t>(Mub);
+	(void)ctx->etile.as<uint32_t>(Mub);
+	(void)ctx->entry_w.as<float>(Mub);
+	(void)ctx->est.as<float>(Mub * pl.C * pl.sc.P);
+	(void)ctx->erec.as<uint4>(Mub);
+	(void)ctx->lastf.as<uint32_t>(G);
+	(void)ctx->fidx.as<uint32_t>(G);
+	(void)ctx->bm_wins.as<int4>(G * sWt);
+	(void)ctx->bm_wmeta.as<int2>(G);
+	if (pl.learned) {
+		(void)ctx->gflag.as<uint8_t>(G);
+		(void)ctx->fmask.as<uint8_t>(pl.WHF);
+	}
+	reserve_scan(ctx, (int64_t)std::max(pl.WHF + 1, G + 1));
+}
+
+// Launches one step on ctx->stream.  With `tm` it records stage events; a
+// captured enqueue passes none.  Needs reserve_step(plan) first.
+void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const float *noisy_w,
+                  const float *basic_w, const float *ff, const float *bf, float *out_w, Timer *tm,
+                  const LearnedArgs *la)
+{
+	const bool step1 = pl.step1;
+	const StepConst &sc = pl.sc;
+	const int G = pl.G, np = pl.np, C = pl.C;
+	const size_t WHF = pl.WHF, N_all = pl.N_all;
+	auto mark = [&](const char *a, const char *b1, const char *b2) {
+		if (tm) tm->mark(la ? a : step1 ? b1 : b2);
+	};
 	uint32_t *d_idx = ctx->idx.as<uint32_t>((size_t)std::max(G, 1) * np);
 	uint32_t *d_nsim = ctx->nsim.as<uint32_t>(std::max(G, 1));
 	uint8_t *d_early = ctx->early.as<uint8_t>(std::max(G, 1));
 	float *d_gw = ctx->gweight.as<float>(std::max(G, 1));
 	uint32_t *d_cnt = ctx->counters.as<uint32_t>(16);
+	unsigned long long *d_rows = ctx->stat64.as<unsigned long long>(4) + (step1 ? 0 : 1);
 	float *d_num = ctx->num.as<float>(N_all);
 	float *d_wsum = ctx->wsum.as<float>(WHF);
 	uint32_t *d_kstart = ctx->lstart.as<uint32_t>(WHF + 1);
 	uint32_t *d_pcnt = ctx->lend.as<uint32_t>(WHF + 1);
-	HIPCHK(hipMemsetAsync(d_cnt, 0, 64, ctx->stream));
-	HIPCHK(hipMemsetAsync(d_num, 0, N_all * 4, ctx->stream));
-	HIPCHK(hipMemsetAsync(d_wsum, 0, WHF * 4, ctx->stream));
-	tm.mark(la ? "l_setup" : step1 ? "s1_setup" : "s2_setup");
+	fill_u32(ctx, d_cnt, 0u, 4);   // early-out count (slots 4.. are run totals)
+	fill_f32(ctx, d_num, 0.f, (int64_t)N_all);
+	fill_f32(ctx, d_wsum, 0.f, (int64_t)WHF);
+	mark("l_setup", "s1_setup", "s2_setup");
 
-	run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, d_gpos, d_gtile, d_tiles, G,
-	       d_idx, d_nsim);
-	tm.mark(la ? "l_bm" : step1 ? "s1_bm" : "s2_bm");
-
-	// Chunks of consecutive groups bounded by the estimate buffer budget.
-	const size_t row_bytes = (size_t)C * sc.P * 4;
-	const size_t max_rows = std::max<size_t>(np, ctx->est_budget / row_bytes);
-	const int chunk_groups = (int)std::min<size_t>((size_t)std::max(G, 1), max_rows / np);
-	uint32_t n_early_seen = 0;
-
-	const size_t filt_lds = step1 ? ((size_t)C * sc.P * np + (size_t)sc.P * np) * 4
-	                              : ((size_t)C * sc.P * np + 2 * (size_t)sc.P * np) * 4;
-	if (filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
-	// register-resident filter: one channel, 64-sample patches, <= 32 neighbours
-	const bool fast_shape = C == 1 && sc.P == 64 && np <= 32 &&
-	                        ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1));
-	const bool fast = ctx->fast_filter && fast_shape && !sc.order_inv;
-	if (la && !fast_shape)
-		throw ArgError("learned pass: one channel, 4x4x4 or 8x8x1 patches and at most 32 neighbours",
-		               BM4DOF_EUNSUPP);
+	run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, dv.gpos, dv.gtile, dv.tiles, G, d_idx,
+	       d_nsim);
+	mark("l_bm", "s1_bm", "s2_bm");
 	uint8_t *d_gflag = la ? ctx->gflag.as<uint8_t>(std::max(G, 1)) : nullptr;
 
-	for (int g0 = 0; g0 < G; g0 += chunk_groups) {
-		const int Gc = std::min(chunk_groups, G - g0);
-		// row offsets of the chunk's groups
-		uint32_t *d_base = ctx->base.as<uint32_t>(Gc + 1);
-		size_t tmp_bytes = 0;
-		HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_nsim + g0, d_base, Gc, ctx->stream));
-		void *d_tmp = ctx->sort_tmp.get(tmp_bytes);
-		HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_nsim + g0, d_base, Gc, ctx->stream));
-		uint32_t last_base = 0, last_n = 0;
-		HIPCHK(hipMemcpyAsync(&last_base, d_base + Gc - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-		HIPCHK(hipMemcpyAsync(&last_n, d_nsim + g0 + Gc - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-		HIPCHK(hipStreamSynchronize(ctx->stream));
-		const int64_t M = (int64_t)last_base + last_n;
-		ctx->step_groups[step1 ? 0 : 1] += (uint64_t)Gc;
-		ctx->step_rows[step1 ? 0 : 1] += (uint64_t)M;
-
-		uint32_t *d_epos = ctx->keys.as<uint32_t>(M);
-		uint32_t *d_vals = ctx->vals.as<uint32_t>(M);
-		uint32_t *d_skeys = ctx->skeys.as<uint32_t>(M);
-		uint32_t *d_svals = ctx->svals.as<uint32_t>(M);
-		uint32_t *d_etile = ctx->etile.as<uint32_t>(M);
-		float *d_ew = ctx->entry_w.as<float>(M);
-		float *d_est = ctx->est.as<float>((size_t)M * C * sc.P);
-		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk((int64_t)Gc * np, 256)), dim3(256), 0,
-		                   ctx->stream, d_idx, d_nsim, d_base, np, g0, Gc, d_epos, d_vals);
+	for (int g0 = 0; g0 < G; g0 += pl.chunk_groups) {
+		const int Gc = std::min(pl.chunk_groups, G - g0);
+		const int64_t Mub = (int64_t)Gc * np;
+		// row offsets of the chunk's groups; base[Gc] = rows of the chunk
+		uint32_t *d_base = ctx->base.as<uint32_t>((size_t)Gc + 1);
+		dscan(ctx, 0, d_nsim + g0, Gc, (int64_t)Gc + 1, d_base);
+		const uint32_t *d_M = d_base + Gc;
+		uint32_t *d_epos = ctx->keys.as<uint32_t>(Mub);
+		uint32_t *d_svals = ctx->svals.as<uint32_t>(Mub);
+		uint32_t *d_etile = ctx->etile.as<uint32_t>(Mub);
+		float *d_ew = ctx->entry_w.as<float>(Mub);
+		float *d_est = ctx->est.as<float>((size_t)Mub * C * sc.P);
+		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, d_idx, d_nsim,
+		                   d_base, np, g0, Gc, d_epos);
 		HIPCHK(hipGetLastError());
-		// rows sorted by patch position, stable (row order kept within a position)
-		int end_bit = 1;
-		while (end_bit < 32 && (1ull << end_bit) < WHF) end_bit++;
-		size_t sort_bytes = 0;
-		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, d_epos, d_skeys, d_vals, d_svals,
-		                                          (int)M, 0, end_bit, ctx->stream));
-		d_tmp = ctx->sort_tmp.get(sort_bytes);
-		HIPCHK(hipcub::DeviceRadixSort::SortPairs(d_tmp, sort_bytes, d_epos, d_skeys, d_vals, d_svals,
-		                                          (int)M, 0, end_bit, ctx->stream));
-		// kstart[k] = number of rows at positions < k
-		HIPCHK(hipMemsetAsync(d_pcnt, 0, (WHF + 1) * 4, ctx->stream));
-		hipLaunchKernelGGL(count_positions_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, d_epos, M,
+		hipLaunchKernelGGL(add_rows_kernel, dim3(1), dim3(64), 0, ctx->stream, d_M, d_rows);
+		// rows listed by patch position, in row order within a position
+		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1);
+		hipLaunchKernelGGL(count_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, d_epos, d_M, Mub,
 		                   d_pcnt);
+		dscan(ctx, 0, d_pcnt, (int64_t)WHF, (int64_t)WHF + 1, d_kstart);
+		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1);
+		hipLaunchKernelGGL(scatter_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, d_epos, d_M, Mub,
+		                   d_kstart, d_pcnt, d_svals);
+		hipLaunchKernelGGL(sort_runs_kernel, dim3(nblk((int64_t)WHF, 256)), dim3(256), 0, ctx->stream, d_kstart,
+		                   (int64_t)WHF, d_svals);
 		HIPCHK(hipGetLastError());
-		tmp_bytes = 0;
-		HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
-		d_tmp = ctx->sort_tmp.get(tmp_bytes);
-		HIPCHK(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_pcnt, d_kstart, (int)(WHF + 1), ctx->stream));
-		tm.mark(la ? "l_sort" : step1 ? "s1_sort" : "s2_sort");
+		mark("l_sort", "s1_sort", "s2_sort");
 
 		if (la)
 			hipLaunchKernelGGL(sc.sPx == 4 ? filter_learned_kernel<4> : filter_learned_kernel<8>, dim3(Gc),
-			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc,
-			                   d_basis, d_binv, d_win, la->net, la->sigma, la->gvar_thr, d_est, d_ew, d_etile,
+			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, dv.gtile, sc, g0, Gc,
+			                   dv.basis, dv.binv, dv.win, la->net, la->sigma, la->gvar_thr, d_est, d_ew, d_etile,
 			                   d_early, d_gw, d_gflag, la->nflag);
-		else if (fast && step1)
+		else if (pl.fast && step1)
 			hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
-			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, d_gtile, sc, g0,
-			                   Gc, d_basis, d_binv, d_win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
-		else if (fast)
+			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, dv.gtile, sc, g0,
+			                   Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
+		else if (pl.fast)
 			hipLaunchKernelGGL(sc.sPx == 4 ? filter2_fast_kernel<4> : filter2_fast_kernel<8>, dim3(Gc),
-			                   dim3(64), 0, ctx->stream, noisy_w, basic_w, d_idx, d_nsim, d_base, d_gtile,
-			                   sc, g0, Gc, d_basis, d_binv, d_win, d_est, d_ew, d_etile, d_early, d_gw,
+			                   dim3(64), 0, ctx->stream, noisy_w, basic_w, d_idx, d_nsim, d_base, dv.gtile,
+			                   sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early, d_gw,
 			                   d_cnt);
 		else if (step1)
-			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
-			                   d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc, d_basis, d_binv, d_win, d_est,
+			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), pl.filt_lds, ctx->stream, noisy_w,
+			                   d_idx, d_nsim, d_base, dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est,
 			                   d_ew, d_etile, d_early, d_gw, d_cnt);
 		else
-			hipLaunchKernelGGL(filter2_kernel, dim3(Gc), dim3(256), filt_lds, ctx->stream, noisy_w,
-			                   basic_w, d_idx, d_nsim, d_base, d_gtile, sc, g0, Gc, d_basis, d_binv,
-			                   d_win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
+			hipLaunchKernelGGL(filter2_kernel, dim3(Gc), dim3(256), pl.filt_lds, ctx->stream, noisy_w,
+			                   basic_w, d_idx, d_nsim, d_base, dv.gtile, sc, g0, Gc, dv.basis, dv.binv,
+			                   dv.win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
 		HIPCHK(hipGetLastError());
-		tm.mark(la ? "l_filter" : step1 ? "s1_filter" : "s2_filter");
-
-		// Early-out groups reuse the previous group's weights (rare).
-		uint32_t n_early = 0;
-		HIPCHK(hipMemcpyAsync(&n_early, d_cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
-		HIPCHK(hipStreamSynchronize(ctx->stream));
-		if (n_early != n_early_seen) {
-			n_early_seen = n_early;
+		if (!la) {
+			// equality early-outs take the stale weights of their tile
 			const int gend = g0 + Gc;
-			std::vector<uint8_t> h_early(gend);
-			std::vector<float> h_gw(gend);
-			std::vector<uint32_t> h_nsim(gend);
-			HIPCHK(hipMemcpy(h_early.data(), d_early, gend, hipMemcpyDeviceToHost));
-			HIPCHK(hipMemcpy(h_gw.data(), d_gw, (size_t)gend * 4, hipMemcpyDeviceToHost));
-			HIPCHK(hipMemcpy(h_nsim.data(), d_nsim, (size_t)gend * 4, hipMemcpyDeviceToHost));
-			std::vector<uint32_t> glist;
-			std::vector<float> stale((size_t)gend * np, 0.f);
-			// aggreWeights start at 1 in every tile (nldct.cpp:749, 818) and keep
-			// the last written value per slot.
-			std::vector<float> slots(np, 1.f);
-			uint32_t tile_cur = 0xffffffffu;
-			for (int g = 0; g < gend; g++) {
-				if (geo.gtile[g] != tile_cur) { tile_cur = geo.gtile[g]; std::fill(slots.begin(), slots.end(), 1.f); }
-				if (h_early[g]) {
-					if (g >= g0) {
-						glist.push_back((uint32_t)g);
-						for (uint32_t n = 0; n < h_nsim[g]; n++) stale[(size_t)g * np + n] = slots[n];
-					}
-				} else
-					for (uint32_t n = 0; n < h_nsim[g]; n++) slots[n] = h_gw[g];
-			}
-			if (!glist.empty()) {
-				uint32_t *d_gl = ctx->elist.as<uint32_t>(glist.size());
-				float *d_st = ctx->stale.as<float>(stale.size());
-				HIPCHK(hipMemcpyAsync(d_gl, glist.data(), glist.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-				HIPCHK(hipMemcpyAsync(d_st, stale.data(), stale.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-				hipLaunchKernelGGL(fixup_early_kernel, dim3((unsigned)glist.size()), dim3(256), 0, ctx->stream,
-				                   d_gl, (int)glist.size(), d_nsim, d_base, g0, d_st, d_win, np, C, sc.P,
-				                   d_est, d_ew);
-				HIPCHK(hipGetLastError());
-				HIPCHK(hipStreamSynchronize(ctx->stream));
-			}
+			uint32_t *d_fidx = ctx->fidx.as<uint32_t>((size_t)std::max(G, 1));
+			uint32_t *d_lastf = ctx->lastf.as<uint32_t>((size_t)std::max(G, 1));
+			hipLaunchKernelGGL(filtered_index_kernel, dim3(nblk(gend, 256)), dim3(256), 0, ctx->stream, d_early,
+			                   gend, d_fidx);
+			dscan(ctx, 1, d_fidx, gend, gend, d_lastf);
+			hipLaunchKernelGGL(early_fix_kernel, dim3(nblk(Gc, 64)), dim3(64), 0, ctx->stream, d_early, d_nsim, d_gw,
+			                   d_lastf, dv.gtile, dv.tile_first, d_base, g0, Gc, dv.win, C, sc.P, d_est, d_ew);
+			HIPCHK(hipGetLastError());
 		}
+		mark("l_filter", "s1_filter", "s2_filter");
 
-		// Ordered aggregation over the frames this chunk touches.
-		uint32_t kfirst = 0, klast = 0;
-		HIPCHK(hipMemcpyAsync(&kfirst, d_skeys, 4, hipMemcpyDeviceToHost, ctx->stream));
-		HIPCHK(hipMemcpyAsync(&klast, d_skeys + M - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-		HIPCHK(hipStreamSynchronize(ctx->stream));
-		const int t0c = (int)(kfirst / (uint32_t)sc.WH);
-		const int t1c = std::min(F, (int)(klast / (uint32_t)sc.WH) + sc.sPt);
 		std::vector<uint32_t> band_groups;
-		for (uint32_t gf : geo.frame_first)
+		for (uint32_t gf : pl.geo.frame_first)
 			if ((int)gf > g0 && (int)gf < g0 + Gc) band_groups.push_back(gf);
-		aggregate_chunk(ctx, sc, band_groups, g0, M, t0c, t1c, d_base, d_kstart, d_svals, d_epos, d_etile,
-		                d_ew, d_est, d_win, d_num, d_wsum, d_cnt + 4);
-		tm.mark(la ? "l_aggregate" : step1 ? "s1_aggregate" : "s2_aggregate");
+		aggregate_chunk(ctx, sc, band_groups, g0, Mub, d_M, 0, pl.F, d_base, d_kstart, d_svals, d_epos, d_etile,
+		                d_ew, d_est, dv.win, d_num, d_wsum, d_cnt + 4);
+		mark("l_aggregate", "s1_aggregate", "s2_aggregate");
 	}
 	hipLaunchKernelGGL(normalize_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream,
 	                   d_num, d_wsum, noisy_w, sc, out_w);
 	HIPCHK(hipGetLastError());
-	tm.mark(la ? "l_normalize" : step1 ? "s1_normalize" : "s2_normalize");
+	mark("l_normalize", "s1_normalize", "s2_normalize");
 	if (la && G > 0) {
 		// pixels under flagged groups, for the plain two-step fallback
 		uint8_t *d_mask = ctx->fmask.as<uint8_t>(WHF);
-		HIPCHK(hipMemsetAsync(d_mask, 0, WHF, ctx->stream));
+		hipLaunchKernelGGL(fill_u8_kernel, dim3(nblk((int64_t)WHF, 256)), dim3(256), 0, ctx->stream, d_mask,
+		                   (uint8_t)0, (int64_t)WHF);
 		hipLaunchKernelGGL(fallback_mark_kernel, dim3(G), dim3(64), 0, ctx->stream, d_idx, d_nsim, d_gflag, sc,
 		                   G, d_mask);
 		HIPCHK(hipGetLastError());
 	}
-	return (uint64_t)G;
+}
+
+// One step, eager: plan, upload, reserve and enqueue.
+uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
+                  const float *noisy_w, const float *basic_w, const float *ff, const float *bf,
+                  float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr)
+{
+	const bool use_flow = ff != nullptr && bf != nullptr;
+	StepPlan pl = plan_step(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr);
+	HIPCHK(hipStreamSynchronize(ctx->stream));   // the geometry buffer may be in use
+	StepDev dv = upload_plan(ctx, pl, ctx->geo);
+	reserve_step(ctx, pl);
+	ctx->step_groups[pl.step1 ? 0 : 1] += (uint64_t)pl.G;
+	enqueue_step(ctx, pl, dv, noisy_w, basic_w, use_flow ? ff : nullptr, use_flow ? bf : nullptr, out_w, &tm, la);
+	return (uint64_t)pl.G;
+}
+
+// Run statistics: zeroed on the device at a run's start, read at its end.
+void stats_begin(bm4dof_ctx *ctx)
+{
+	ctx->agg_blocks = ctx->agg_overflow_blocks = 0;
+	ctx->step_groups[0] = ctx->step_groups[1] = ctx->step_rows[0] = ctx->step_rows[1] = 0;
+	fill_u32(ctx, ctx->counters.as<uint32_t>(16), 0u, 16);
+	fill_u32(ctx, (uint32_t *)ctx->stat64.as<unsigned long long>(4), 0u, 8);
+}
+void stats_end(bm4dof_ctx *ctx)
+{
+	unsigned long long rows[2] = {0, 0};
+	uint32_t cnt[16];
+	HIPCHK(hipMemcpyAsync(rows, ctx->stat64.as<unsigned long long>(4), 16, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(hipMemcpyAsync(cnt, ctx->counters.as<uint32_t>(16), 64, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	ctx->step_rows[0] = rows[0];
+	ctx->step_rows[1] = rows[1];
+	ctx->agg_overflow_blocks = cnt[4];
 }
 
 // Whole runNlBayes on device buffers (nldct.cpp:267-478).
@@ -780,9 +891,8 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 	HIPCHK(hipSetDevice(ctx->device));
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
 	ctx->ev.clear();
-	ctx->agg_blocks = ctx->agg_overflow_blocks = 0;
-	ctx->step_groups[0] = ctx->step_groups[1] = ctx->step_rows[0] = ctx->step_rows[1] = 0;
 	ctx->timings.clear();
+	stats_begin(ctx);
 	Timer tm(ctx);
 	tm.mark("start");
 	const size_t n = (size_t)W * H * F * C;
@@ -820,6 +930,7 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 		}
 	}
 	HIPCHK(hipStreamSynchronize(ctx->stream));
+	stats_end(ctx);
 	// group ratios: per tile 100*groups/whf, summed over tiles in order
 	for (int s = 0; s < 2; s++) {
 		if ((s == 0 && step == 2) || (s == 1 && step == 1)) continue;
@@ -859,9 +970,8 @@ void run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
 	HIPCHK(hipSetDevice(ctx->device));
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
 	ctx->ev.clear();
-	ctx->agg_blocks = ctx->agg_overflow_blocks = 0;
-	ctx->step_groups[0] = ctx->step_groups[1] = ctx->step_rows[0] = ctx->step_rows[1] = 0;
 	ctx->timings.clear();
+	stats_begin(ctx);
 	Timer tm(ctx);
 	tm.mark("start");
 	LearnedArgs la;
@@ -870,7 +980,6 @@ void run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
 	la.gvar_thr = gvar_thr >= 0.f ? gvar_thr : 3.4e38f;
 	la.nflag = ctx->counters.as<uint32_t>(16) + 8;
 	la.gflag = nullptr;
-	HIPCHK(hipMemsetAsync(la.nflag, 0, 4, ctx->stream));
 	bm4dof_params pl = *p1;
 	pl.isFirstStep = 1;
 	const uint64_t G = run_step(ctx, pl, W, H, F, 1, noisy, nullptr, ff, bf, out, nparts, tm, &la);
@@ -896,6 +1005,7 @@ void run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
 		(void)timings;
 	}
 	HIPCHK(hipStreamSynchronize(ctx->stream));
+	stats_end(ctx);
 	for (size_t i = 1; i < ctx->ev.size(); i++) {
 		float ms = 0.f;
 		HIPCHK(hipEventElapsedTime(&ms, ctx->ev[i - 1], ctx->ev[i]));

@@ -1379,20 +1379,18 @@ template __global__ void filter2_fast_kernel<8>(FILTER2_ARGS);
 // interiors, lib_videot.hpp:1128-1160).
 // ---------------------------------------------------------------------------
 
-// keys[e] = patch position of row e, vals[e] = e, for the groups
+// keys[e] = patch position of row e, for the groups
 // [g0, g0+Gc) of a chunk; base = exclusive scan of nsim over the chunk.
 __global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim,
                                     const uint32_t *__restrict__ base, int npmax, int g0, int Gc,
-                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+                                    uint32_t *__restrict__ keys)
 {
 	const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
 	if (o >= (int64_t)Gc * npmax) return;
 	const int gl = (int)(o / npmax), n = (int)(o - (int64_t)gl * npmax);
 	const int g = g0 + gl;
 	if (n >= (int)nsim[g]) return;
-	const uint32_t e = base[gl] + n;
-	keys[e] = idx[(size_t)g * npmax + n];
-	vals[e] = e;
+	keys[base[gl] + n] = idx[(size_t)g * npmax + n];
 }
 
 // erec[e] = (position, tile, weight bits, 0): the row metadata the chunked
@@ -1402,36 +1400,6 @@ __global__ void pack_rows_kernel(const uint32_t *__restrict__ epos, const uint32
 {
 	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
 	if (e < M) erec[e] = make_uint4(epos[e], etile[e], __float_as_uint(entry_w[e]), 0u);
-}
-
-// Number of rows per patch position.
-__global__ void count_positions_kernel(const uint32_t *__restrict__ keys, int64_t M,
-                                       uint32_t *__restrict__ cnt)
-{
-	const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (r < M) atomicAdd(&cnt[keys[r]], 1u);
-}
-
-// Early-out groups: multiply the raw patch rows by the stale weight w_s and
-// the window (nldct.cpp:2296-2299).  stale[g*np+n] holds w_s per slot.
-__global__ void fixup_early_kernel(const uint32_t *__restrict__ glist, int ng,
-                                   const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ base,
-                                   int g0, const float *__restrict__ stale, const float *__restrict__ win,
-                                   int npmax, int C, int P, float *__restrict__ est,
-                                   float *__restrict__ entry_w)
-{
-	const int k = blockIdx.x;
-	if (k >= ng) return;
-	const uint32_t g = glist[k];
-	const int N = (int)nsim[g];
-	for (int o = threadIdx.x; o < N * C * P; o += blockDim.x) {
-		const int n = o / (C * P), r = o - n * C * P;
-		const int pix = r % P;
-		const uint32_t row = base[g - g0] + n;
-		const float ww = stale[(size_t)g * npmax + n] * win[pix];
-		est[(size_t)row * C * P + r] = ww * est[(size_t)row * C * P + r];
-		if (r == 0) entry_w[row] = stale[(size_t)g * npmax + n];
-	}
 }
 
 #define AGG_CAPB 1024   // rows of one band held in LDS

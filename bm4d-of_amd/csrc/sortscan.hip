@@ -1,0 +1,204 @@
+// Scans, the position-ordered row lists and the early-out weight fixup of
+// the denoiser, as plain kernels: no host round trip and no memset or copy
+// nodes, so a whole step can be captured into a hipGraph and replayed.
+#include "common.hpp"
+
+namespace bm4dof {
+
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint32_t v, int64_t n)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+__global__ void fill_f32_kernel(float *__restrict__ p, float v, int64_t n)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+__global__ void fill_u8_kernel(uint8_t *__restrict__ p, uint8_t v, int64_t n)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ scans
+// Three passes over blocks of SCAN_ITEMS elements: block scans with block
+// totals, one block scanning the totals, then the block offsets added.
+// OP 0: sum (exclusive), OP 1: maximum (inclusive).  Inputs at index
+// >= n_in read as the identity (0).
+
+#define SCAN_THREADS 256
+#define SCAN_PER 4
+#define SCAN_ITEMS (SCAN_THREADS * SCAN_PER)
+
+template <int OP> __device__ __forceinline__ uint32_t scan_op(uint32_t a, uint32_t b)
+{
+	return OP == 0 ? a + b : (a > b ? a : b);
+}
+
+// inclusive block scan of one value per thread (256 threads)
+template <int OP> __device__ uint32_t block_incl_scan(uint32_t v, uint32_t *sh)
+{
+	const int t = threadIdx.x;
+	sh[t] = v;
+	__syncthreads();
+	for (int d = 1; d < SCAN_THREADS; d <<= 1) {
+		const uint32_t o = t >= d ? sh[t - d] : 0u;
+		__syncthreads();
+		if (t >= d) sh[t] = scan_op<OP>(sh[t], o);
+		__syncthreads();
+	}
+	return sh[t];
+}
+
+template <int OP>
+__global__ __launch_bounds__(SCAN_THREADS)
+void scan_block_kernel(const uint32_t *__restrict__ in, int64_t n_in, int64_t n, uint32_t *__restrict__ out,
+                       uint32_t *__restrict__ bsum)
+{
+	__shared__ uint32_t sh[SCAN_THREADS];
+	const int64_t b0 = (int64_t)blockIdx.x * SCAN_ITEMS + (int64_t)threadIdx.x * SCAN_PER;
+	uint32_t v[SCAN_PER];
+	uint32_t acc = 0;
+#pragma unroll
+	for (int i = 0; i < SCAN_PER; i++) {
+		const int64_t j = b0 + i;
+		v[i] = j < n_in ? in[j] : 0u;
+		acc = scan_op<OP>(acc, v[i]);
+	}
+	const uint32_t incl = block_incl_scan<OP>(acc, sh);
+	uint32_t run = OP == 0 ? incl - acc : (threadIdx.x ? sh[threadIdx.x - 1] : 0u);
+#pragma unroll
+	for (int i = 0; i < SCAN_PER; i++) {
+		const int64_t j = b0 + i;
+		if (OP == 0) {
+			if (j < n) out[j] = run;   // exclusive
+			run += v[i];
+		} else {
+			run = scan_op<OP>(run, v[i]);
+			if (j < n) out[j] = run;   // inclusive
+		}
+	}
+	if (threadIdx.x == SCAN_THREADS - 1) bsum[blockIdx.x] = incl;
+}
+
+// exclusive scan (OP) of nb block totals in place, one block
+template <int OP>
+__global__ __launch_bounds__(SCAN_THREADS)
+void scan_top_kernel(uint32_t *__restrict__ bsum, int64_t nb)
+{
+	__shared__ uint32_t sh[SCAN_THREADS];
+	uint32_t carry = 0;
+	for (int64_t c0 = 0; c0 < nb; c0 += SCAN_THREADS) {
+		const int64_t j = c0 + threadIdx.x;
+		const uint32_t v = j < nb ? bsum[j] : 0u;
+		const uint32_t incl = block_incl_scan<OP>(v, sh);
+		const uint32_t ex = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+		if (j < nb) bsum[j] = scan_op<OP>(carry, OP == 0 ? incl - v : ex);
+		const uint32_t tot = sh[SCAN_THREADS - 1];
+		__syncthreads();
+		carry = scan_op<OP>(carry, tot);
+	}
+}
+
+template <int OP>
+__global__ void scan_add_kernel(uint32_t *__restrict__ out, int64_t n, const uint32_t *__restrict__ bsum)
+{
+	const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (j < n) out[j] = scan_op<OP>(bsum[j / SCAN_ITEMS], out[j]);
+}
+
+template __global__ void scan_block_kernel<0>(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
+template __global__ void scan_block_kernel<1>(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
+template __global__ void scan_top_kernel<0>(uint32_t *, int64_t);
+template __global__ void scan_top_kernel<1>(uint32_t *, int64_t);
+template __global__ void scan_add_kernel<0>(uint32_t *, int64_t, const uint32_t *);
+template __global__ void scan_add_kernel<1>(uint32_t *, int64_t, const uint32_t *);
+
+// ------------------------------------------------------ position-ordered rows
+// The rows (group, neighbour) of a chunk, e = base[g] + n, listed by patch
+// position: counts per position, exclusive scan -> kstart, scatter, then each
+// position's run sorted by e (runs are short), which is the stable order.
+
+// rows of the chunk: *M = base[Gc] (the scan's total)
+__global__ void count_rows_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
+                                  int64_t Mub, uint32_t *__restrict__ cnt)
+{
+	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (e < Mub && e < (int64_t)*Mp) atomicAdd(&cnt[epos[e]], 1u);
+}
+
+__global__ void scatter_rows_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
+                                    int64_t Mub, const uint32_t *__restrict__ kstart, uint32_t *__restrict__ fill,
+                                    uint32_t *__restrict__ svals)
+{
+	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (e >= Mub || e >= (int64_t)*Mp) return;
+	const uint32_t k = epos[e];
+	svals[kstart[k] + atomicAdd(&fill[k], 1u)] = (uint32_t)e;
+}
+
+__global__ void sort_runs_kernel(const uint32_t *__restrict__ kstart, int64_t npos, uint32_t *__restrict__ svals)
+{
+	const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (k >= npos) return;
+	const uint32_t a = kstart[k], b = kstart[k + 1];
+	for (uint32_t i = a + 1; i < b; i++) {
+		const uint32_t v = svals[i];
+		uint32_t j = i;
+		while (j > a && svals[j - 1] > v) {
+			svals[j] = svals[j - 1];
+			j--;
+		}
+		svals[j] = v;
+	}
+}
+
+// running totals of the run (statistics read once at its end)
+__global__ void add_rows_kernel(const uint32_t *__restrict__ Mp, unsigned long long *__restrict__ total)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(total, (unsigned long long)*Mp);
+}
+
+// ------------------------------------------------------------ early-out fixup
+// A group whose patches are all equal is not filtered (nldct.cpp:1986-1999,
+// 2222-2235) and is aggregated with the weights the previous groups left in
+// aggreWeights: slot n holds the weight of the latest filtered group of the
+// same tile with more than n neighbours, 1 at the tile's start
+// (nldct.cpp:749, 818).  lastf[g] = 1 + index of the latest filtered group
+// at or before g (inclusive max-scan), 0 when none.  One thread per group of
+// the chunk; rows of early groups hold the raw patches and are scaled here.
+__global__ void early_fix_kernel(const uint8_t *__restrict__ early, const uint32_t *__restrict__ nsim,
+                                 const float *__restrict__ gweight, const uint32_t *__restrict__ lastf,
+                                 const uint32_t *__restrict__ gtile, const uint32_t *__restrict__ tile_first,
+                                 const uint32_t *__restrict__ base, int g0, int Gc, const float *__restrict__ win,
+                                 int C, int P, float *__restrict__ est, float *__restrict__ entry_w)
+{
+	const int gl = blockIdx.x * blockDim.x + threadIdx.x;
+	if (gl >= Gc) return;
+	const int g = g0 + gl;
+	if (!early[g]) return;
+	const int N = (int)nsim[g];
+	const int tf = (int)tile_first[gtile[g]];
+	for (int n = 0; n < N; n++) {
+		float w = 1.f;
+		int gp = g - 1 >= tf ? (int)lastf[g - 1] - 1 : -1;
+		while (gp >= tf && (int)nsim[gp] <= n) gp = gp - 1 >= tf ? (int)lastf[gp - 1] - 1 : -1;
+		if (gp >= tf) w = gweight[gp];
+		const uint32_t row = base[gl] + (uint32_t)n;
+		float *r = est + (size_t)row * C * P;
+		for (int i = 0; i < C * P; i++) r[i] = (w * win[i % P]) * r[i];
+		entry_w[row] = w;
+	}
+}
+
+// v[g] = 1 + g for filtered groups, 0 for early ones (input of the max-scan)
+__global__ void filtered_index_kernel(const uint8_t *__restrict__ early, int n, uint32_t *__restrict__ v)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g < n) v[g] = early[g] ? 0u : (uint32_t)g + 1u;
+}
+
+} // namespace bm4dof
