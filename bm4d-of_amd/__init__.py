@@ -69,10 +69,21 @@ EXPORTS = [
     "bm4dof_last_timings", "bm4dof_last_stats", "Dual_TVL1_optic_flow_multiscale", "bm4dof_tvl1",
     "bm4dof_tvl1_sequence", "bm4dof_ctx_set_option", "bm4dof_mlp_set", "bm4dof_mlp_get",
     "bm4dof_mlp_forward", "bm4dof_mlp_grad", "bm4dof_mlp_train", "bm4dof_learned_pairs",
-    "bm4dof_run_learned", "bm4dof_run_learned_dev",
+    "bm4dof_run_learned", "bm4dof_run_learned_dev", "bm4dof_batch_create", "bm4dof_batch_run",
+    "bm4dof_batch_last_ms", "bm4dof_batch_report", "bm4dof_batch_destroy", "bm4dof_autotune",
 ]
 
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
+
 MLP_NPARAMS = 33344
+
+
+class Seq(C.Structure):
+    """bm4dof_seq: one gray sequence of a batch (device pointers)."""
+    _fields_ = [("noisy", C.c_void_p), ("fflow", C.c_void_p), ("bflow", C.c_void_p),
+                ("basic", C.c_void_p), ("final", C.c_void_p), ("w", C.c_uint), ("h", C.c_uint),
+                ("f", C.c_uint)]
 
 
 class BM4DOFError(RuntimeError):
@@ -137,6 +148,17 @@ def lib():
     L.bm4dof_learned_pairs.argtypes = [vp, P, vp, vp, C.c_uint, C.c_uint, C.c_uint, vp, vp, C.c_int,
                                        C.c_int, C.c_float, vp, vp, vp, C.c_int64,
                                        C.POINTER(C.c_int64)]
+    L.bm4dof_batch_create.argtypes = [vp, C.POINTER(Seq), P, P, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_void_p)]
+    L.bm4dof_batch_run.argtypes = [vp]
+    L.bm4dof_batch_last_ms.argtypes = [vp]
+    L.bm4dof_batch_last_ms.restype = C.c_float
+    L.bm4dof_batch_report.argtypes = [vp]
+    L.bm4dof_batch_report.restype = C.c_char_p
+    L.bm4dof_batch_destroy.argtypes = [vp]
+    L.bm4dof_batch_destroy.restype = None
+    L.bm4dof_autotune.argtypes = [vp, P, P, vp, C.c_uint, C.c_uint, C.c_uint, vp, vp, vp, vp, C.c_int,
+                                  C.c_char_p, C.c_int]
     learned = [vp, P, P, vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, vp, vp, vp, C.c_int, C.c_float,
                C.POINTER(C.c_float)]
     L.bm4dof_run_learned.argtypes = learned
@@ -291,6 +313,26 @@ class Context:
                                       tile.ctypes.data))
         return idx.reshape(G, npm), nsim, ref, tile
 
+    # ------------------------------------------------------------- batches
+
+    def batch(self, seqs, p1s, p2s, nparts=2, graph=True, autotune=False):
+        """bm4dof_batch_create over Seq descriptors; returns a Batch."""
+        n = len(seqs)
+        arr = (Seq * n)(*seqs)
+        a1 = (Params * n)(*p1s)
+        a2 = (Params * n)(*p2s)
+        h = C.c_void_p()
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
+        self._check(self._L.bm4dof_batch_create(self.h, arr, a1, a2, n, nparts, flags, C.byref(h)))
+        return Batch(self, h)
+
+    def autotune(self, p1, p2, noisy_ptr, shape, ff_ptr, bf_ptr, basic_ptr, final_ptr, nparts=2):
+        F, Cc, H, W = shape
+        rep = C.create_string_buffer(4096)
+        self._check(self._L.bm4dof_autotune(self.h, C.byref(p1), C.byref(p2), noisy_ptr, W, H, F, ff_ptr,
+                                            bf_ptr, basic_ptr, final_ptr, nparts, rep, 4096))
+        return rep.value.decode()
+
     # ------------------------------------------------------------- learned pass
 
     def mlp_set(self, theta):
@@ -396,3 +438,29 @@ class Context:
         prm = TVL1Params(tau, lam, theta, nscales, fscale, zfactor, warps, eps, 0)
         self._check(self._L.bm4dof_tvl1_sequence(self.h, C.byref(prm), frames_ptr, W, H, F,
                                                  fflow_ptr, bflow_ptr, 1))
+
+
+class Batch:
+    """A created batch (bm4dof_batch); run() replays it."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def run(self):
+        self.ctx._check(self.ctx._L.bm4dof_batch_run(self.h))
+        return float(self.ctx._L.bm4dof_batch_last_ms(self.h))
+
+    def report(self):
+        return self.ctx._L.bm4dof_batch_report(self.h).decode()
+
+    def close(self):
+        if self.h:
+            self.ctx._L.bm4dof_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -23,7 +23,7 @@ __global__ void bm_window_kernel(const float *, const float *, const uint32_t *,
                                  const TileBox *, StepConst, int, int4 *, int2 *);
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
-template <int PX, int PT> __global__ void bm_fast_kernel(BMF_ARGS);
+template <int PX, int PT, int NF> __global__ void bm_fast_kernel(BMF_ARGS);
 __global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
                                const uint32_t *, StepConst, int, int, const float *, const float *,
                                const float *, float *, float *, uint32_t *, uint8_t *, float *,
@@ -469,7 +469,8 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	// channel 0; step 2 sums all channels), patches 4x4x4 or 8x8x1
 	const int PX = sc.sPx, PT = sc.sPt;
 	const bool shape_ok = (PX == 4 && PT == 4) || (PX == 8 && PT == 1);
-	const size_t stage_floats = std::max<size_t>(2 * (size_t)(sc.sWx + PX - 1) * (sc.sWx + PX - 1) * PT,
+	const int NF = ctx->bm_frames;   // frames staged per round: 1, 2 or 4
+	const size_t stage_floats = std::max<size_t>((size_t)NF * (sc.sWx + PX - 1) * (sc.sWx + PX - 1) * PT,
 	                                             (size_t)maxCand);
 	const size_t lds_fast = ((size_t)maxCand + stage_floats) * 4;
 	if (ctx->fast_bm && shape_ok && (step == 1 || sc.C == 1) && lds_fast <= 56 * 1024) {
@@ -478,7 +479,9 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 		hipLaunchKernelGGL(bm_window_kernel, dim3(nblk(G, 256)), dim3(256), 0, ctx->stream, ff, bf, d_gpos,
 		                   d_gtile, d_tiles, sc, G, d_wins, d_wmeta);
 		HIPCHK(hipGetLastError());
-		auto kern = PX == 4 ? bm_fast_kernel<4, 4> : bm_fast_kernel<8, 1>;
+		auto kern = PX == 4 ? bm_fast_kernel<4, 4, 2> : bm_fast_kernel<8, 1, 2>;
+		if (NF == 1) kern = PX == 4 ? bm_fast_kernel<4, 4, 1> : bm_fast_kernel<8, 1, 1>;
+		else if (NF == 4) kern = PX == 4 ? bm_fast_kernel<4, 4, 4> : bm_fast_kernel<8, 1, 4>;
 		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_fast,
 		                   ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile, d_tiles, sc, (int)stage_floats,
 		                   d_idx, d_nsim, G);
@@ -1061,6 +1064,82 @@ int64_t learned_pairs_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const float 
 	return rows;
 }
 
+// --------------------------------------------------------------- autotuner
+// Execution options that do not change the output (block-matching staging
+// depth, aggregation e-range capacity) are timed on a probe sequence and the
+// fastest set is kept per shape class (w, h, f, patch, search window).
+struct TuneChoice { int bm_frames, agg_cap; };
+
+void apply_choice(bm4dof_ctx *ctx, const TuneChoice &c)
+{
+	ctx->bm_frames = c.bm_frames;
+	ctx->agg_cap = c.agg_cap;
+}
+
+// Times both steps of one gray sequence (device buffers) under every
+// candidate; returns the fastest choice and fills `report`.
+TuneChoice autotune(bm4dof_ctx *ctx, const bm4dof_params &p1, const bm4dof_params &p2, const float *noisy,
+                    int W, int H, int F, const float *ff, const float *bf, float *basic, float *final,
+                    int nparts, std::string &report)
+{
+	static const int frames[] = {1, 2, 4};
+	static const int caps[] = {768, 1024, 1536};
+	TuneChoice best{ctx->bm_frames, ctx->agg_cap};
+	float best_ms = 3.4e38f;
+	const TuneChoice saved{ctx->bm_frames, ctx->agg_cap};
+	char line[160];
+	for (int fr : frames)
+		for (int cap : caps) {
+			apply_choice(ctx, {fr, cap});
+			run_dev(ctx, &p1, &p2, noisy, W, H, F, 1, ff, bf, basic, final, nparts, nullptr);   // warm
+			run_dev(ctx, &p1, &p2, noisy, W, H, F, 1, ff, bf, basic, final, nparts, nullptr);
+			float ms = 0.f;
+			for (auto &t : ctx->timings) ms += t.second;
+			snprintf(line, sizeof line, "bm_frames=%d agg_cap=%d: %.3f ms\n", fr, cap, ms);
+			report += line;
+			if (ms < best_ms) { best_ms = ms; best = {fr, cap}; }
+		}
+	apply_choice(ctx, saved);
+	snprintf(line, sizeof line, "chosen bm_frames=%d agg_cap=%d (%.3f ms)\n", best.bm_frames, best.agg_cap, best_ms);
+	report += line;
+	return best;
+}
+
+} // namespace
+
+// --------------------------------------------------------------- batches
+// Many gray sequences of uneven size and noise level, both steps each, as
+// one launch set: planned and uploaded once, then either enqueued eagerly or
+// captured into one hipGraph that every bm4dof_batch_run replays.
+struct bm4dof_batch {
+	bm4dof_ctx *ctx = nullptr;
+	std::vector<bm4dof_seq> seqs;
+	std::vector<StepPlan> plans;          // 2 per sequence
+	std::vector<DevBuf> geo;              // 2 per sequence
+	std::vector<StepDev> dev;             // 2 per sequence
+	std::vector<TuneChoice> choice;       // per sequence
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	float last_ms = 0.f;
+	std::string report;
+};
+
+namespace {
+
+void batch_enqueue(bm4dof_batch *b)
+{
+	bm4dof_ctx *ctx = b->ctx;
+	for (size_t i = 0; i < b->seqs.size(); i++) {
+		const bm4dof_seq &q = b->seqs[i];
+		apply_choice(ctx, b->choice[i]);
+		const float *ff = (q.fflow && q.bflow) ? q.fflow : nullptr;
+		const float *bf = (q.fflow && q.bflow) ? q.bflow : nullptr;
+		enqueue_step(ctx, b->plans[2 * i], b->dev[2 * i], q.noisy, nullptr, ff, bf, q.basic, nullptr, nullptr);
+		enqueue_step(ctx, b->plans[2 * i + 1], b->dev[2 * i + 1], q.noisy, q.basic, ff, bf, q.final, nullptr,
+		             nullptr);
+	}
+}
+
 } // namespace
 
 // ====================================================================== C-ABI
@@ -1163,7 +1242,10 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
 		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
 		else if (k == "fast_bm") ctx->fast_bm = value != 0;
-		else if (k == "agg_cap") {
+		else if (k == "bm_frames") {
+			if (value != 1 && value != 2 && value != 4) throw ArgError("bm_frames must be 1, 2 or 4");
+			ctx->bm_frames = (int)value;
+		} else if (k == "agg_cap") {
 			if (value != 512 && value != 768 && value != 1024 && value != 1536)
 				throw ArgError("agg_cap must be 512, 768, 1024 or 1536");
 			ctx->agg_cap = (int)value;
@@ -1183,6 +1265,143 @@ int bm4dof_run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params
 		run_dev(ctx, p1, p2, noisy, (int)w, (int)h, (int)f, (int)c, (ff && bf) ? ff : nullptr,
 		        (ff && bf) ? bf : nullptr, basic, final, nparts, groups_ratio);
 	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_autotune(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, const float *noisy,
+                    unsigned w, unsigned h, unsigned f, const float *fflow, const float *bflow, float *basic,
+                    float *final, int nparts, char *report, int report_len)
+{
+	if (!ctx || !p1 || !p2 || !noisy || !basic || !final) { set_err(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		std::string rep;
+		const TuneChoice c = autotune(ctx, *p1, *p2, noisy, (int)w, (int)h, (int)f, (fflow && bflow) ? fflow : nullptr,
+		                              (fflow && bflow) ? bflow : nullptr, basic, final, nparts, rep);
+		apply_choice(ctx, c);
+		if (report && report_len > 0) {
+			strncpy(report, rep.c_str(), (size_t)report_len - 1);
+			report[report_len - 1] = 0;
+		}
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_params *p1s,
+                        const bm4dof_params *p2s, int n, int nparts, int flags, bm4dof_batch **out)
+{
+	if (!ctx || !seqs || !p1s || !p2s || !out || n <= 0) { set_err(ctx, "bad batch arguments"); return BM4DOF_EINVAL; }
+	*out = nullptr;
+	bm4dof_batch *b = new bm4dof_batch;
+	b->ctx = ctx;
+	const int rc = guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		if (nparts < 1 || nparts > 255) throw ArgError("nparts must be in 1..255");
+		b->seqs.assign(seqs, seqs + n);
+		b->plans.reserve(2 * n);
+		b->geo.resize(2 * n);
+		b->dev.resize(2 * n);
+		const TuneChoice saved{ctx->bm_frames, ctx->agg_cap};
+		b->choice.assign(n, saved);
+		std::vector<std::pair<std::string, TuneChoice>> tuned;
+		for (int i = 0; i < n; i++) {
+			const bm4dof_seq &q = seqs[i];
+			if (!q.noisy || !q.basic || !q.final) throw ArgError("sequence " + std::to_string(i) + ": NULL buffer");
+			if (!p1s[i].sizePatch || !p2s[i].sizePatch) throw ArgError("batches run both steps");
+			const int W = (int)q.w, H = (int)q.h, F = (int)q.f;
+			if (W <= 0 || H <= 0 || F <= 0) throw ArgError("sequence " + std::to_string(i) + ": empty");
+			if ((uint64_t)W * H * F >= (1ull << 32)) throw ArgError("sequence above 2^32 samples", BM4DOF_EUNSUPP);
+			const bool flow = q.fflow && q.bflow;
+			bm4dof_params a = p1s[i], c = p2s[i];
+			a.isFirstStep = 1;
+			c.isFirstStep = 0;
+			b->plans.push_back(plan_step(ctx, a, W, H, F, 1, flow, nparts, false));
+			b->plans.push_back(plan_step(ctx, c, W, H, F, 1, flow, nparts, false));
+			for (int s = 0; s < 2; s++) {
+				const StepPlan &pl = b->plans[2 * i + s];
+				if ((flags & BM4DOF_BATCH_GRAPH) && (pl.sc.wTmp < 8 || pl.sc.hTmp < 8 || !ctx->agg_chunked))
+					throw ArgError("a captured batch needs tiles of at least 8 x 8 pixels", BM4DOF_EUNSUPP);
+				b->dev[2 * i + s] = upload_plan(ctx, pl, b->geo[2 * i + s]);
+				reserve_step(ctx, pl);
+			}
+			if (flags & BM4DOF_BATCH_AUTOTUNE) {
+				char key[96];
+				snprintf(key, sizeof key, "%dx%dx%d px%u pt%u wx%u np%u", W, H, F, a.sizePatch, a.sizePatchTime,
+				         a.sizeSearchWindow, a.nSimilarPatches);
+				auto it = std::find_if(tuned.begin(), tuned.end(), [&](const auto &t) { return t.first == key; });
+				if (it == tuned.end()) {
+					std::string rep;
+					const TuneChoice ch = autotune(ctx, a, c, q.noisy, W, H, F, flow ? q.fflow : nullptr,
+					                               flow ? q.bflow : nullptr, q.basic, q.final, nparts, rep);
+					b->report += std::string(key) + ":\n" + rep;
+					tuned.push_back({key, ch});
+					b->choice[i] = ch;
+				} else
+					b->choice[i] = it->second;
+			}
+		}
+		// every buffer the launches use exists now: nothing allocates below
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		if (flags & BM4DOF_BATCH_GRAPH) {
+			HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+			try {
+				batch_enqueue(b);
+			} catch (...) {
+				hipGraph_t g = nullptr;
+				(void)hipStreamEndCapture(ctx->stream, &g);
+				if (g) (void)hipGraphDestroy(g);
+				throw;
+			}
+			HIPCHK(hipStreamEndCapture(ctx->stream, &b->graph));
+			HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+		}
+		apply_choice(ctx, saved);
+	});
+	if (rc) {
+		bm4dof_batch_destroy(b);
+		return rc;
+	}
+	*out = b;
+	return BM4DOF_OK;
+}
+
+__attribute__((visibility("default")))
+int bm4dof_batch_run(bm4dof_batch *b)
+{
+	if (!b || !b->ctx) return BM4DOF_EINVAL;
+	bm4dof_ctx *ctx = b->ctx;
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		const TuneChoice saved{ctx->bm_frames, ctx->agg_cap};
+		hipEvent_t e0, e1;
+		HIPCHK(hipEventCreate(&e0));
+		HIPCHK(hipEventCreate(&e1));
+		HIPCHK(hipEventRecord(e0, ctx->stream));
+		if (b->exec) HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
+		else batch_enqueue(b);
+		HIPCHK(hipEventRecord(e1, ctx->stream));
+		HIPCHK(hipEventSynchronize(e1));
+		HIPCHK(hipEventElapsedTime(&b->last_ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		apply_choice(ctx, saved);
+	});
+}
+
+__attribute__((visibility("default")))
+float bm4dof_batch_last_ms(const bm4dof_batch *b) { return b ? b->last_ms : 0.f; }
+
+__attribute__((visibility("default")))
+const char *bm4dof_batch_report(const bm4dof_batch *b) { return b ? b->report.c_str() : ""; }
+
+__attribute__((visibility("default")))
+void bm4dof_batch_destroy(bm4dof_batch *b)
+{
+	if (!b) return;
+	if (b->ctx) (void)hipSetDevice(b->ctx->device);
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	for (auto &g : b->geo) g.release();
+	delete b;
 }
 
 __attribute__((visibility("default")))

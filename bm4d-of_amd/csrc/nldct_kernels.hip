@@ -344,7 +344,7 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 // the shifted search window of every frame of the temporal range
 // (nldct.cpp:1173-1259), in the reference's frame schedule, as
 // (qt, rx0 | ry0 << 16, nx | ny << 16, first candidate index).
-// bm_fast_kernel: one wave per group; the search regions of two frames at a
+// bm_fast_kernel: one wave per group; the search regions of NF frames at a
 // time are staged in LDS, lane = candidate, and each distance is the
 // sequential (ht, hy, hx) sum of the reference (nldct.cpp:1262-1274,
 // 1723-1737).  Selection as bm_kernel.
@@ -407,7 +407,7 @@ __global__ void bm_window_kernel(const float *__restrict__ fflow, const float *_
 	wmeta[g] = make_int2(nf, base);
 }
 
-template <int PX, int PT>
+template <int PX, int PT, int NF>
 __global__ __launch_bounds__(64)
 void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
                     const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
@@ -457,24 +457,28 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 			for (int hx = 0; hx < PX; hx++)
 				ref[(ht * PX + hy) * PX + hx] = img[(size_t)(gt + ht) * sc.WHC + (size_t)(gy + hy) * sc.W + gx + hx];
 
-	for (int ii = 0; ii < nframes; ii += 2) {
-		// stage frames ii and ii+1
-		int4 f[2];
-		int rw[2], rh[2], off[2], ncand[2];
-		int tot = 0;
+	for (int ii = 0; ii < nframes; ii += NF) {
+		// stage frames ii .. ii+NF-1
+		int4 f[NF];
+		int rw[NF], rh[NF], off[NF], cst[NF], ncand[NF];
+		int tot = 0, n01 = 0;
 #pragma unroll
-		for (int j = 0; j < 2; j++) {
+		for (int j = 0; j < NF; j++) {
 			f[j] = ii + j < nframes ? gw[ii + j] : make_int4(0, 0, 0, 0);
 			const int nx = f[j].z & 0xffff, ny = f[j].z >> 16;
 			rw[j] = nx + PX - 1;
 			rh[j] = ny + PX - 1;
 			ncand[j] = (ii + j < nframes) ? nx * ny : 0;
 			off[j] = tot;
+			cst[j] = n01;
 			tot += ncand[j] ? rw[j] * rh[j] * PT : 0;
+			n01 += ncand[j];
 		}
 		__syncthreads();
 		for (int i = lane; i < tot; i += 64) {
-			const int j = i >= off[1] && ncand[1] ? 1 : 0;
+			int j = 0;
+#pragma unroll
+			for (int jj = 1; jj < NF; jj++) if (ncand[jj] && i >= off[jj]) j = jj;
 			const int r = i - off[j];
 			const int ht = r / (rw[j] * rh[j]);
 			const int q = r - ht * rw[j] * rh[j];
@@ -483,10 +487,11 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 			stage[i] = img[(size_t)(f[j].x + ht) * sc.WHC + (size_t)ay * sc.W + ax];
 		}
 		__syncthreads();
-		const int n01 = ncand[0] + ncand[1];
 		for (int o = lane; o < n01; o += 64) {
-			const int j = o >= ncand[0] ? 1 : 0;
-			const int lo = o - (j ? ncand[0] : 0);
+			int j = 0;
+#pragma unroll
+			for (int jj = 1; jj < NF; jj++) if (ncand[jj] && o >= cst[jj]) j = jj;
+			const int lo = o - cst[j];
 			const int nx = f[j].z & 0xffff;
 			const int cy = lo / nx, cx = lo - cy * nx;
 			const float *b = stage + off[j] + cy * rw[j] + cx;
@@ -609,8 +614,12 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
-template __global__ void bm_fast_kernel<4, 4>(BMF_ARGS);
-template __global__ void bm_fast_kernel<8, 1>(BMF_ARGS);
+template __global__ void bm_fast_kernel<4, 4, 1>(BMF_ARGS);
+template __global__ void bm_fast_kernel<4, 4, 2>(BMF_ARGS);
+template __global__ void bm_fast_kernel<4, 4, 4>(BMF_ARGS);
+template __global__ void bm_fast_kernel<8, 1, 1>(BMF_ARGS);
+template __global__ void bm_fast_kernel<8, 1, 2>(BMF_ARGS);
+template __global__ void bm_fast_kernel<8, 1, 4>(BMF_ARGS);
 
 // ---------------------------------------------------------------------------
 // Group filtering, one workgroup per group.

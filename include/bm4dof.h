@@ -110,6 +110,8 @@ const char *bm4dof_global_error(void);
  *   "fast_bm"      1 (default): block matching by one wave per group on
  *                  LDS-staged search windows (patches 4x4x4 or 8x8x1, one
  *                  channel in the distance); 0: the general kernel.
+ *   "bm_frames"    frames of search window staged in LDS per round of the
+ *                  staged block matching: 1, 2 (default) or 4.
  *   "agg_chunked"  1 (default): ordered aggregation by sorted row chunks
  *                  with per-pixel masks (tiles at least 8 pixels wide and
  *                  high); 0: the banded aggregation for every block.
@@ -186,6 +188,42 @@ int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, in
  * stats[5..6] = groups and group rows (patches over all groups) of step 1,
  * stats[7..8] = the same for step 2.  Returns the number of slots filled. */
 int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
+
+/* ---- batches and the autotuner ----
+ *
+ * No counterpart in the reference (it denoises one video per process).  A
+ * batch runs both steps on many one-channel sequences of uneven size and
+ * noise level as one launch set: planned and uploaded at creation, then
+ * either enqueued on every run or captured once into a hipGraph
+ * (BM4DOF_BATCH_GRAPH) that every run replays.  Each sequence's output is
+ * bitwise the output of bm4dof_run_dev on it alone (same nparts).  With
+ * BM4DOF_BATCH_AUTOTUNE the execution options that cannot change results
+ * ("bm_frames", "agg_cap") are timed on the first sequence of every shape
+ * class and the fastest set is used for that class.  Buffers are device
+ * pointers and must stay valid for the batch's lifetime; a captured batch
+ * holds its scratch memory in the context, so runs on that context of other
+ * work in between are allowed but not concurrently.
+ */
+typedef struct bm4dof_seq {
+	const float *noisy;          /* w*h*f, one channel */
+	const float *fflow, *bflow;  /* w*h*f*2 each, or both NULL */
+	float *basic, *final;        /* outputs, w*h*f each */
+	unsigned w, h, f;
+} bm4dof_seq;
+typedef struct bm4dof_batch bm4dof_batch;
+#define BM4DOF_BATCH_GRAPH 1
+#define BM4DOF_BATCH_AUTOTUNE 2
+int   bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_params *p1s,
+                          const bm4dof_params *p2s, int n, int nparts, int flags, bm4dof_batch **out);
+int   bm4dof_batch_run(bm4dof_batch *b);
+float bm4dof_batch_last_ms(const bm4dof_batch *b);
+const char *bm4dof_batch_report(const bm4dof_batch *b);
+void  bm4dof_batch_destroy(bm4dof_batch *b);
+/* Times both steps on one sequence (device buffers, one channel) under every
+ * candidate option set, keeps the fastest in ctx, and writes a report. */
+int   bm4dof_autotune(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, const float *noisy,
+                      unsigned w, unsigned h, unsigned f, const float *fflow, const float *bflow, float *basic,
+                      float *final, int nparts, char *report, int report_len);
 
 /* ---- learned single pass ----
  *
