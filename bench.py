@@ -102,6 +102,45 @@ def cpu_baseline(noisy, ff, bf, crop, threads):
                       f"{threads} OpenMP threads, nParts={2 * threads}, {dt:.1f} s"}
 
 
+def batch32(A, S, ctx, torch, nparts):
+    """32 gray sequences, seeded: frames uniform in [3, 30], size from
+    {320x240, 640x360, 960x540, 1280x720}, sigma from {10, 20, 40}, px=pt=4,
+    np 32, 11x11 search; both steps of all of them in one captured hipGraph,
+    options picked per shape class by the autotuner.  Flow: none (the batch
+    measures the denoiser)."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    sizes = [(320, 240), (640, 360), (960, 540), (1280, 720)]
+    seqs, keep, p1s, p2s = [], [], [], []
+    pix = 0
+    t0 = time.perf_counter()
+    for i in range(32):
+        w, h = sizes[rng.integers(0, 4)]
+        f = int(rng.integers(3, 31))
+        sigma = float(rng.choice([10.0, 20.0, 40.0]))
+        clean = S.video(w, h, f, seed=7000 + i)
+        noisy = S.add_noise(clean, sigma, 8000 + i)
+        dn = torch.from_numpy(noisy).cuda()
+        db, df = torch.empty_like(dn), torch.empty_like(dn)
+        keep += [dn, db, df]
+        seqs.append(A.Seq(dn.data_ptr(), None, None, db.data_ptr(), df.data_ptr(), w, h, f))
+        p1s.append(A.cli_params(1, sigma, noisy.shape, px=4, pt=4, wx=11, np_=32))
+        p2s.append(A.cli_params(2, sigma, noisy.shape, px=4, pt=4, wx=11, np_=32))
+        pix += w * h * f
+        print(f"[bench] batch sequence {i + 1}/32: {w}x{h}x{f} sigma {sigma:g}", file=sys.stderr, flush=True)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = ctx.batch(seqs, p1s, p2s, nparts=nparts, graph=True, autotune=True)
+    create_s = time.perf_counter() - t0
+    batch.run()   # warm
+    runs = [batch.run() for _ in range(3)]
+    batch.close()
+    ms = float(np.median(runs))
+    return {"value": round(pix / (ms * 1e-3) / 1e6, 4), "unit": "Mpix/s", "sequences": 32,
+            "mpix": round(pix / 1e6, 3), "ms_per_batch": round(ms, 3), "replays_ms": [round(r, 3) for r in runs],
+            "create_s_incl_autotune": round(create_s, 2), "generation_s": round(gen_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +152,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned single pass")
+    ap.add_argument("--batch32", action="store_true",
+                    help="also time the batched workload: 32 uneven gray sequences in one hipGraph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -257,6 +298,8 @@ def main():
             "weights": "bm4d-of_amd/weights/gain_mlp_66_128_128_64.f32",
         }
         print(f"[bench] learned single pass: {lv:.2f} Mpix/s", file=sys.stderr, flush=True)
+    if args.batch32:
+        out["batch32"] = batch32(A, S, ctx, torch, args.nparts)
     ctx.close()
     if not args.no_cpu:
         cw, ch = (int(v) for v in args.cpu_crop.lower().split("x"))

@@ -1325,7 +1325,9 @@ int bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_pa
 			}
 			if (flags & BM4DOF_BATCH_AUTOTUNE) {
 				char key[96];
-				snprintf(key, sizeof key, "%dx%dx%d px%u pt%u wx%u np%u", W, H, F, a.sizePatch, a.sizePatchTime,
+				// shape class: frame size and patch / search parameters (the
+				// frame count scales the work but not the kernels' balance)
+				snprintf(key, sizeof key, "%dx%d px%u pt%u wx%u np%u", W, H, a.sizePatch, a.sizePatchTime,
 				         a.sizeSearchWindow, a.nSimilarPatches);
 				auto it = std::find_if(tuned.begin(), tuned.end(), [&](const auto &t) { return t.first == key; });
 				if (it == tuned.end()) {
