@@ -15,7 +15,15 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbm4dof.so")
+IO_PATH = os.path.join(_HERE, "libbm4dof_io.so")
+VIDEONLB_PATH = os.path.join(_HERE, "libbm4dof_videonlb.so")
+# command-line tools: bm4d-of (src/main_vnlb.cpp), tvl1flow
+# (lib/tvl1flow/main.c) and tvl1flow-seq (scripts/tvl1flow-seq.sh)
+TOOLS = {name: os.path.join(_HERE, "bin", name) for name in ("bm4d-of", "tvl1flow", "tvl1flow-seq")}
+IO_EXPORTS = ["bm4dof_io_error", "bm4dof_io_free", "bm4dof_io_read", "bm4dof_io_read_gray",
+              "bm4dof_io_write"]
 _LIB = None
+_IO = None
 
 BM4DOF_RGB, BM4DOF_YUV = 0, 1
 BM4DOF_DCT, BM4DOF_BIOR1_5 = 0, 1
@@ -464,3 +472,59 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# --------------------------------------------------------------- image files
+
+def io_lib():
+    """Load libbm4dof_io.so (host code: PNG, TIFF and .flo files)."""
+    global _IO
+    if _IO is not None:
+        return _IO
+    if not os.path.exists(IO_PATH):
+        raise ImportError(f"{IO_PATH} is missing: build it with `make -C {_HERE}`")
+    L = C.CDLL(IO_PATH)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int)
+    L.bm4dof_io_read.argtypes = [C.c_char_p, C.POINTER(vp), ip, ip, ip]
+    L.bm4dof_io_read_gray.argtypes = [C.c_char_p, C.POINTER(vp), ip, ip]
+    L.bm4dof_io_write.argtypes = [C.c_char_p, vp, C.c_int, C.c_int, C.c_int]
+    L.bm4dof_io_free.argtypes = [vp]
+    L.bm4dof_io_free.restype = None
+    L.bm4dof_io_error.restype = C.c_char_p
+    _IO = L
+    return L
+
+
+def _take(L, ptr, n):
+    out = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_float)), shape=(n,)).copy()
+    L.bm4dof_io_free(ptr)
+    return out
+
+
+def read_image(path):
+    """A PNG, TIFF or .flo file as a (C, H, W) float32 array."""
+    L = io_lib()
+    ptr, w, h, c = C.c_void_p(), C.c_int(), C.c_int(), C.c_int()
+    if L.bm4dof_io_read(os.fsencode(path), C.byref(ptr), C.byref(w), C.byref(h), C.byref(c)):
+        raise OSError(L.bm4dof_io_error().decode())
+    return _take(L, ptr, c.value * h.value * w.value).reshape(c.value, h.value, w.value)
+
+
+def read_gray(path):
+    """One gray (H, W) channel, read the way tvl1flow reads its frames."""
+    L = io_lib()
+    ptr, w, h = C.c_void_p(), C.c_int(), C.c_int()
+    if L.bm4dof_io_read_gray(os.fsencode(path), C.byref(ptr), C.byref(w), C.byref(h)):
+        raise OSError(L.bm4dof_io_error().decode())
+    return _take(L, ptr, h.value * w.value).reshape(h.value, w.value)
+
+
+def write_image(path, img):
+    """Write a (C, H, W) or (H, W) array; the format follows the extension."""
+    a = np.ascontiguousarray(img, np.float32)
+    if a.ndim == 2:
+        a = a[None]
+    L = io_lib()
+    if L.bm4dof_io_write(os.fsencode(path), a.ctypes.data, a.shape[2], a.shape[1], a.shape[0]):
+        raise OSError(L.bm4dof_io_error().decode())
