@@ -60,3 +60,17 @@ def test_dual_tvl1_dropin_symbol(ctx):
                                             0.01, False)
     uo, vo, _ = O.tvl1(I0, I1, nscales=3, warps=3)
     assert np.sqrt((u - uo) ** 2 + (v - vo) ** 2).mean() < 1e-3
+
+
+def test_tvl1_matches_reference_vectors(ctx):
+    """Against flows of the reference's compiled lib/tvl1flow committed under
+    tests/golden (same criterion as against the oracle)."""
+    from test_golden import tvl1_cases
+    for i, I0, I1, ur, vr, kw in tvl1_cases():
+        print("golden case", i, kw, flush=True)
+        u, v, it = ctx.tvl1(I0, I1, **kw)
+        _, _, ito = O.tvl1(I0, I1, **kw)
+        if np.array_equal(it, ito):
+            assert np.array_equal(u, ur) and np.array_equal(v, vr)
+        else:
+            assert np.sqrt((u - ur) ** 2 + (v - vr) ** 2).mean() < 1e-3

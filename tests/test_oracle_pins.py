@@ -104,6 +104,20 @@ def test_dct_basis_is_scipy_orthonormal_dct(px, pt):
         assert np.all(win == 1)
 
 
+@pytest.mark.parametrize("px,pt", [(8, 1), (8, 2), (10, 1)])
+def test_kaiser_window_is_outer_product_of_kaiser_beta2(px, pt):
+    """The aggregation windows (nldct.cpp:698-737) are BM3D's: the outer
+    product of a 1-D Kaiser window with beta = 2, rounded to 4 decimals,
+    repeated over the patch frames."""
+    p = O.cli_params(1, 20, 1, px=px, pt=1, wx=21)
+    p.sizePatchTime = pt
+    p.agg_window = 1
+    _, _, win = O.bases(p)
+    k = np.kaiser(px, 2.0)
+    want = np.tile(np.round(np.outer(k, k), 4).ravel(), pt).astype(np.float32)
+    assert np.array_equal(win, want)
+
+
 def test_bior_tables_are_not_perfect_reconstruction():
     """bior1.5 (nldct.cpp:640-695): the 2-D forward basis is the Kronecker
     square of a 1-D analysis table F, the inverse transform the Kronecker
