@@ -383,23 +383,49 @@ struct StepGeom {
 	std::vector<uint32_t> frame_first;      // first group of every (tile, reference frame)
 };
 
-StepGeom make_geometry(const bm4dof_params &p, int W, int H, int F, int nparts)
+// Reference positions of a step, tile by tile in the reference's processing
+// order (nldct.cpp:756-760).  For C > 1 the mask clear after each group,
+// io_mask(index[0]) = false (nldct.cpp:2325-2329), is applied: index[0] is
+// the reference patch itself (it comes first in the neighbour order) but in
+// the C-channel index t*whc + y*w + x, so it clears grid position
+// (x, y, C*t) of the one-channel mask, which is then skipped; indices past
+// the mask (t >= F/C) are dropped (undefined in the reference).  For C = 1
+// the clear only touches the group's own position.  `mask_clear` false gives
+// every grid position (the neighbour-list entry points).
+StepGeom make_geometry(const bm4dof_params &p, int W, int H, int F, int nparts, int C = 1,
+                       bool mask_clear = true)
 {
 	StepGeom g;
 	g.til = make_tiling(W, H, (int)p.boundary, nparts);
 	std::vector<int> xs, ys, ts;
+	std::vector<uint8_t> mask;
 	const uint32_t WH = (uint32_t)W * (uint32_t)H;
 	for (int n = 0; n < nparts; n++) {
 		const TileBox &c = g.til.crop[n];
 		g.tile_first.push_back((uint32_t)g.gpos.size());
-		tile_grid(p, c.ex - c.ox, c.ey - c.oy, F, c.ox > 0, c.oy > 0, c.ex < W, c.ey < H, xs, ys, ts);
+		const int wc = c.ex - c.ox, hc = c.ey - c.oy;
+		tile_grid(p, wc, hc, F, c.ox > 0, c.oy > 0, c.ex < W, c.ey < H, xs, ys, ts);
+		const bool clear = mask_clear && C > 1 && !p.doPasteBoost;
+		const size_t whc = (size_t)wc * hc, whf = whc * F;
+		if (clear) {
+			mask.assign(whf, 0);
+			for (int t : ts)
+				for (int y : ys)
+					for (int x : xs) mask[(size_t)t * whc + (size_t)y * wc + x] = 1;
+		}
 		for (int t : ts) {
-			if (!xs.empty() && !ys.empty()) g.frame_first.push_back((uint32_t)g.gpos.size());
+			const uint32_t before = (uint32_t)g.gpos.size();
 			for (int y : ys)
 				for (int x : xs) {
+					if (clear) {
+						if (!mask[(size_t)t * whc + (size_t)y * wc + x]) continue;
+						const size_t i0 = (size_t)t * whc * C + (size_t)y * wc + x;
+						if (i0 < whf) mask[i0] = 0;
+					}
 					g.gpos.push_back((uint32_t)t * WH + (uint32_t)(y + c.oy) * (uint32_t)W + (uint32_t)(x + c.ox));
 					g.gtile.push_back((uint32_t)n);
 				}
+			if (g.gpos.size() > before) g.frame_first.push_back(before);
 		}
 	}
 	return g;
@@ -624,7 +650,7 @@ StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F,
 	pl.learned = learned;
 	check_step_params(p, (unsigned)C);
 	build_bases(p, pl.hb, pl.hbi, pl.hw);
-	pl.geo = make_geometry(p, W, H, F, nparts);
+	pl.geo = make_geometry(p, W, H, F, nparts, C);
 	pl.sc = make_const(p, W, H, F, C, use_flow, pl.geo.til);
 	pl.G = (int)pl.geo.gpos.size();
 	pl.np = pl.sc.npmax;
@@ -938,7 +964,7 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 	for (int s = 0; s < 2; s++) {
 		if ((s == 0 && step == 2) || (s == 1 && step == 1)) continue;
 		const bm4dof_params &p = s == 0 ? *p1 : *p2;
-		StepGeom geo = make_geometry(p, W, H, F, nparts);
+		StepGeom geo = make_geometry(p, W, H, F, nparts, C);
 		float r = 0.f;
 		for (int t = 0; t < nparts; t++) {
 			const uint32_t g0 = geo.tile_first[t];
@@ -1502,7 +1528,7 @@ int bm4dof_bm_count(bm4dof_ctx *ctx, const bm4dof_params *p, unsigned w, unsigne
 {
 	(void)c;
 	return guarded(ctx, [&] {
-		StepGeom geo = make_geometry(*p, (int)w, (int)h, (int)f, nparts);
+		StepGeom geo = make_geometry(*p, (int)w, (int)h, (int)f, nparts, 1, false);
 		*n_groups = (int64_t)geo.gpos.size();
 	});
 }
@@ -1518,7 +1544,7 @@ int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image, unsig
 		check_step_params(*p, c);
 		const int W = (int)w, H = (int)h, F = (int)f, C = (int)c;
 		const bool flow = ff && bf;
-		StepGeom geo = make_geometry(*p, W, H, F, nparts);
+		StepGeom geo = make_geometry(*p, W, H, F, nparts, C, false);
 		StepConst sc = make_const(*p, W, H, F, C, flow, geo.til);
 		const int G = (int)geo.gpos.size();
 		const size_t n = (size_t)W * H * F * C, nf = (size_t)W * H * F * 2;

@@ -191,6 +191,30 @@ def test_color_image_parity(ctx):
     _same(gf2, of2, 'final from CPU basic')
 
 
+@pytest.mark.parametrize("channels,flow", [(3, False), (3, True), (4, True)])
+def test_color_video_parity(ctx, channels, flow):
+    """Colour video: opponent transform, step-1 matching on Y, step-2 on all
+    channels, and the mask clear with the multichannel index
+    (nldct.cpp:2325-2329) that skips grid positions of frames C*t."""
+    clean, noisy = _video(40, 32, 7, seed=30 + channels, channels=channels)
+    a1, o1 = _params(noisy.shape, 1, px=4, pt=2, wx=7, np_=8)
+    a2, o2 = _params(noisy.shape, 2, px=4, pt=2, wx=7, np_=8)
+    ff = bf = None
+    if flow:
+        ff = _smooth_flow(7, 32, 40, 5)
+        bf = _smooth_flow(7, 32, 40, 6)
+    ob, of_, ogr = O.run(o1, o2, noisy, ff, bf, nparts=2, tie_mode=1)
+    gb, gf, ggr = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+    _same(gb, ob, "basic")
+    _same(gf, of_, "final")
+    assert np.allclose(ggr, ogr, rtol=0, atol=1e-4)
+    # the skipped positions: fewer groups than the same video in gray
+    g1 = noisy[:, :1].copy()
+    _, _, gr_gray = ctx.run(A.cli_params(1, 20, g1.shape, px=4, pt=2, wx=7, np_=8),
+                            A.cli_params(2, 20, g1.shape, px=4, pt=2, wx=7, np_=8), g1, ff, bf, nparts=2)
+    assert ggr[0] < gr_gray[0] and ggr[1] < gr_gray[1]
+
+
 def test_order_invariance_and_np1(ctx):
     clean, noisy = _video(40, 32, 6, seed=77)
     for kw in (dict(order_inv=True), dict(np_=1)):
