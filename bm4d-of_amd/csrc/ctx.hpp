@@ -73,7 +73,7 @@ struct bm4dof_ctx {
 	int agg_region[3] = {0, 0, 0};
 	uint64_t step_groups[2] = {0, 0}, step_rows[2] = {0, 0};   // groups / group rows per step
 	bool fast_filter = true;   // register-resident filter where the group shape allows it
-	bool fast_bm = true;       // staged one-wave block matching where the patch shape allows it
+	int fast_bm = 2;           // block matching: 2 register windows, 1 LDS-staged windows, 0 general
 	int bm_frames = 2;         // frames of search window staged per round of the staged matching
 	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
 	int agg_cap = 1024;        // rows per e-range of the chunked aggregation (one channel)

@@ -23,6 +23,9 @@ __global__ void bm_window_kernel(const float *, const float *, const uint32_t *,
                                  const TileBox *, StepConst, int, int4 *, int2 *);
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+template <int PX, int PT> __global__ void bm_reg_kernel(const float *, const int4 *, const int2 *,
+                                                        const uint32_t *, const uint32_t *, const TileBox *,
+                                                        StepConst, int, uint32_t *, uint32_t *, int);
 template <int PX, int PT, int NF> __global__ void bm_fast_kernel(BMF_ARGS);
 __global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
                                const uint32_t *, StepConst, int, int, const float *, const float *,
@@ -499,6 +502,21 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	const size_t stage_floats = std::max<size_t>((size_t)NF * (sc.sWx + PX - 1) * (sc.sWx + PX - 1) * PT,
 	                                             (size_t)maxCand);
 	const size_t lds_fast = ((size_t)maxCand + stage_floats) * 4;
+	const size_t lds_reg = (size_t)maxCand * 8;
+	if (ctx->fast_bm == 2 && shape_ok && (step == 1 || sc.C == 1) && lds_reg <= 56 * 1024) {
+		int4 *d_wins = ctx->bm_wins.as<int4>((size_t)G * sWt);
+		int2 *d_wmeta = ctx->bm_wmeta.as<int2>((size_t)G);
+		hipLaunchKernelGGL(bm_window_kernel, dim3(nblk(G, 256)), dim3(256), 0, ctx->stream, ff, bf, d_gpos,
+		                   d_gtile, d_tiles, sc, G, d_wins, d_wmeta);
+		HIPCHK(hipGetLastError());
+		// row chunks per candidate column: enough work items for a wave
+		const int rchunks = std::max(1, std::min(sc.sWx, 64 / std::max(1, sWt * sc.sWx)));
+		auto kern = PX == 4 ? bm_reg_kernel<4, 4> : bm_reg_kernel<8, 1>;
+		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_reg, ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile,
+		                   d_tiles, sc, rchunks, d_idx, d_nsim, G);
+		HIPCHK(hipGetLastError());
+		return;
+	}
 	if (ctx->fast_bm && shape_ok && (step == 1 || sc.C == 1) && lds_fast <= 56 * 1024) {
 		int4 *d_wins = ctx->bm_wins.as<int4>((size_t)G * sWt);
 		int2 *d_wmeta = ctx->bm_wmeta.as<int2>((size_t)G);
@@ -1267,7 +1285,10 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		const std::string k = key ? key : "";
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
 		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
-		else if (k == "fast_bm") ctx->fast_bm = value != 0;
+		else if (k == "fast_bm") {
+			if (value < 0 || value > 2) throw ArgError("fast_bm must be 0, 1 or 2");
+			ctx->fast_bm = (int)value;
+		}
 		else if (k == "bm_frames") {
 			if (value != 1 && value != 2 && value != 4) throw ArgError("bm_frames must be 1, 2 or 4");
 			ctx->bm_frames = (int)value;

@@ -407,6 +407,121 @@ __global__ void bm_window_kernel(const float *__restrict__ fflow, const float *_
 	wmeta[g] = make_int2(nf, base);
 }
 
+// Selection of the kk smallest of the keys skey[0..nsrch) in the total order
+// (key, reference patch first, scan index) into out_key / out_rank[0..kk),
+// in that order (rank -1 = the reference patch, s_self).  One wave: a radix
+// select of the kk-th key over four 8-bit digits, then the keys below it and
+// the lowest-ranked ties at it, sorted.  stie: scratch of nsrch ints.
+__device__ void bm_radix_select(const uint32_t *__restrict__ skey, int *__restrict__ stie, int nsrch, int kk,
+                                int s_self, uint32_t *__restrict__ out_key, int *__restrict__ out_rank)
+{
+	__shared__ uint32_t hist[256];
+	__shared__ uint32_t s_bucket, s_below;
+	__shared__ uint32_t sel_key[64];
+	__shared__ int sel_rank[64];
+	__shared__ int s_na, s_nb;
+	const int lane = threadIdx.x;
+	if (lane == 0) { s_na = 0; s_nb = 0; }
+	__syncthreads();
+	uint32_t prefix = 0, pmask = 0;
+	uint32_t remaining = (uint32_t)kk;
+	for (int pass = 0; pass < 4; pass++) {
+		const int shift = 24 - 8 * pass;
+#pragma unroll
+		for (int b = 0; b < 4; b++) hist[4 * lane + b] = 0;
+		__syncthreads();
+		for (int s2 = lane; s2 < nsrch; s2 += 64) {
+			const uint32_t k = skey[s2];
+			if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+		}
+		__syncthreads();
+		{
+			const uint32_t c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2], c3 = hist[4 * lane + 3];
+			const int local = (int)(c0 + c1 + c2 + c3);
+			const int incl = wave_incl_scan(local);
+			const int excl = incl - local;
+			if ((uint32_t)excl < remaining && remaining <= (uint32_t)incl) {
+				uint32_t cum = (uint32_t)excl;
+				const uint32_t cs[4] = {c0, c1, c2, c3};
+				for (int b = 0; b < 4; b++) {
+					if (cum + cs[b] >= remaining) { s_bucket = 4 * lane + b; s_below = cum; break; }
+					cum += cs[b];
+				}
+			}
+		}
+		__syncthreads();
+		prefix |= s_bucket << shift;
+		pmask |= 255u << shift;
+		remaining -= s_below;
+		__syncthreads();
+	}
+	const uint32_t T = prefix;
+	const int na_expected = kk - (int)remaining;
+	for (int s2 = lane; s2 < nsrch; s2 += 64) {
+		const uint32_t k = skey[s2];
+		const int rank = s2 == s_self ? -1 : s2;
+		if (k < T) {
+			const int slot = atomicAdd(&s_na, 1);
+			sel_key[slot] = k;
+			sel_rank[slot] = rank;
+		} else if (k == T) {
+			const int slot = atomicAdd(&s_nb, 1);
+			stie[slot] = rank;
+		}
+	}
+	__syncthreads();
+	const int nb = s_nb;
+	for (int j = lane; j < nb; j += 64) {
+		const int r = stie[j];
+		int c = 0;
+		for (int i = 0; i < nb; i++) c += stie[i] < r;
+		if (c < (int)remaining) {
+			sel_key[na_expected + c] = T;
+			sel_rank[na_expected + c] = r;
+		}
+	}
+	__syncthreads();
+	if (lane < kk) {
+		const uint32_t k = sel_key[lane];
+		const int r = sel_rank[lane];
+		int p = 0;
+		for (int j = 0; j < kk; j++) {
+			const uint32_t kj = sel_key[j];
+			p += (kj < k) || (kj == k && sel_rank[j] < r);
+		}
+		out_key[p] = k;
+		out_rank[p] = r;
+	}
+	__syncthreads();
+}
+
+// Power-of-two tau cut (nldct.cpp:1285-1289) on the sorted selection and the
+// neighbour positions t*W*H + y*W + x of the kept candidates.
+__device__ void bm_emit(const uint32_t *__restrict__ out_key, const int *__restrict__ out_rank, int nSimP,
+                        const StepConst &sc, const int4 *__restrict__ gw, int nframes, uint32_t pos,
+                        const TileBox &tb, uint32_t *__restrict__ oidx, uint32_t *__restrict__ onsim)
+{
+	const int lane = threadIdx.x;
+	int n = 1;
+	while (2 * n <= nSimP && __uint_as_float(out_key[n]) < sc.tau2) n *= 2;
+	if (lane == 0) *onsim = n;
+	if (lane < n) {
+		const int r = out_rank[lane];
+		uint32_t q;
+		if (r < 0) q = pos;
+		else {
+			int ii = 0;
+			while (ii + 1 < nframes && gw[ii + 1].w <= r) ii++;
+			const int4 f = gw[ii];
+			const int o = r - f.w;
+			const int nx = f.z & 0xffff;
+			const int qy = (f.y >> 16) + o / nx, qx = (f.y & 0xffff) + o % nx;
+			q = (uint32_t)f.x * (uint32_t)sc.WH + (uint32_t)(qy + tb.oy) * (uint32_t)sc.W + (uint32_t)(qx + tb.ox);
+		}
+		oidx[lane] = q;
+	}
+}
+
 template <int PX, int PT, int NF>
 __global__ __launch_bounds__(64)
 void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
@@ -422,13 +537,8 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 	uint32_t *skey = (uint32_t *)smem;                    // maxCand
 	float *stage = (float *)(skey + maxCand);             // stage_floats (two frames' regions)
 	int *stie = (int *)stage;                             // ties, after the distances
-	__shared__ uint32_t hist[256];
-	__shared__ uint32_t s_bucket, s_below;
-	__shared__ uint32_t sel_key[64];
-	__shared__ int sel_rank[64];
 	__shared__ uint32_t out_key[64];
 	__shared__ int out_rank[64];
-	__shared__ int s_na, s_nb;
 
 	const int g = blockIdx.x;
 	if (g >= G) return;
@@ -516,101 +626,209 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 		const int4 f0 = gw[0];
 		s_self = f0.w + (py - (f0.y >> 16)) * (f0.z & 0xffff) + (px - (f0.y & 0xffff));
 	}
-	if (lane == 0) { s_na = 0; s_nb = 0; }
-	__syncthreads();
-
 	const int nSimP = min(sc.npmax, nsrch);
 	int kk = 1;
 	while (kk * 2 <= nSimP) kk *= 2;
-	uint32_t prefix = 0, pmask = 0;
-	uint32_t remaining = (uint32_t)kk;
-	for (int pass = 0; pass < 4; pass++) {
-		const int shift = 24 - 8 * pass;
+	__syncthreads();
+	bm_radix_select(skey, stie, nsrch, kk, s_self, out_key, out_rank);
+	bm_emit(out_key, out_rank, nSimP, sc, gw, nframes, pos, tb, oidx, out_nsim + g);
+}
+
+// ---------------------------------------------------------------------------
+// Block matching, register-window form (one channel, patches PX x PX x PT,
+// PX = 4 or 8).  One wave per group; a work item is one candidate column
+// (frame j, column cx) and a range of candidate rows.  The item walks down
+// its rows with the PX rows x PT frames of PX samples that the current
+// candidate covers held in registers: each step loads one new row per frame
+// (vector loads from global memory, cached) and computes one distance, the
+// sequential (ht, hy, hx) sum of the reference (nldct.cpp:1262-1274,
+// 1723-1737), with the reference patch in scalar registers.  Distances go to
+// LDS by scan index.  Selection: the kk-th smallest of the 64 per-lane
+// minima bounds the kk-th smallest distance from above (kk lanes each own a
+// distance at or below it), so only the candidates at or below that bound
+// are ranked, in the total order of bm_radix_select; more than BMR_CAP of
+// them (or fewer than kk lanes with candidates) falls back to the radix
+// select over all distances.  Same output as bm_fast_kernel / bm_kernel.
+// ---------------------------------------------------------------------------
+
+#define BMR_CAP 128
+
+template <int PX, int PT>
+__device__ __forceinline__ void bmr_load_row(float (&dst)[PT][PX], const float *__restrict__ p, int frame_stride)
+{
 #pragma unroll
-		for (int b = 0; b < 4; b++) hist[4 * lane + b] = 0;
-		__syncthreads();
-		for (int s2 = lane; s2 < nsrch; s2 += 64) {
-			const uint32_t k = skey[s2];
-			if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+	for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+		for (int q = 0; q < PX / 4; q++) {
+			float4 v;
+			__builtin_memcpy(&v, p + (size_t)ht * frame_stride + 4 * q, sizeof v);
+			dst[ht][4 * q + 0] = v.x;
+			dst[ht][4 * q + 1] = v.y;
+			dst[ht][4 * q + 2] = v.z;
+			dst[ht][4 * q + 3] = v.w;
 		}
-		__syncthreads();
-		{
-			const uint32_t c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2], c3 = hist[4 * lane + 3];
-			const int local = (int)(c0 + c1 + c2 + c3);
-			const int incl = wave_incl_scan(local);
-			const int excl = incl - local;
-			if ((uint32_t)excl < remaining && remaining <= (uint32_t)incl) {
-				uint32_t cum = (uint32_t)excl;
-				const uint32_t cs[4] = {c0, c1, c2, c3};
-				for (int b = 0; b < 4; b++) {
-					if (cum + cs[b] >= remaining) { s_bucket = 4 * lane + b; s_below = cum; break; }
-					cum += cs[b];
+}
+
+template <int PX, int PT>
+__global__ __launch_bounds__(64)
+void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
+                   const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
+                   const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
+                   StepConst sc, int rchunks, uint32_t *__restrict__ out_idx,
+                   uint32_t *__restrict__ out_nsim, int G)
+{
+	constexpr int PP = PX * PX * PT;
+	extern __shared__ unsigned char smem[];
+	const int sWt = sc.wtf + sc.wtb + 1;
+	const int maxCand = sc.sWx * sc.sWx * sWt;
+	uint32_t *skey = (uint32_t *)smem;          // maxCand distances
+	int *stie = (int *)(skey + maxCand);         // maxCand (radix fallback)
+	__shared__ uint32_t smin[64];
+	__shared__ uint32_t cand_key[BMR_CAP];
+	__shared__ int cand_rank[BMR_CAP];
+	__shared__ uint32_t out_key[64];
+	__shared__ int out_rank[64];
+	__shared__ uint32_t s_T0;
+
+	const int g = blockIdx.x;
+	if (g >= G) return;
+	const int lane = threadIdx.x;
+	const uint32_t pos = gpos[g];
+	uint32_t *oidx = out_idx + (size_t)g * sc.npmax;
+	if (sc.npmax <= 1) {
+		if (lane == 0) { oidx[0] = pos; out_nsim[g] = 1; }
+		return;
+	}
+	int gx, gy, gt;
+	pos_decode(pos, sc.W, sc.WH, gx, gy, gt);
+	const TileBox tb = tiles[gtile[g]];
+	const int px = gx - tb.ox, py = gy - tb.oy;
+	const int2 meta = wmeta[g];
+	const int nframes = meta.x, nsrch = meta.y;
+	const int4 *gw = wins + (size_t)g * sWt;
+
+	// reference patch (channel 0), wave-uniform
+	float ref[PP];
+#pragma unroll
+	for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+		for (int hy = 0; hy < PX; hy++)
+#pragma unroll
+			for (int hx = 0; hx < PX; hx++)
+				ref[(ht * PX + hy) * PX + hx] = img[(size_t)(gt + ht) * sc.WHC + (size_t)(gy + hy) * sc.W + gx + hx];
+
+	// items: (frame j, column cx, row chunk r), frame-major
+	int total = 0;
+	for (int j = 0; j < nframes; j++) total += (gw[j].z & 0xffff) * rchunks;
+	uint32_t lmin = 0xffffffffu;
+	for (int it = lane; it < total; it += 64) {
+		int j = 0, loc = it;
+		int4 f = gw[0];
+		for (;;) {
+			const int nit = (f.z & 0xffff) * rchunks;
+			if (loc < nit) break;
+			loc -= nit;
+			f = gw[++j];
+		}
+		const int nx = f.z & 0xffff, ny = f.z >> 16;
+		const int r = loc / nx, cx = loc - r * nx;
+		const int y0 = r * ny / rchunks, y1 = (r + 1) * ny / rchunks;
+		if (y0 >= y1) continue;
+		const float *p = img + (size_t)f.x * sc.WHC + (size_t)(tb.oy + (f.y >> 16)) * sc.W + (tb.ox + (f.y & 0xffff) + cx);
+		uint32_t *kout = skey + f.w + cx;
+		float w[PX][PT][PX];
+#pragma unroll
+		for (int k = 0; k < PX - 1; k++) bmr_load_row<PX, PT>(w[k], p + (size_t)(y0 + k) * sc.W, sc.WHC);
+		for (int y = y0; y < y1; y += PX) {
+#pragma unroll
+			for (int u = 0; u < PX; u++) {
+				if (y + u < y1) {
+					bmr_load_row<PX, PT>(w[(u + PX - 1) % PX], p + (size_t)(y + u + PX - 1) * sc.W, sc.WHC);
+					float d = 0.f;
+#pragma unroll
+					for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+						for (int hy = 0; hy < PX; hy++)
+#pragma unroll
+							for (int hx = 0; hx < PX; hx++) {
+								const float dif = ref[(ht * PX + hy) * PX + hx] - w[(u + hy) % PX][ht][hx];
+								d = d + dif * dif;
+							}
+					const uint32_t kb = __float_as_uint(d);
+					kout[(y + u) * nx] = kb;
+					lmin = min(lmin, kb);
 				}
 			}
 		}
+	}
+	// index of the reference patch itself (frame pt is the first of the list)
+	int s_self;
+	{
+		const int4 f0 = gw[0];
+		s_self = f0.w + (py - (f0.y >> 16)) * (f0.z & 0xffff) + (px - (f0.y & 0xffff));
+	}
+	const int nSimP = min(sc.npmax, nsrch);
+	int kk = 1;
+	while (kk * 2 <= nSimP) kk *= 2;
+
+	// bound: the kk-th smallest lane minimum (lanes without candidates hold ~0)
+	smin[lane] = lmin;
+	if (lane == 0) s_T0 = 0xffffffffu;
+	__syncthreads();
+	{
+		int rk = 0;
+		for (int j = 0; j < 64; j++) {
+			const uint32_t v = smin[j];
+			rk += (v < lmin) || (v == lmin && j < lane);
+		}
+		if (rk == kk - 1 && lmin != 0xffffffffu) s_T0 = lmin;
+	}
+	__syncthreads();
+	const uint32_t T0 = s_T0;
+	int cnt = 0;
+	bool fits = T0 != 0xffffffffu;
+	if (fits) {
+		// candidates at or below the bound, in scan order
+		for (int b = 0; b < nsrch; b += 64) {
+			const int s2 = b + lane;
+			const uint32_t k = s2 < nsrch ? skey[s2] : 0xffffffffu;
+			const bool take = s2 < nsrch && k <= T0;
+			const uint64_t m = __ballot(take);
+			const int before = __popcll(m & ((1ull << lane) - 1ull));
+			if (take && cnt + before < BMR_CAP) {
+				cand_key[cnt + before] = k;
+				cand_rank[cnt + before] = s2 == s_self ? -1 : s2;
+			}
+			cnt += __popcll(m);
+		}
+		fits = cnt <= BMR_CAP;
+	}
+	__syncthreads();
+	if (fits) {
+		for (int e = lane; e < cnt; e += 64) {
+			const uint32_t k = cand_key[e];
+			const int r = cand_rank[e];
+			int p = 0;
+			for (int j = 0; j < cnt; j++) {
+				const uint32_t kj = cand_key[j];
+				p += (kj < k) || (kj == k && cand_rank[j] < r);
+			}
+			if (p < kk) {
+				out_key[p] = k;
+				out_rank[p] = r;
+			}
+		}
 		__syncthreads();
-		prefix |= s_bucket << shift;
-		pmask |= 255u << shift;
-		remaining -= s_below;
-		__syncthreads();
+	} else {
+		bm_radix_select(skey, stie, nsrch, kk, s_self, out_key, out_rank);
 	}
-	const uint32_t T = prefix;
-	const int na_expected = kk - (int)remaining;
-	for (int s2 = lane; s2 < nsrch; s2 += 64) {
-		const uint32_t k = skey[s2];
-		const int rank = s2 == s_self ? -1 : s2;
-		if (k < T) {
-			const int slot = atomicAdd(&s_na, 1);
-			sel_key[slot] = k;
-			sel_rank[slot] = rank;
-		} else if (k == T) {
-			const int slot = atomicAdd(&s_nb, 1);
-			stie[slot] = rank;
-		}
-	}
-	__syncthreads();
-	const int nb = s_nb;
-	for (int j = lane; j < nb; j += 64) {
-		const int r = stie[j];
-		int c = 0;
-		for (int i = 0; i < nb; i++) c += stie[i] < r;
-		if (c < (int)remaining) {
-			sel_key[na_expected + c] = T;
-			sel_rank[na_expected + c] = r;
-		}
-	}
-	__syncthreads();
-	if (lane < kk) {
-		const uint32_t k = sel_key[lane];
-		const int r = sel_rank[lane];
-		int p = 0;
-		for (int j = 0; j < kk; j++) {
-			const uint32_t kj = sel_key[j];
-			p += (kj < k) || (kj == k && sel_rank[j] < r);
-		}
-		out_key[p] = k;
-		out_rank[p] = r;
-	}
-	__syncthreads();
-	int n = 1;
-	while (2 * n <= nSimP && __uint_as_float(out_key[n]) < sc.tau2) n *= 2;
-	if (lane == 0) out_nsim[g] = n;
-	if (lane < n) {
-		const int r = out_rank[lane];
-		uint32_t q;
-		if (r < 0) q = pos;
-		else {
-			int ii = 0;
-			while (ii + 1 < nframes && gw[ii + 1].w <= r) ii++;
-			const int4 f = gw[ii];
-			const int o = r - f.w;
-			const int nx = f.z & 0xffff;
-			const int qy = (f.y >> 16) + o / nx, qx = (f.y & 0xffff) + o % nx;
-			q = (uint32_t)f.x * (uint32_t)sc.WH + (uint32_t)(qy + tb.oy) * (uint32_t)sc.W + (uint32_t)(qx + tb.ox);
-		}
-		oidx[lane] = q;
-	}
+	bm_emit(out_key, out_rank, nSimP, sc, gw, nframes, pos, tb, oidx, out_nsim + g);
 }
+
+#define BMR_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
+	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+template __global__ void bm_reg_kernel<4, 4>(BMR_ARGS);
+template __global__ void bm_reg_kernel<8, 1>(BMR_ARGS);
 
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int

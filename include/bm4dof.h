@@ -107,11 +107,13 @@ const char *bm4dof_global_error(void);
  *                  group shape allows it (one channel, 64-sample patches,
  *                  at most 32 neighbours, no orderInvariance); 0: the
  *                  general LDS filter for every group.
- *   "fast_bm"      1 (default): block matching by one wave per group on
- *                  LDS-staged search windows (patches 4x4x4 or 8x8x1, one
- *                  channel in the distance); 0: the general kernel.
+ *   "fast_bm"      block matching for patches 4x4x4 or 8x8x1 with one
+ *                  channel in the distance: 2 (default): one wave per group,
+ *                  candidate windows in registers; 1: one wave per group on
+ *                  LDS-staged search windows; 0: the general kernel for
+ *                  every shape.
  *   "bm_frames"    frames of search window staged in LDS per round of the
- *                  staged block matching: 1, 2 (default) or 4.
+ *                  LDS-staged block matching: 1, 2 (default) or 4.
  *   "agg_chunked"  1 (default): ordered aggregation by sorted row chunks
  *                  with per-pixel masks (tiles at least 8 pixels wide and
  *                  high); 0: the banded aggregation for every block.

@@ -137,25 +137,35 @@ def test_chunked_and_banded_aggregation_agree(ctx, w, h, f, nparts, px, pt):
     assert np.array_equal(b0, b1) and np.array_equal(f0, f1)
 
 
-@pytest.mark.parametrize("px,pt,flow,nparts", [(4, 4, True, 4), (8, 1, False, 2), (4, 4, False, 6)])
-def test_staged_and_general_block_matching_agree(ctx, px, pt, flow, nparts):
-    """Neighbour lists of the staged one-wave block matching equal those of
-    the general kernel, in both steps."""
+@pytest.mark.parametrize("px,pt,flow,nparts,wx,flat", [(4, 4, True, 4, 11, False), (8, 1, False, 2, 11, False),
+                                                       (4, 4, False, 6, 11, False), (4, 4, True, 2, 11, True),
+                                                       (8, 1, False, 1, 21, True), (8, 1, False, 2, 39, False),
+                                                       (4, 4, False, 2, 5, False)])
+def test_block_matching_kernels_agree(ctx, px, pt, flow, nparts, wx, flat):
+    """Neighbour lists of the register-window and the LDS-staged one-wave
+    block matching equal those of the general kernel, in both steps.  `flat`
+    puts a constant region in the image: many candidates at distance 0, which
+    sends the register-window selection to its radix fallback."""
     f = 8 if pt > 1 else 1
-    clean, noisy = _video(72, 56, f, seed=px + 7 * nparts)
+    clean, noisy = _video(72, 56, f, seed=px + 7 * nparts + wx)
+    if flat:
+        noisy[:, :, :, :30] = 77.0
+        clean[:, :, :, :30] = 77.0
     ff = bf = None
     if flow:
         ff, bf = _smooth_flow(f, 56, 72, 7), _smooth_flow(f, 56, 72, 8)
     for step, img in ((1, noisy), (2, clean)):
-        a = A.cli_params(step, 20, noisy.shape, px=px, pt=pt, wx=11, np_=32)
+        a = A.cli_params(step, 20, noisy.shape, px=px, pt=pt, wx=wx, np_=32)
+        res = {}
         try:
-            ctx.set_option("fast_bm", 0)
-            r0 = ctx.bm(a, img, ff, bf, nparts=nparts)
+            for mode in (0, 1, 2):
+                ctx.set_option("fast_bm", mode)
+                res[mode] = ctx.bm(a, img, ff, bf, nparts=nparts)
         finally:
-            ctx.set_option("fast_bm", 1)
-        r1 = ctx.bm(a, img, ff, bf, nparts=nparts)
-        for u, v in zip(r0, r1):
-            assert np.array_equal(u, v), step
+            ctx.set_option("fast_bm", 2)
+        for mode in (1, 2):
+            for u, v in zip(res[0], res[mode]):
+                assert np.array_equal(u, v), (step, mode)
 
 
 def test_image_config0_bior_and_kaiser(ctx):
