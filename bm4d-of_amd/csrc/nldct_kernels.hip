@@ -1980,6 +1980,9 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	__shared__ uint32_t m_pos2[2][64];   // (dx0 + 16) | (dy0 + 16) << 8 | dt << 16
 	__shared__ uint32_t m_e2[2][64];
 	__shared__ float m_w2[2][64];
+	// per row of a chunk: stage index and window index of the block pixel
+	// (0, 0) (offsets of the row's patch folded in) and the weight bits
+	__shared__ int4 m_rec2[2][64];
 	uint32_t *m_pos = m_pos2[0], *m_e = m_e2[0];
 	float *m_w = m_w2[0];
 	const int lane = threadIdx.x;
@@ -2060,6 +2063,7 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	const bool stage_lane = jl < rpp;
 	const int cl = rl / SL, sl = rl - cl * SL;
 	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
+	const int pix_l = ly * sPx + lx;   // this pixel's index in a patch slice anchored at the block origin
 	__syncthreads();
 
 	uint32_t left = ksum, lo = emin;
@@ -2150,6 +2154,8 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 						m_pos2[buf][lane] = (uint32_t)(dx0 + 16) | ((uint32_t)(dy0 + 16) << 8) | ((uint32_t)dt << 16);
 						m_e2[buf][lane] = eN;
 						m_w2[buf][lane] = wN;
+						const int off = dy0 * sPx + dx0;
+						m_rec2[buf][lane] = make_int4(lane * RS - off, dt * SL - off, (int)__float_as_uint(wN), 0);
 					}
 					return mrow;
 				};
@@ -2187,18 +2193,31 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 					__syncthreads();
 					if (c + 1 < nch) stage_issue(c + 1);
 					const int buf = c & 1;
+					// each lane folds the rows that cover its pixel, in row
+					// order, two rows per round (their loads overlap)
 					uint64_t mine = transpose_bits64(mcur);
 					while (mine) {
-						const int j = __ffsll((unsigned long long)mine) - 1;
+						const int j1 = __ffsll((unsigned long long)mine) - 1;
 						mine &= mine - 1;
-						const uint32_t mp = m_pos2[buf][j];
-						const int dx = lx + 16 - (int)(mp & 255u);
-						const int dy = ly + 16 - (int)((mp >> 8) & 255u);
-						const int dt = (int)((mp >> 16) & 255u);
-						const int sidx = dy * sPx + dx;
-						aw = aw + m_w2[buf][j] * swin[dt * SL + sidx];
+						const bool two = mine != 0;
+						const int j2 = two ? __ffsll((unsigned long long)mine) - 1 : j1;
+						if (two) mine &= mine - 1;
+						const int4 r1 = m_rec2[buf][j1], r2 = m_rec2[buf][j2];
+						const float v1 = swin[r1.y + pix_l], v2 = swin[r2.y + pix_l];
+						float x1[CC], x2[CC];
 #pragma unroll
-						for (int cc = 0; cc < CC; cc++) an[cc] = an[cc] + stage[j * RS + cc * SL + sidx];
+						for (int cc = 0; cc < CC; cc++) {
+							x1[cc] = stage[r1.x + cc * SL + pix_l];
+							x2[cc] = stage[r2.x + cc * SL + pix_l];
+						}
+						aw = aw + __int_as_float(r1.z) * v1;
+#pragma unroll
+						for (int cc = 0; cc < CC; cc++) an[cc] = an[cc] + x1[cc];
+						if (two) {
+							aw = aw + __int_as_float(r2.z) * v2;
+#pragma unroll
+							for (int cc = 0; cc < CC; cc++) an[cc] = an[cc] + x2[cc];
+						}
 					}
 					__syncthreads();
 					if (c + 1 < nch) stage_write(c + 1);
