@@ -1109,16 +1109,20 @@ int64_t learned_pairs_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const float 
 }
 
 // --------------------------------------------------------------- autotuner
-// Execution options that do not change the output (block-matching staging
-// depth, aggregation e-range capacity) are timed on a probe sequence and the
-// fastest set is kept per shape class (w, h, f, patch, search window).
-struct TuneChoice { int bm_frames, agg_cap; };
+// Execution options that do not change the output (block-matching kernel and
+// its staging depth, aggregation e-range capacity) are timed on a probe
+// sequence and the fastest set is kept per shape class (w, h, patch, search
+// window).
+struct TuneChoice { int fast_bm, bm_frames, agg_cap; };
 
 void apply_choice(bm4dof_ctx *ctx, const TuneChoice &c)
 {
+	ctx->fast_bm = c.fast_bm;
 	ctx->bm_frames = c.bm_frames;
 	ctx->agg_cap = c.agg_cap;
 }
+
+TuneChoice current_choice(const bm4dof_ctx *ctx) { return {ctx->fast_bm, ctx->bm_frames, ctx->agg_cap}; }
 
 // Times both steps of one gray sequence (device buffers) under every
 // candidate; returns the fastest choice and fills `report`.
@@ -1126,25 +1130,26 @@ TuneChoice autotune(bm4dof_ctx *ctx, const bm4dof_params &p1, const bm4dof_param
                     int W, int H, int F, const float *ff, const float *bf, float *basic, float *final,
                     int nparts, std::string &report)
 {
-	static const int frames[] = {1, 2, 4};
-	static const int caps[] = {768, 1024, 1536};
-	TuneChoice best{ctx->bm_frames, ctx->agg_cap};
+	static const TuneChoice cands[] = {{2, 2, 768}, {2, 2, 1024}, {2, 2, 1536},
+	                                   {1, 1, 1024}, {1, 2, 1024}, {1, 4, 1024}};
+	const TuneChoice saved = current_choice(ctx);
+	TuneChoice best = saved;
 	float best_ms = 3.4e38f;
-	const TuneChoice saved{ctx->bm_frames, ctx->agg_cap};
 	char line[160];
-	for (int fr : frames)
-		for (int cap : caps) {
-			apply_choice(ctx, {fr, cap});
-			run_dev(ctx, &p1, &p2, noisy, W, H, F, 1, ff, bf, basic, final, nparts, nullptr);   // warm
-			run_dev(ctx, &p1, &p2, noisy, W, H, F, 1, ff, bf, basic, final, nparts, nullptr);
-			float ms = 0.f;
-			for (auto &t : ctx->timings) ms += t.second;
-			snprintf(line, sizeof line, "bm_frames=%d agg_cap=%d: %.3f ms\n", fr, cap, ms);
-			report += line;
-			if (ms < best_ms) { best_ms = ms; best = {fr, cap}; }
-		}
+	for (const TuneChoice &c : cands) {
+		apply_choice(ctx, c);
+		run_dev(ctx, &p1, &p2, noisy, W, H, F, 1, ff, bf, basic, final, nparts, nullptr);   // warm
+		run_dev(ctx, &p1, &p2, noisy, W, H, F, 1, ff, bf, basic, final, nparts, nullptr);
+		float ms = 0.f;
+		for (auto &t : ctx->timings) ms += t.second;
+		snprintf(line, sizeof line, "fast_bm=%d bm_frames=%d agg_cap=%d: %.3f ms\n", c.fast_bm, c.bm_frames,
+		         c.agg_cap, ms);
+		report += line;
+		if (ms < best_ms) { best_ms = ms; best = c; }
+	}
 	apply_choice(ctx, saved);
-	snprintf(line, sizeof line, "chosen bm_frames=%d agg_cap=%d (%.3f ms)\n", best.bm_frames, best.agg_cap, best_ms);
+	snprintf(line, sizeof line, "chosen fast_bm=%d bm_frames=%d agg_cap=%d (%.3f ms)\n", best.fast_bm,
+	         best.bm_frames, best.agg_cap, best_ms);
 	report += line;
 	return best;
 }
@@ -1347,7 +1352,7 @@ int bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_pa
 		b->plans.reserve(2 * n);
 		b->geo.resize(2 * n);
 		b->dev.resize(2 * n);
-		const TuneChoice saved{ctx->bm_frames, ctx->agg_cap};
+		const TuneChoice saved = current_choice(ctx);
 		b->choice.assign(n, saved);
 		std::vector<std::pair<std::string, TuneChoice>> tuned;
 		for (int i = 0; i < n; i++) {
@@ -1420,7 +1425,7 @@ int bm4dof_batch_run(bm4dof_batch *b)
 	bm4dof_ctx *ctx = b->ctx;
 	return guarded(ctx, [&] {
 		HIPCHK(hipSetDevice(ctx->device));
-		const TuneChoice saved{ctx->bm_frames, ctx->agg_cap};
+		const TuneChoice saved = current_choice(ctx);
 		hipEvent_t e0, e1;
 		HIPCHK(hipEventCreate(&e0));
 		HIPCHK(hipEventCreate(&e1));

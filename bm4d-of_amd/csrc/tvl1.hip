@@ -302,14 +302,25 @@ __global__ void tv_u_kernel(const float *__restrict__ I1wx, const float *__restr
 
 // Stopping test of iteration k (tvl1flow_lib.c:165, 225): state becomes k
 // when error <= epsilon^2 or k reaches MAX_ITERATIONS.
+// The block partials are summed by TV_THREADS threads in a fixed order
+// (strided partial sums, then a tree), so the result does not depend on
+// timing.
 __global__ void tv_decide_kernel(const double *__restrict__ part, int nblocks, int size, float eps2,
                                  int k, int *__restrict__ state, int *__restrict__ n_active)
 {
 	const int pr = blockIdx.x;
-	if (threadIdx.x != 0 || state[pr] >= 0) return;
-	double sum = 0.0;
-	for (int b = 0; b < nblocks; b++) sum += part[(size_t)pr * nblocks + b];
-	const float error = (float)sum / (float)size;
+	if (state[pr] >= 0) return;
+	__shared__ double s[TV_THREADS];
+	double e = 0.0;
+	for (int b = threadIdx.x; b < nblocks; b += TV_THREADS) e += part[(size_t)pr * nblocks + b];
+	s[threadIdx.x] = e;
+	__syncthreads();
+	for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
+		if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+		__syncthreads();
+	}
+	if (threadIdx.x != 0) return;
+	const float error = (float)s[0] / (float)size;
 	if (!(error > eps2) || k >= TV_MAX_ITERATIONS) {
 		state[pr] = k;
 		atomicSub(n_active, 1);
@@ -544,7 +555,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 			for (int k = 1; k <= TV_MAX_ITERATIONS; k++) {
 				hipLaunchKernelGGL(tv_u_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1wx, I1wy, grad, rho_c,
 				                   u1, u2, p11, p12, p21, p22, nx, ny, n, l_t, prm.theta, d_state, d_part);
-				hipLaunchKernelGGL(tv_decide_kernel, dim3(P), dim3(64), 0, st, d_part, g, n, eps2, k,
+				hipLaunchKernelGGL(tv_decide_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, g, n, eps2, k,
 				                   d_state, d_nact);
 				hipLaunchKernelGGL(tv_p_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, u1, u2, p11, p12, p21,
 				                   p22, nx, ny, n, taut, k, d_state);

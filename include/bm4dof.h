@@ -200,7 +200,7 @@ int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
  * (BM4DOF_BATCH_GRAPH) that every run replays.  Each sequence's output is
  * bitwise the output of bm4dof_run_dev on it alone (same nparts).  With
  * BM4DOF_BATCH_AUTOTUNE the execution options that cannot change results
- * ("bm_frames", "agg_cap") are timed on the first sequence of every shape
+ * ("fast_bm", "bm_frames", "agg_cap") are timed on the first sequence of every shape
  * class and the fastest set is used for that class.  Buffers are device
  * pointers and must stay valid for the batch's lifetime; a captured batch
  * holds its scratch memory in the context, so runs on that context of other
