@@ -5,6 +5,7 @@
 #include "common.hpp"
 #include "../../include/bm4dof.h"
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -81,6 +82,9 @@ struct bm4dof_ctx {
 	bm4dof::DevBuf mlp_theta, mlp_m, mlp_v, mlp_wq, mlp_inf, mlp_ws, mlp_grad, mlp_xb, mlp_tb, mlp_sse;
 	int mlp_step = 0;
 	bool mlp_ready = false;
+	// step plans (grids, bases, tiles) kept on the device for repeated runs of
+	// the same shape; owned by the host code that defines them
+	std::shared_ptr<void> plan_cache;
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

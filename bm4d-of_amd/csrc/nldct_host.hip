@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <memory>
 #include <algorithm>
 #include <stdexcept>
 
@@ -885,18 +886,65 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	}
 }
 
-// One step, eager: plan, upload, reserve and enqueue.
+// Plans of recent step shapes, each with its own device copy, so that a
+// repeated run of the same shape plans and uploads nothing.  The key holds
+// everything plan_step reads.
+struct CachedPlan {
+	std::string key;
+	StepPlan pl;
+	DevBuf geo;
+	StepDev dv;
+	~CachedPlan() { geo.release(); }
+};
+struct PlanCache {
+	std::vector<std::unique_ptr<CachedPlan>> v;   // most recent first
+};
+constexpr size_t kPlanCacheSize = 4;
+
+std::string plan_key(const bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C, bool use_flow,
+                     int nparts, bool learned)
+{
+	const long long f[] = {W, H, F, C, use_flow, nparts, learned, ctx->fast_filter, (long long)ctx->est_budget};
+	std::string k(reinterpret_cast<const char *>(&p), sizeof p);
+	k.append(reinterpret_cast<const char *>(f), sizeof f);
+	return k;
+}
+
+const CachedPlan &cached_plan(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
+                              bool use_flow, int nparts, bool learned)
+{
+	if (!ctx->plan_cache) ctx->plan_cache = std::make_shared<PlanCache>();
+	auto &v = static_cast<PlanCache *>(ctx->plan_cache.get())->v;
+	const std::string key = plan_key(ctx, p, W, H, F, C, use_flow, nparts, learned);
+	for (size_t i = 0; i < v.size(); i++)
+		if (v[i]->key == key) {
+			std::rotate(v.begin(), v.begin() + i, v.begin() + i + 1);
+			return *v[0];
+		}
+	auto e = std::make_unique<CachedPlan>();
+	e->key = key;
+	e->pl = plan_step(ctx, p, W, H, F, C, use_flow, nparts, learned);
+	e->dv = upload_plan(ctx, e->pl, e->geo);
+	if (v.size() >= kPlanCacheSize) {
+		HIPCHK(hipStreamSynchronize(ctx->stream));   // the evicted plan may still be in use
+		v.pop_back();
+	}
+	v.insert(v.begin(), std::move(e));
+	return *v[0];
+}
+
+// One step, eager: (cached) plan, reserve and enqueue.
 uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
                   const float *noisy_w, const float *basic_w, const float *ff, const float *bf,
                   float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr)
 {
 	const bool use_flow = ff != nullptr && bf != nullptr;
-	StepPlan pl = plan_step(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr);
-	HIPCHK(hipStreamSynchronize(ctx->stream));   // the geometry buffer may be in use
-	StepDev dv = upload_plan(ctx, pl, ctx->geo);
+	const CachedPlan &cp = cached_plan(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr);
+	const StepPlan &pl = cp.pl;
 	reserve_step(ctx, pl);
 	ctx->step_groups[pl.step1 ? 0 : 1] += (uint64_t)pl.G;
-	enqueue_step(ctx, pl, dv, noisy_w, basic_w, use_flow ? ff : nullptr, use_flow ? bf : nullptr, out_w, &tm, la);
+	enqueue_step(ctx, pl, cp.dv, noisy_w, basic_w, use_flow ? ff : nullptr, use_flow ? bf : nullptr, out_w, &tm,
+	             la);
 	return (uint64_t)pl.G;
 }
 
@@ -1264,13 +1312,17 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	(void)hipSetDevice(ctx->device);
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
+	ctx->plan_cache.reset();
 	DevBuf *bufs[] = {&ctx->gpos, &ctx->gtile, &ctx->tiles, &ctx->idx, &ctx->nsim, &ctx->base,
 	                  &ctx->keys, &ctx->vals, &ctx->skeys, &ctx->svals, &ctx->sort_tmp, &ctx->lstart,
 	                  &ctx->lend, &ctx->rank_of, &ctx->etile, &ctx->entry_w, &ctx->est, &ctx->early,
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec};
+	                  &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
+	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
+	                  &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
+	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};
 	for (DevBuf *b : bufs) b->release();
 	DevBuf *tvb[] = {&ctx->tv_frames, &ctx->tv_pairs, &ctx->tv_part, &ctx->tv_mm, &ctx->tv_state,
 	                 &ctx->tv_work, &ctx->tv_pyr, &ctx->tv_taps};
