@@ -934,13 +934,26 @@ const CachedPlan &cached_plan(bm4dof_ctx *ctx, const bm4dof_params &p, int W, in
 }
 
 // One step, eager: (cached) plan, reserve and enqueue.
+// `ratio` (may be NULL) receives the reference's groupsRatio term of the step:
+// per tile 100 * groups / (W*H*F), summed over the tiles in order
+// (nldct.cpp:403-404, 471-472).
 uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
                   const float *noisy_w, const float *basic_w, const float *ff, const float *bf,
-                  float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr)
+                  float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr, float *ratio = nullptr)
 {
 	const bool use_flow = ff != nullptr && bf != nullptr;
 	const CachedPlan &cp = cached_plan(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr);
 	const StepPlan &pl = cp.pl;
+	if (ratio) {
+		const float whf = (float)((size_t)W * H * F);
+		float r = 0.f;
+		for (int t = 0; t < nparts; t++) {
+			const uint32_t g0 = pl.geo.tile_first[t];
+			const uint32_t g1 = t + 1 < nparts ? pl.geo.tile_first[t + 1] : (uint32_t)pl.geo.gpos.size();
+			r += 100.f * (float)(g1 - g0) / whf;
+		}
+		*ratio = r;
+	}
 	reserve_step(ctx, pl);
 	ctx->step_groups[pl.step1 ? 0 : 1] += (uint64_t)pl.G;
 	enqueue_step(ctx, pl, cp.dv, noisy_w, basic_w, use_flow ? ff : nullptr, use_flow ? bf : nullptr, out_w, &tm,
@@ -992,7 +1005,6 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 	tm.mark("start");
 	const size_t n = (size_t)W * H * F * C;
 	float gr[2] = {0.f, 0.f};
-	const float whf = (float)((size_t)W * H * F);
 
 	if (step != 2) {
 		const float *nw = noisy;
@@ -1002,12 +1014,8 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 			color(ctx, t, W, H, F, C, true);
 			nw = t;
 		}
-		const uint64_t G = run_step(ctx, *p1, W, H, F, C, nw, nullptr, ff, bf, basic, nparts, tm);
+		run_step(ctx, *p1, W, H, F, C, nw, nullptr, ff, bf, basic, nparts, tm, nullptr, &gr[0]);
 		if (C > 1 && p1->colorSpace == BM4DOF_YUV) color(ctx, basic, W, H, F, C, false);
-		// groupsRatio[0] += 100 * groups / whf per tile (nldct.cpp:403-404); the
-		// per-tile counts are summed here in the same order.
-		gr[0] = 0.f;
-		(void)G;
 	}
 	if (step != 1) {
 		const float *nw = noisy;
@@ -1018,7 +1026,7 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 			color(ctx, basic, W, H, F, C, true);
 			nw = t;
 		}
-		run_step(ctx, *p2, W, H, F, C, nw, basic, ff, bf, final, nparts, tm);
+		run_step(ctx, *p2, W, H, F, C, nw, basic, ff, bf, final, nparts, tm, nullptr, &gr[1]);
 		if (C > 1 && p2->colorSpace == BM4DOF_YUV) {
 			color(ctx, basic, W, H, F, C, false);
 			color(ctx, final, W, H, F, C, false);
@@ -1026,19 +1034,6 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 	}
 	HIPCHK(hipStreamSynchronize(ctx->stream));
 	stats_end(ctx);
-	// group ratios: per tile 100*groups/whf, summed over tiles in order
-	for (int s = 0; s < 2; s++) {
-		if ((s == 0 && step == 2) || (s == 1 && step == 1)) continue;
-		const bm4dof_params &p = s == 0 ? *p1 : *p2;
-		StepGeom geo = make_geometry(p, W, H, F, nparts, C);
-		float r = 0.f;
-		for (int t = 0; t < nparts; t++) {
-			const uint32_t g0 = geo.tile_first[t];
-			const uint32_t g1 = t + 1 < nparts ? geo.tile_first[t + 1] : (uint32_t)geo.gpos.size();
-			r += 100.f * (float)(g1 - g0) / whf;
-		}
-		gr[s] = r;
-	}
 	if (ratio) { ratio[0] = gr[0]; ratio[1] = gr[1]; }
 	// timings
 	for (size_t i = 1; i < ctx->ev.size(); i++) {

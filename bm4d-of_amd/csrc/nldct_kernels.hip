@@ -1198,7 +1198,27 @@ __device__ __forceinline__ void fast_forward(const float *__restrict__ img, cons
                                              float *__restrict__ zs)
 {
 	constexpr int PT = FAST_P / (PX * PX);
-	for (int n = 0; n < N; n++) {
+	// two patches per round: two independent sums in flight
+	int n = 0;
+	for (; n + 1 < N; n += 2) {
+		const float *src0 = img + gidx[n], *src1 = img + gidx[n + 1];
+		float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+		for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+			for (int hy = 0; hy < PX; hy++) {
+				const float *r0 = src0 + ht * WH + hy * W, *r1 = src1 + ht * WH + hy * W;
+#pragma unroll
+				for (int hx = 0; hx < PX; hx++) {
+					const float b = brow[(ht * PX + hy) * PX + hx];
+					acc0 = acc0 + r0[hx] * b;
+					acc1 = acc1 + r1[hx] * b;
+				}
+			}
+		zs[n * FAST_P + threadIdx.x] = acc0;
+		zs[(n + 1) * FAST_P + threadIdx.x] = acc1;
+	}
+	if (n < N) {
 		const float *src = img + gidx[n];
 		float acc = 0.f;
 #pragma unroll
@@ -1221,7 +1241,27 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 	float bcol[FAST_P];
 #pragma unroll
 	for (int k = 0; k < FAST_P; k++) bcol[k] = binv[k * FAST_P + threadIdx.x];
-	for (int n = 0; n < N; n++) {
+	int n = 0;
+	for (; n + 1 < N; n += 2) {
+		const float4 *zr0 = reinterpret_cast<const float4 *>(zs + n * FAST_P);
+		const float4 *zr1 = reinterpret_cast<const float4 *>(zs + (n + 1) * FAST_P);
+		float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+		for (int q = 0; q < FAST_P / 4; q++) {
+			const float4 z0 = zr0[q], z1 = zr1[q];
+			acc0 = acc0 + z0.x * bcol[4 * q + 0];
+			acc1 = acc1 + z1.x * bcol[4 * q + 0];
+			acc0 = acc0 + z0.y * bcol[4 * q + 1];
+			acc1 = acc1 + z1.y * bcol[4 * q + 1];
+			acc0 = acc0 + z0.z * bcol[4 * q + 2];
+			acc1 = acc1 + z1.z * bcol[4 * q + 2];
+			acc0 = acc0 + z0.w * bcol[4 * q + 3];
+			acc1 = acc1 + z1.w * bcol[4 * q + 3];
+		}
+		est[(size_t)(row0 + n) * FAST_P + threadIdx.x] = ww * acc0;
+		est[(size_t)(row0 + n + 1) * FAST_P + threadIdx.x] = ww * acc1;
+	}
+	if (n < N) {
 		const float4 *zr = reinterpret_cast<const float4 *>(zs + n * FAST_P);
 		float acc = 0.f;
 #pragma unroll
