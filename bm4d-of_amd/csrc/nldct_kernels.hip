@@ -1192,15 +1192,15 @@ __device__ __forceinline__ int wave_sum_int(int v)
 
 // Forward transform of the N patches at anchors gidx[0..N) of `img`
 // (one channel, anchor address = position) into zs[n*64 + lane].
-template <int PX>
+// PAIRS: two patches per round (two independent sums in flight) or one.
+template <int PX, bool PAIRS = true>
 __device__ __forceinline__ void fast_forward(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
                                              int N, int W, int WH, const float (&brow)[FAST_P],
                                              float *__restrict__ zs)
 {
 	constexpr int PT = FAST_P / (PX * PX);
-	// two patches per round: two independent sums in flight
 	int n = 0;
-	for (; n + 1 < N; n += 2) {
+	for (; PAIRS && n + 1 < N; n += 2) {
 		const float *src0 = img + gidx[n], *src1 = img + gidx[n + 1];
 		float acc0 = 0.f, acc1 = 0.f;
 #pragma unroll
@@ -1218,7 +1218,7 @@ __device__ __forceinline__ void fast_forward(const float *__restrict__ img, cons
 		zs[n * FAST_P + threadIdx.x] = acc0;
 		zs[(n + 1) * FAST_P + threadIdx.x] = acc1;
 	}
-	if (n < N) {
+	for (; n < N; n++) {
 		const float *src = img + gidx[n];
 		float acc = 0.f;
 #pragma unroll
@@ -1235,6 +1235,7 @@ __device__ __forceinline__ void fast_forward(const float *__restrict__ img, cons
 
 // Inverse transform of the coefficients zs[n*64 + k] and the weighted est
 // rows: est[(row0+n)*64 + pix] = (w*win[pix]) * sum_k z[k][n] Bi[k][pix].
+template <bool PAIRS = true>
 __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const float *__restrict__ binv,
                                              int N, float ww, uint32_t row0, float *__restrict__ est)
 {
@@ -1242,7 +1243,7 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 #pragma unroll
 	for (int k = 0; k < FAST_P; k++) bcol[k] = binv[k * FAST_P + threadIdx.x];
 	int n = 0;
-	for (; n + 1 < N; n += 2) {
+	for (; PAIRS && n + 1 < N; n += 2) {
 		const float4 *zr0 = reinterpret_cast<const float4 *>(zs + n * FAST_P);
 		const float4 *zr1 = reinterpret_cast<const float4 *>(zs + (n + 1) * FAST_P);
 		float acc0 = 0.f, acc1 = 0.f;
@@ -1261,7 +1262,7 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 		est[(size_t)(row0 + n) * FAST_P + threadIdx.x] = ww * acc0;
 		est[(size_t)(row0 + n + 1) * FAST_P + threadIdx.x] = ww * acc1;
 	}
-	if (n < N) {
+	for (; n < N; n++) {
 		const float4 *zr = reinterpret_cast<const float4 *>(zs + n * FAST_P);
 		float acc = 0.f;
 #pragma unroll
@@ -1445,7 +1446,7 @@ __device__ __forceinline__ void learned_coeffs(const float *__restrict__ img, co
 		float brow[FAST_P];
 #pragma unroll
 		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
-		fast_forward<PX>(img, gidx, N, sc.W, sc.WH, brow, zs);
+		fast_forward<PX, false>(img, gidx, N, sc.W, sc.WH, brow, zs);
 	}
 	__syncthreads();
 #pragma unroll
@@ -1539,7 +1540,7 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 	}
 	if (lane < N) entry_w[row0 + lane] = w;
 	__syncthreads();
-	fast_inverse(zs, binv, N, w * win[lane], row0, est);
+	fast_inverse<false>(zs, binv, N, w * win[lane], row0, est);
 }
 
 // Training pairs: for every `stride`-th group, the features of its rows
