@@ -291,11 +291,12 @@ class Context:
     def stats(self):
         """Statistics of the last run: dict with the ordered aggregation's
         pixel blocks / banded blocks / block shape and the groups and group
-        rows of each step."""
-        v = (C.c_int64 * 9)()
-        n = self._L.bm4dof_last_stats(self.h, v, 9)
+        rows of each step, and the executed TV-L1 iterations x pixels of the
+        last flow computation."""
+        v = (C.c_int64 * 10)()
+        n = self._L.bm4dof_last_stats(self.h, v, 10)
         keys = ("agg_blocks", "agg_banded_blocks", "agg_bx", "agg_by", "agg_bt",
-                "groups1", "rows1", "groups2", "rows2")
+                "groups1", "rows1", "groups2", "rows2", "tvl1_pixel_iters")
         return {keys[i]: int(v[i]) for i in range(n)}
 
     def bm(self, p, image, fflow=None, bflow=None, nparts=2):

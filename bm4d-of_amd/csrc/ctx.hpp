@@ -85,6 +85,9 @@ struct bm4dof_ctx {
 	// step plans (grids, bases, tiles) kept on the device for repeated runs of
 	// the same shape; owned by the host code that defines them
 	std::shared_ptr<void> plan_cache;
+	// TV-L1: sum over pairs, scales and warps of executed iterations x pixels
+	// of the last flow computation
+	uint64_t tv_pixel_iters = 0;
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
 };

@@ -1665,12 +1665,13 @@ __attribute__((visibility("default")))
 int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots)
 {
 	if (!ctx || !stats) return 0;
-	const int64_t v[9] = {(int64_t)ctx->agg_blocks, (int64_t)ctx->agg_overflow_blocks,
-	                      ctx->agg_region[0], ctx->agg_region[1], ctx->agg_region[2],
-	                      (int64_t)ctx->step_groups[0], (int64_t)ctx->step_rows[0],
-	                      (int64_t)ctx->step_groups[1], (int64_t)ctx->step_rows[1]};
+	const int64_t v[10] = {(int64_t)ctx->agg_blocks, (int64_t)ctx->agg_overflow_blocks,
+	                       ctx->agg_region[0], ctx->agg_region[1], ctx->agg_region[2],
+	                       (int64_t)ctx->step_groups[0], (int64_t)ctx->step_rows[0],
+	                       (int64_t)ctx->step_groups[1], (int64_t)ctx->step_rows[1],
+	                       (int64_t)ctx->tv_pixel_iters};
 	int n = 0;
-	for (; n < 9 && n < max_slots; n++) stats[n] = v[n];
+	for (; n < 10 && n < max_slots; n++) stats[n] = v[n];
 	return n;
 }
 

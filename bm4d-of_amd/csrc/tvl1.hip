@@ -477,6 +477,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		throw ArgError("invalid TV-L1 parameters");
 	const int nscales = prm.nscales;
 	TVPlan pl = tv_plan(P, nxx, nyy, nscales, prm.zfactor);
+	ctx->tv_pixel_iters = 0;
 	for (int s = 0; s < nscales; s++)
 		if (pl.nx[s] < 3 || pl.ny[s] < 3)
 			throw ArgError("TV-L1 pyramid level " + std::to_string(s) + " is smaller than 3x3; lower nscales");
@@ -569,6 +570,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 			}
 			HIPCHK(hipMemcpyAsync(h_state.data(), d_state, P * sizeof(int), hipMemcpyDeviceToHost, st));
 			HIPCHK(hipStreamSynchronize(st));
+			for (int pr = 0; pr < P; pr++) ctx->tv_pixel_iters += (uint64_t)std::max(h_state[pr], 0) * (uint64_t)n;
 			if (iters)
 				for (int pr = 0; pr < P; pr++) iters[((size_t)pr * nscales + s) * prm.warps + w] = h_state[pr];
 			if (prm.verbose)

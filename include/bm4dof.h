@@ -188,7 +188,9 @@ int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, in
  * did not fit the block capacity and were processed in several row bands,
  * stats[2..4] = block shape (x, y, frames) of the last aggregation launch,
  * stats[5..6] = groups and group rows (patches over all groups) of step 1,
- * stats[7..8] = the same for step 2.  Returns the number of slots filled. */
+ * stats[7..8] = the same for step 2; stats[9] = executed TV-L1 iterations
+ * times pixels, summed over pairs, scales and warps, of the last flow
+ * computation on the context.  Returns the number of slots filled. */
 int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
 
 /* ---- batches and the autotuner ----
