@@ -46,14 +46,15 @@ template <int PX> __global__ void filter1_fast_kernel(FILTER1_ARGS);
 template <int PX> __global__ void filter2_fast_kernel(FILTER2_ARGS);
 #define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	StepConst, int, int, const float *, const float *, const float *, MlpInfer, float, float, float *, \
-	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *
+	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *, int, int
 #define PAIRS_ARGS const float *, const float *, const uint32_t *, const uint32_t *, StepConst, int, int, \
 	const float *, float, float, float *, float *, uint8_t *
 template <int PX> __global__ void filter_learned_kernel(LEARNED_ARGS);
 template <int PX> __global__ void learned_pairs_kernel(PAIRS_ARGS);
 __global__ void fallback_mark_kernel(const uint32_t *, const uint32_t *, const uint8_t *, StepConst, int,
                                      uint8_t *);
-__global__ void fallback_blend_kernel(const uint8_t *, const float *, int64_t, int, float *);
+__global__ void fallback_blend_kernel(const uint8_t *, const float *, int64_t, int, int, float *);
+__global__ void planes_kernel(const float *, int, int, int64_t, float *);
 __global__ void emit_entries_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int,
                                     int, uint32_t *);
 // sortscan.hip
@@ -682,13 +683,14 @@ StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F,
 	pl.filt_lds = pl.step1 ? ((size_t)C * pl.sc.P * pl.np + (size_t)pl.sc.P * pl.np) * 4
 	                       : ((size_t)C * pl.sc.P * pl.np + 2 * (size_t)pl.sc.P * pl.np) * 4;
 	if (pl.filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
-	// register-resident filter: one channel, 64-sample patches, <= 32 neighbours
-	pl.fast_shape = C == 1 && pl.sc.P == 64 && pl.np <= 32 &&
-	                ((pl.sc.sPx == 4 && pl.sc.sPt == 4) || (pl.sc.sPx == 8 && pl.sc.sPt == 1));
+	// register-resident filter: one channel, 64-sample patches, at most 32
+	// neighbours (np < 64: the power-of-two cut keeps at most 32)
+	const bool shape64 = pl.sc.P == 64 && pl.np < 64 &&
+	                     ((pl.sc.sPx == 4 && pl.sc.sPt == 4) || (pl.sc.sPx == 8 && pl.sc.sPt == 1));
+	pl.fast_shape = C == 1 && shape64;
 	pl.fast = ctx->fast_filter && pl.fast_shape && !pl.sc.order_inv;
-	if (learned && !pl.fast_shape)
-		throw ArgError("learned pass: one channel, 4x4x4 or 8x8x1 patches and at most 32 neighbours",
-		               BM4DOF_EUNSUPP);
+	if (learned && !shape64)
+		throw ArgError("learned pass: 4x4x4 or 8x8x1 patches and fewer than 64 similar patches", BM4DOF_EUNSUPP);
 	return pl;
 }
 
@@ -761,6 +763,7 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 	if (pl.learned) {
 		(void)ctx->gflag.as<uint8_t>(G);
 		(void)ctx->fmask.as<uint8_t>(pl.WHF);
+		if (pl.C > 1) (void)ctx->planes.as<float>(pl.N_all);
 	}
 	reserve_scan(ctx, (int64_t)std::max(pl.WHF + 1, G + 1));
 }
@@ -797,6 +800,15 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	       d_nsim);
 	mark("l_bm", "s1_bm", "s2_bm");
 	uint8_t *d_gflag = la ? ctx->gflag.as<uint8_t>(std::max(G, 1)) : nullptr;
+	// the learned filter reads one-channel planes
+	const float *d_planes = noisy_w;
+	if (la && C > 1) {
+		float *pl_buf = ctx->planes.as<float>(N_all);
+		hipLaunchKernelGGL(planes_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream, noisy_w, C,
+		                   sc.WH, (int64_t)N_all, pl_buf);
+		HIPCHK(hipGetLastError());
+		d_planes = pl_buf;
+	}
 
 	for (int g0 = 0; g0 < G; g0 += pl.chunk_groups) {
 		const int Gc = std::min(pl.chunk_groups, G - g0);
@@ -828,10 +840,11 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		mark("l_sort", "s1_sort", "s2_sort");
 
 		if (la)
-			hipLaunchKernelGGL(sc.sPx == 4 ? filter_learned_kernel<4> : filter_learned_kernel<8>, dim3(Gc),
-			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, dv.gtile, sc, g0, Gc,
-			                   dv.basis, dv.binv, dv.win, la->net, la->sigma, la->gvar_thr, d_est, d_ew, d_etile,
-			                   d_early, d_gw, d_gflag, la->nflag);
+			for (int c = 0; c < C; c++)   // channel planes, the first one sets the weights
+				hipLaunchKernelGGL(sc.sPx == 4 ? filter_learned_kernel<4> : filter_learned_kernel<8>, dim3(Gc),
+				                   dim3(64), 0, ctx->stream, d_planes + (size_t)c * WHF, d_idx, d_nsim, d_base,
+				                   dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, la->net, la->sigma,
+				                   la->gvar_thr, d_est, d_ew, d_etile, d_early, d_gw, d_gflag, la->nflag, C, c);
 		else if (pl.fast && step1)
 			hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
 			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, dv.gtile, sc, g0,
@@ -981,31 +994,14 @@ void stats_end(bm4dof_ctx *ctx)
 	ctx->agg_overflow_blocks = cnt[4];
 }
 
-// Whole runNlBayes on device buffers (nldct.cpp:267-478).
-void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, const float *noisy,
-             int W, int H, int F, int C, const float *ff, const float *bf, float *basic, float *final,
-             int nparts, float *ratio)
+// The two steps of runNlBayes on device buffers (nldct.cpp:352-470), with
+// the YUV transform of colour video around each step (nldct.cpp:363, 401,
+// 422-423, 467-468): step 0 both, 1 the first only, 2 the second only.
+static void two_step(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, int step,
+                     const float *noisy, int W, int H, int F, int C, const float *ff, const float *bf,
+                     float *basic, float *final, int nparts, Timer &tm, float *gr)
 {
-	if (!(C == 1 || C == 3 || C == 4))
-		throw ArgError("VideoNLB::runNlBayes: Wrong number of channels. Must be 1, 3 or 4.");
-	const int step = p1->sizePatch ? (p2->sizePatch ? 0 : 1) : (p2->sizePatch ? 2 : -1);
-	if (step == -1) throw ArgError("VideoNLB::runNlBayes: Both patch sizes are zero.");
-	if (nparts < 1) throw ArgError("nparts must be >= 1");
-	if (nparts > 255) throw ArgError("nparts above 255 is not supported", BM4DOF_EUNSUPP);
-	if (W <= 0 || H <= 0 || F <= 0) throw ArgError("empty video");
-	if ((uint64_t)W * H * F * C >= (1ull << 32))
-		throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
-	if (step != 1 && !final) throw ArgError("final output buffer is NULL");
-	HIPCHK(hipSetDevice(ctx->device));
-	for (auto e : ctx->ev) (void)hipEventDestroy(e);
-	ctx->ev.clear();
-	ctx->timings.clear();
-	stats_begin(ctx);
-	Timer tm(ctx);
-	tm.mark("start");
 	const size_t n = (size_t)W * H * F * C;
-	float gr[2] = {0.f, 0.f};
-
 	if (step != 2) {
 		const float *nw = noisy;
 		if (C > 1 && p1->colorSpace == BM4DOF_YUV) {
@@ -1032,6 +1028,32 @@ void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, 
 			color(ctx, final, W, H, F, C, false);
 		}
 	}
+}
+
+// Whole runNlBayes on device buffers (nldct.cpp:267-478).
+void run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2, const float *noisy,
+             int W, int H, int F, int C, const float *ff, const float *bf, float *basic, float *final,
+             int nparts, float *ratio)
+{
+	if (!(C == 1 || C == 3 || C == 4))
+		throw ArgError("VideoNLB::runNlBayes: Wrong number of channels. Must be 1, 3 or 4.");
+	const int step = p1->sizePatch ? (p2->sizePatch ? 0 : 1) : (p2->sizePatch ? 2 : -1);
+	if (step == -1) throw ArgError("VideoNLB::runNlBayes: Both patch sizes are zero.");
+	if (nparts < 1) throw ArgError("nparts must be >= 1");
+	if (nparts > 255) throw ArgError("nparts above 255 is not supported", BM4DOF_EUNSUPP);
+	if (W <= 0 || H <= 0 || F <= 0) throw ArgError("empty video");
+	if ((uint64_t)W * H * F * C >= (1ull << 32))
+		throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
+	if (step != 1 && !final) throw ArgError("final output buffer is NULL");
+	HIPCHK(hipSetDevice(ctx->device));
+	for (auto e : ctx->ev) (void)hipEventDestroy(e);
+	ctx->ev.clear();
+	ctx->timings.clear();
+	stats_begin(ctx);
+	Timer tm(ctx);
+	tm.mark("start");
+	float gr[2] = {0.f, 0.f};
+	two_step(ctx, p1, p2, step, noisy, W, H, F, C, ff, bf, basic, final, nparts, tm, gr);
 	HIPCHK(hipStreamSynchronize(ctx->stream));
 	stats_end(ctx);
 	if (ratio) { ratio[0] = gr[0]; ratio[1] = gr[1]; }
@@ -1052,11 +1074,14 @@ void run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
                      int W, int H, int F, int C, const float *ff, const float *bf, float *out, int nparts,
                      float gvar_thr, float *flag_frac)
 {
-	if (C != 1) throw ArgError("the learned pass takes one-channel video", BM4DOF_EUNSUPP);
+	if (!(C == 1 || C == 3 || C == 4))
+		throw ArgError("VideoNLB::runNlBayes: Wrong number of channels. Must be 1, 3 or 4.");
 	if (nparts < 1 || nparts > 255) throw ArgError("nparts must be in 1..255");
 	if (W <= 0 || H <= 0 || F <= 0) throw ArgError("empty video");
-	if ((uint64_t)W * H * F >= (1ull << 32)) throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
+	if ((uint64_t)W * H * F * C >= (1ull << 32))
+		throw ArgError("video above 2^32 samples is not supported", BM4DOF_EUNSUPP);
 	if (!p1->sizePatch) throw ArgError("learned pass: sizePatch of the first-step parameters is zero");
+	if (gvar_thr >= 0.f && (!p2 || !p2->sizePatch)) throw ArgError("fallback needs second-step parameters");
 	HIPCHK(hipSetDevice(ctx->device));
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
 	ctx->ev.clear();
@@ -1072,27 +1097,32 @@ void run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
 	la.gflag = nullptr;
 	bm4dof_params pl = *p1;
 	pl.isFirstStep = 1;
-	const uint64_t G = run_step(ctx, pl, W, H, F, 1, noisy, nullptr, ff, bf, out, nparts, tm, &la);
+	const size_t n = (size_t)W * H * F * C;
+	// colour video is filtered in the first step's colour space
+	const bool yuv = C > 1 && p1->colorSpace == BM4DOF_YUV;
+	const float *nw = noisy;
+	if (yuv) {
+		float *t = ctx->noisy_w.as<float>(n);
+		HIPCHK(hipMemcpyAsync(t, noisy, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+		color(ctx, t, W, H, F, C, true);
+		nw = t;
+	}
+	const uint64_t G = run_step(ctx, pl, W, H, F, C, nw, nullptr, ff, bf, out, nparts, tm, &la);
+	if (yuv) color(ctx, out, W, H, F, C, false);
 	uint32_t nflag = 0;
 	HIPCHK(hipMemcpyAsync(&nflag, la.nflag, 4, hipMemcpyDeviceToHost, ctx->stream));
 	HIPCHK(hipStreamSynchronize(ctx->stream));
 	if (flag_frac) *flag_frac = G ? (float)nflag / (float)G : 0.f;
 	if (gvar_thr >= 0.f && nflag > 0) {
-		if (!p2 || !p2->sizePatch) throw ArgError("fallback needs second-step parameters");
-		const size_t n = (size_t)W * H * F;
+		// the plain two-step result for the pixels under flagged groups
 		float *pb = ctx->basic_w.as<float>(n);
 		float *pf = ctx->out_w.as<float>(n);
-		// keep the learned statistics; the plain run appends its own stages
-		const auto ev = ctx->ev;
-		const auto timings = ctx->timings;
-		run_step(ctx, *p1, W, H, F, 1, noisy, nullptr, ff, bf, pb, nparts, tm);
-		run_step(ctx, *p2, W, H, F, 1, noisy, pb, ff, bf, pf, nparts, tm);
+		float gr[2];
+		two_step(ctx, p1, p2, 0, noisy, W, H, F, C, ff, bf, pb, pf, nparts, tm, gr);
 		hipLaunchKernelGGL(fallback_blend_kernel, dim3(nblk((int64_t)n, 256)), dim3(256), 0, ctx->stream,
-		                   ctx->fmask.as<uint8_t>(n), pf, (int64_t)n, 1, out);
+		                   ctx->fmask.as<uint8_t>((size_t)W * H * F), pf, (int64_t)n, W * H, C, out);
 		HIPCHK(hipGetLastError());
 		tm.mark("fallback_blend");
-		(void)ev;
-		(void)timings;
 	}
 	HIPCHK(hipStreamSynchronize(ctx->stream));
 	stats_end(ctx);

@@ -1234,10 +1234,12 @@ __device__ __forceinline__ void fast_forward(const float *__restrict__ img, cons
 }
 
 // Inverse transform of the coefficients zs[n*64 + k] and the weighted est
-// rows: est[(row0+n)*64 + pix] = (w*win[pix]) * sum_k z[k][n] Bi[k][pix].
+// rows: est[(row0+n)*rs + ro + pix] = (w*win[pix]) * sum_k z[k][n] Bi[k][pix]
+// (rs = floats per est row, ro = offset of this channel in the row).
 template <bool PAIRS = true>
 __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const float *__restrict__ binv,
-                                             int N, float ww, uint32_t row0, float *__restrict__ est)
+                                             int N, float ww, uint32_t row0, float *__restrict__ est,
+                                             int rs = FAST_P, int ro = 0)
 {
 	float bcol[FAST_P];
 #pragma unroll
@@ -1259,8 +1261,8 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 			acc0 = acc0 + z0.w * bcol[4 * q + 3];
 			acc1 = acc1 + z1.w * bcol[4 * q + 3];
 		}
-		est[(size_t)(row0 + n) * FAST_P + threadIdx.x] = ww * acc0;
-		est[(size_t)(row0 + n + 1) * FAST_P + threadIdx.x] = ww * acc1;
+		est[(size_t)(row0 + n) * rs + ro + threadIdx.x] = ww * acc0;
+		est[(size_t)(row0 + n + 1) * rs + ro + threadIdx.x] = ww * acc1;
 	}
 	for (; n < N; n++) {
 		const float4 *zr = reinterpret_cast<const float4 *>(zs + n * FAST_P);
@@ -1273,7 +1275,7 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 			acc = acc + z.z * bcol[4 * q + 2];
 			acc = acc + z.w * bcol[4 * q + 3];
 		}
-		est[(size_t)(row0 + n) * FAST_P + threadIdx.x] = ww * acc;
+		est[(size_t)(row0 + n) * rs + ro + threadIdx.x] = ww * acc;
 	}
 }
 
@@ -1489,7 +1491,7 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
                            const float *__restrict__ win, MlpInfer net, float sigma, float gvar_thr,
                            float *__restrict__ est, float *__restrict__ entry_w, uint32_t *__restrict__ etile,
                            uint8_t *__restrict__ early, float *__restrict__ gweight,
-                           uint8_t *__restrict__ gflag, uint32_t *__restrict__ n_flag)
+                           uint8_t *__restrict__ gflag, uint32_t *__restrict__ n_flag, int C, int c)
 {
 	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];   // coefficients, then gains
 	__shared__ __align__(16) uint16_t xb[32 * MLP_XS];
@@ -1501,7 +1503,7 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 	const int N = (int)nsim[g];
 	const uint32_t row0 = base[gl];
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
-	if (lane < N) etile[row0 + lane] = gtile[g];
+	if (c == 0 && lane < N) etile[row0 + lane] = gtile[g];
 
 	__shared__ float rstat[FAST_NMAX];
 	float z[FAST_NMAX];
@@ -1530,17 +1532,25 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
 	const float cnt = (float)(64 * N);
 	const float gvar = s2 / cnt - (s1 / cnt) * (s1 / cnt);
-	const float w = (float)(1. / ((double)sigma * sigma) / (double)s2);
+	// the group's weight comes from its first channel (as the reference
+	// aggregates colour groups, nldct.cpp:2295-2297); a group is flagged when
+	// any channel's gain variance exceeds the threshold
+	const float w = c == 0 ? (float)(1. / ((double)sigma * sigma) / (double)s2) : gweight[g];
+	const uint8_t f = gvar > gvar_thr ? 1 : 0;
 	if (lane == 0) {
-		gweight[g] = w;
-		early[g] = 0;
-		const uint8_t f = gvar > gvar_thr ? 1 : 0;
-		gflag[g] = f;
-		if (f) atomicAdd(n_flag, 1u);
+		if (c == 0) {
+			gweight[g] = w;
+			early[g] = 0;
+			gflag[g] = f;
+			if (f) atomicAdd(n_flag, 1u);
+		} else if (f && !gflag[g]) {
+			gflag[g] = 1;
+			atomicAdd(n_flag, 1u);
+		}
 	}
-	if (lane < N) entry_w[row0 + lane] = w;
+	if (c == 0 && lane < N) entry_w[row0 + lane] = w;
 	__syncthreads();
-	fast_inverse<false>(zs, binv, N, w * win[lane], row0, est);
+	fast_inverse<false>(zs, binv, N, w * win[lane], row0, est, C * FAST_P, c * FAST_P);
 }
 
 // Training pairs: for every `stride`-th group, the features of its rows
@@ -1602,18 +1612,32 @@ __global__ void fallback_mark_kernel(const uint32_t *__restrict__ idx, const uin
 	}
 }
 
-// out = plain where the mask is set (one channel)
-__global__ void fallback_blend_kernel(const uint8_t *__restrict__ mask, const float *__restrict__ plain,
-                                      int64_t n, int unused, float *__restrict__ out)
+// dst[c][t][y][x] = src[t][c][y][x]: the channel planes of a C-channel video
+// as C one-channel videos
+__global__ void planes_kernel(const float *__restrict__ src, int C, int WH, int64_t n, float *__restrict__ dst)
 {
 	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	(void)unused;
-	if (i < n && mask[i]) out[i] = plain[i];
+	if (i >= n) return;
+	const int64_t t = i / ((int64_t)WH * C), r = i - t * (int64_t)WH * C;
+	const int c = (int)(r / WH), q = (int)(r - (int64_t)c * WH);
+	dst[((int64_t)c * (n / C)) + t * WH + q] = src[i];
+}
+
+// out = plain where the mask of the sample's pixel is set
+// (n samples of a [F][C][H][W] video; mask per pixel, W*H per frame)
+__global__ void fallback_blend_kernel(const uint8_t *__restrict__ mask, const float *__restrict__ plain,
+                                      int64_t n, int WH, int C, float *__restrict__ out)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int64_t t = i / ((int64_t)WH * C);
+	const int64_t r = i % WH;
+	if (mask[t * WH + r]) out[i] = plain[i];
 }
 
 #define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	StepConst, int, int, const float *, const float *, const float *, MlpInfer, float, float, float *, \
-	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *
+	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *, int, int
 #define PAIRS_ARGS const float *, const float *, const uint32_t *, const uint32_t *, StepConst, int, int, \
 	const float *, float, float, float *, float *, uint8_t *
 template __global__ void filter_learned_kernel<4>(LEARNED_ARGS);

@@ -149,3 +149,56 @@ def test_fallback_takes_the_plain_result(ctx, trained):
     assert 0.0 <= fs <= 1.0
     mixed = (some == plain) | (some == learned)
     assert mixed.all()
+
+
+def test_colour_channels_follow_the_first_channel(ctx, trained):
+    """RGB video whose three channels are one gray video: matching and
+    weights come from the first channel and every channel is filtered with
+    them, so each output channel equals the one-channel learned pass
+    bitwise."""
+    _, theta, _, _ = trained
+    ctx.mlp_set(theta)
+    clean = synth.video(64, 48, 6, seed=41)
+    gray = synth.add_noise(clean, 20.0, 42)
+    rgb = np.repeat(gray, 3, axis=1)
+    kw = dict(px=4, pt=4, wx=9, np_=16)
+    g1, g2 = A.cli_params(1, 20, gray.shape, **kw), A.cli_params(2, 20, gray.shape, **kw)
+    c1, c2 = A.cli_params(1, 20, rgb.shape, **kw), A.cli_params(2, 20, rgb.shape, **kw)
+    c1.colorSpace = c2.colorSpace = 0   # RGB: no transform
+    want, _ = ctx.run_learned(g1, g2, gray)
+    got, frac = ctx.run_learned(c1, c2, rgb)
+    assert frac == 0.0
+    for c in range(3):
+        assert np.array_equal(got[:, c], want[:, 0]), c
+
+
+@pytest.mark.parametrize("flow", [False, True])
+def test_colour_fallback_takes_the_plain_result(ctx, trained, flow):
+    """Colour video (YUV, 48 similar patches acting as 32, temporal range
+    along the flow): no flagged group gives the learned pass, all flagged
+    gives the plain colour two-step result bitwise, and a threshold between
+    gives each sample one of the two."""
+    _, theta, _, _ = trained
+    ctx.mlp_set(theta)
+    clean = synth.video(64, 48, 6, channels=3, seed=90)
+    noisy = synth.add_noise(clean, 40.0, 91)
+    kw = dict(px=4, pt=4, wx=9, np_=48)
+    p1, p2 = A.cli_params(1, 40, noisy.shape, **kw), A.cli_params(2, 40, noisy.shape, **kw)
+    ff = bf = None
+    if flow:
+        ff, bf = ctx.tvl1_sequence(noisy[:, 0], nscales=3, warps=3)
+        for p in (p1, p2):
+            p.sizeSearchTimeRangeFwd = p.sizeSearchTimeRangeBwd = 2
+    plain = ctx.run(p1, p2, noisy, ff, bf)[1]
+    learned, f0 = ctx.run_learned(p1, p2, noisy, ff, bf, gvar_thr=-1.0)
+    none, fn = ctx.run_learned(p1, p2, noisy, ff, bf, gvar_thr=1e9)
+    assert f0 == 0.0 and fn == 0.0 and np.array_equal(none, learned)
+    allf, fa = ctx.run_learned(p1, p2, noisy, ff, bf, gvar_thr=0.0)
+    assert fa == 1.0 and np.array_equal(allf, plain)
+    some, fs = ctx.run_learned(p1, p2, noisy, ff, bf, gvar_thr=0.02)
+    assert 0.0 < fs < 1.0
+    assert ((some == plain) | (some == learned)).all()
+    ps = lambda v: 10 * np.log10(255.0 ** 2 / np.mean((v.astype(np.float64) - clean) ** 2))
+    print("colour psnr noisy %.2f learned %.2f plain %.2f blend %.2f (flagged %.3f)" %
+          (ps(noisy), ps(learned), ps(plain), ps(some), fs))
+    assert ps(learned) > ps(noisy) + 5.0
