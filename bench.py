@@ -13,7 +13,10 @@ whole video, inputs and flows already resident in HBM (the flows are
 computed once beforehand by the GPU TV-L1; its time is reported apart).
 `value` is Mpix/s = 960*540*20 / seconds per step.  The `learned` entry
 times the learned single pass (shipped network weights) on the same video
-and flows and gives its PSNR against the plain two-step output.
+and flows and gives its PSNR against the plain two-step output.  The
+`color_fallback` entry runs the colour workload (RGB 1920x1080x30, sigma 40,
+np 48, 15x15 search, +-6 frames): plain two-step, learned single pass, and
+learned pass with the plain fallback, with the fallback fraction.
 
 The roofline entry is for the kernel with the largest share of the step,
 from the library's own HIP events on its stream.  The CPU baseline times the
@@ -141,6 +144,83 @@ def batch32(A, S, ctx, torch, nparts):
             "create_s_incl_autotune": round(create_s, 2), "generation_s": round(gen_s, 1)}
 
 
+def color_fallback(A, S, L, ctx, torch, nparts, gvar_thr, threads):
+    """Colour RGB video 1920x1080x30, sigma 40, filtered in the opponent
+    colour space (the parameters' default YUV transform), seeded scene of 40
+    moving shapes with per-channel textures; px=pt=4, np1=np2=48 (the
+    power-of-two cut keeps 32), 15x15 spatial search, temporal +-6 frames
+    along TV-L1 flow computed on the luma of the noisy video.  Times the
+    plain two-step path, the learned single pass, and the learned pass with
+    the plain fallback under groups whose gain variance exceeds gvar_thr,
+    and reports the flagged fraction and PSNRs."""
+    import numpy as np
+    Wc, Hc, Fc, sig = 1920, 1080, 30, 40.0
+    t0 = time.perf_counter()
+    clean = S.video(Wc, Hc, Fc, channels=3, n_shapes=40, seed=4242, threads=threads)
+    noisy = S.add_noise(clean, sig, 4243)
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] colour video {Wc}x{Hc}x{Fc}x3 generated in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    dn = torch.from_numpy(noisy).to(dev)
+    luma = (0.299 * dn[:, 0] + 0.587 * dn[:, 1] + 0.114 * dn[:, 2]).contiguous()
+    ff = torch.empty((Fc, 2, Hc, Wc), dtype=torch.float32, device=dev)
+    bf = torch.empty_like(ff)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.tvl1_sequence_dev(luma.data_ptr(), Wc, Hc, Fc, ff.data_ptr(), bf.data_ptr())
+    torch.cuda.synchronize()
+    tvl1_s = time.perf_counter() - t0
+    p1 = A.cli_params(1, sig, noisy.shape, px=4, pt=4, wx=15, np_=48)
+    p2 = A.cli_params(2, sig, noisy.shape, px=4, pt=4, wx=15, np_=48)
+    for p in (p1, p2):
+        p.sizeSearchTimeRangeFwd = p.sizeSearchTimeRangeBwd = 6
+    db, dfin, dl = torch.empty_like(dn), torch.empty_like(dn), torch.empty_like(dn)
+
+    def timed(fn, reps):
+        out = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+        return dt, out   # the last repetition (the first plans and allocates)
+
+    plain_s, _ = timed(lambda: ctx.run_dev(p1, p2, dn.data_ptr(), noisy.shape, ff.data_ptr(), bf.data_ptr(),
+                                           db.data_ptr(), dfin.data_ptr(), nparts=nparts), 2)
+    plain_st = {}
+    for name, ms in ctx.timings():
+        plain_st[name] = round(plain_st.get(name, 0.0) + ms, 3)
+    plain = dfin.cpu().numpy()
+    print(f"[bench] colour plain two-step: {plain_s:.3f} s", file=sys.stderr, flush=True)
+    ctx.mlp_set(L.load())
+
+    def learned(thr):
+        return ctx.run_learned_dev(p1, p2, dn.data_ptr(), noisy.shape, ff.data_ptr(), bf.data_ptr(),
+                                   dl.data_ptr(), nparts=nparts, gvar_thr=thr)
+
+    l_s, _ = timed(lambda: learned(-1.0), 2)
+    l_psnr = _psnr(dl.cpu().numpy(), clean)
+    fb_s, frac = timed(lambda: learned(gvar_thr), 1)
+    fb_psnr = _psnr(dl.cpu().numpy(), clean)
+    print(f"[bench] colour learned {l_s:.3f} s, with fallback {fb_s:.3f} s (flagged {frac:.4f})",
+          file=sys.stderr, flush=True)
+    mpix = Wc * Hc * Fc / 1e6
+    return {
+        "workload": f"colour RGB {Wc}x{Hc}x{Fc}, sigma 40, opponent colour space, px 4, pt 4, np1=np2=48, "
+                    "wx 15, temporal +-6 along TV-L1 flow (luma)",
+        "unit": "Mpix/s",
+        "plain": {"value": round(mpix / plain_s, 4), "s": round(plain_s, 3),
+                  "psnr_db": round(_psnr(plain, clean), 3), "stages_ms": plain_st},
+        "learned": {"value": round(mpix / l_s, 4), "s": round(l_s, 3), "psnr_db": round(l_psnr, 3)},
+        "learned_with_fallback": {"value": round(mpix / fb_s, 4), "s": round(fb_s, 3),
+                                  "psnr_db": round(fb_psnr, 3), "gvar_thr": gvar_thr,
+                                  "fallback_fraction": round(frac, 5)},
+        "psnr_noisy_db": round(_psnr(noisy, clean), 3),
+        "tvl1_sequence_s": round(tvl1_s, 3), "generation_s": round(gen_s, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +234,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned single pass")
     ap.add_argument("--batch32", action="store_true",
                     help="also time the batched workload: 32 uneven gray sequences in one hipGraph")
+    ap.add_argument("--no-color", action="store_true",
+                    help="skip the colour 1920x1080x30 learned pass with plain fallback")
+    ap.add_argument("--color-gvar", type=float, default=0.02,
+                    help="gain-variance threshold of the colour fallback")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -232,7 +316,7 @@ def main():
 
     # dominant kernel among the transform/filter, matching and aggregation stages
     # kernels this workload runs (one channel, 4x4x4 patches, 32 neighbours)
-    kern = {"s1_bm": "bm_fast_kernel<4, 4>", "s2_bm": "bm_fast_kernel<4, 4>",
+    kern = {"s1_bm": "bm_reg_kernel<4, 4>", "s2_bm": "bm_reg_kernel<4, 4>",
             "s1_filter": "filter1_fast_kernel<4>", "s2_filter": "filter2_fast_kernel<4>",
             "s1_aggregate": "agg_chunk_kernel<1, 1024>", "s2_aggregate": "agg_chunk_kernel<1, 1024>"}
     dom = max((k for k in stage_ms if k in kern), key=lambda k: stage_ms[k])
@@ -309,6 +393,10 @@ def main():
         print(f"[bench] learned single pass: {lv:.2f} Mpix/s", file=sys.stderr, flush=True)
     if args.batch32:
         out["batch32"] = batch32(A, S, ctx, torch, args.nparts)
+    if not args.no_color:
+        L = _load("bm4dof_amd.learned", os.path.join("bm4d-of_amd", "learned.py"))
+        out["color_fallback"] = color_fallback(A, S, L, ctx, torch, args.nparts, args.color_gvar,
+                                               min(16, len(os.sched_getaffinity(0))))
     ctx.close()
     if not args.no_cpu:
         cw, ch = (int(v) for v in args.cpu_crop.lower().split("x"))

@@ -26,7 +26,7 @@ __global__ void bm_window_kernel(const float *, const float *, const uint32_t *,
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
 template <int PX, int PT> __global__ void bm_reg_kernel(const float *, const int4 *, const int2 *,
                                                         const uint32_t *, const uint32_t *, const TileBox *,
-                                                        StepConst, int, uint32_t *, uint32_t *, int);
+                                                        StepConst, int, int, uint32_t *, uint32_t *, int);
 template <int PX, int PT, int NF> __global__ void bm_fast_kernel(BMF_ARGS);
 __global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
                                const uint32_t *, StepConst, int, int, const float *, const float *,
@@ -38,10 +38,10 @@ __global__ void filter2_kernel(const float *, const float *, const uint32_t *, c
                                uint32_t *, uint8_t *, float *, uint32_t *);
 #define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, StepConst, int, int, const float *, const float *, const float *, float *, \
-	float *, uint32_t *, uint8_t *, float *, uint32_t *
+	float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
 #define FILTER2_ARGS const float *, const float *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, const uint32_t *, StepConst, int, int, const float *, const float *, \
-	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *
+	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
 template <int PX> __global__ void filter1_fast_kernel(FILTER1_ARGS);
 template <int PX> __global__ void filter2_fast_kernel(FILTER2_ARGS);
 #define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
@@ -496,8 +496,9 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	const size_t lds = (size_t)maxCand * 12 + (size_t)sc.C * sc.P * 4;
 	if (lds > 60 * 1024) throw ArgError("search region too large for the block-matching kernel", BM4DOF_EUNSUPP);
 	if (G == 0) return;
-	// staged one-wave kernel: distances on one channel (step 1 always takes
-	// channel 0; step 2 sums all channels), patches 4x4x4 or 8x8x1
+	// one-wave kernels, patches 4x4x4 or 8x8x1: step 1 matches on channel 0,
+	// step 2 sums all channels (the register kernel takes any channel count,
+	// the LDS-staged one a single channel)
 	const int PX = sc.sPx, PT = sc.sPt;
 	const bool shape_ok = (PX == 4 && PT == 4) || (PX == 8 && PT == 1);
 	const int NF = ctx->bm_frames;   // frames staged per round: 1, 2 or 4
@@ -505,7 +506,7 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	                                             (size_t)maxCand);
 	const size_t lds_fast = ((size_t)maxCand + stage_floats) * 4;
 	const size_t lds_reg = (size_t)maxCand * 8;
-	if (ctx->fast_bm == 2 && shape_ok && (step == 1 || sc.C == 1) && lds_reg <= 56 * 1024) {
+	if (ctx->fast_bm == 2 && shape_ok && lds_reg <= 56 * 1024) {
 		int4 *d_wins = ctx->bm_wins.as<int4>((size_t)G * sWt);
 		int2 *d_wmeta = ctx->bm_wmeta.as<int2>((size_t)G);
 		hipLaunchKernelGGL(bm_window_kernel, dim3(nblk(G, 256)), dim3(256), 0, ctx->stream, ff, bf, d_gpos,
@@ -515,7 +516,7 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 		const int rchunks = std::max(1, std::min(sc.sWx, 64 / std::max(1, sWt * sc.sWx)));
 		auto kern = PX == 4 ? bm_reg_kernel<4, 4> : bm_reg_kernel<8, 1>;
 		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_reg, ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile,
-		                   d_tiles, sc, rchunks, d_idx, d_nsim, G);
+		                   d_tiles, sc, rchunks, step == 1 ? 1 : sc.C, d_idx, d_nsim, G);
 		HIPCHK(hipGetLastError());
 		return;
 	}
@@ -683,13 +684,12 @@ StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F,
 	pl.filt_lds = pl.step1 ? ((size_t)C * pl.sc.P * pl.np + (size_t)pl.sc.P * pl.np) * 4
 	                       : ((size_t)C * pl.sc.P * pl.np + 2 * (size_t)pl.sc.P * pl.np) * 4;
 	if (pl.filt_lds > 64 * 1024) throw ArgError("group too large for the filter kernel", BM4DOF_EUNSUPP);
-	// register-resident filter: one channel, 64-sample patches, at most 32
-	// neighbours (np < 64: the power-of-two cut keeps at most 32)
-	const bool shape64 = pl.sc.P == 64 && pl.np < 64 &&
-	                     ((pl.sc.sPx == 4 && pl.sc.sPt == 4) || (pl.sc.sPx == 8 && pl.sc.sPt == 1));
-	pl.fast_shape = C == 1 && shape64;
+	// register-resident filters: 64-sample patches, at most 32 neighbours
+	// (np < 64: the power-of-two cut keeps at most 32), one launch per channel
+	pl.fast_shape = pl.sc.P == 64 && pl.np < 64 &&
+	                ((pl.sc.sPx == 4 && pl.sc.sPt == 4) || (pl.sc.sPx == 8 && pl.sc.sPt == 1));
 	pl.fast = ctx->fast_filter && pl.fast_shape && !pl.sc.order_inv;
-	if (learned && !shape64)
+	if (learned && !pl.fast_shape)
 		throw ArgError("learned pass: 4x4x4 or 8x8x1 patches and fewer than 64 similar patches", BM4DOF_EUNSUPP);
 	return pl;
 }
@@ -763,7 +763,11 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 	if (pl.learned) {
 		(void)ctx->gflag.as<uint8_t>(G);
 		(void)ctx->fmask.as<uint8_t>(pl.WHF);
-		if (pl.C > 1) (void)ctx->planes.as<float>(pl.N_all);
+	}
+	if (pl.C > 1 && (pl.learned || pl.fast)) {
+		(void)ctx->planes.as<float>(pl.N_all);
+		(void)ctx->gacc.as<float>(G);
+		if (!pl.step1) (void)ctx->planes_b.as<float>(pl.N_all);
 	}
 	reserve_scan(ctx, (int64_t)std::max(pl.WHF + 1, G + 1));
 }
@@ -800,14 +804,22 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	       d_nsim);
 	mark("l_bm", "s1_bm", "s2_bm");
 	uint8_t *d_gflag = la ? ctx->gflag.as<uint8_t>(std::max(G, 1)) : nullptr;
-	// the learned filter reads one-channel planes
-	const float *d_planes = noisy_w;
-	if (la && C > 1) {
-		float *pl_buf = ctx->planes.as<float>(N_all);
+	// the register-resident filters read one-channel planes
+	const float *d_nplanes = noisy_w, *d_bplanes = basic_w;
+	float *d_gacc = nullptr;
+	if (C > 1 && (la || pl.fast)) {
+		float *pn = ctx->planes.as<float>(N_all);
 		hipLaunchKernelGGL(planes_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream, noisy_w, C,
-		                   sc.WH, (int64_t)N_all, pl_buf);
+		                   sc.WH, (int64_t)N_all, pn);
+		d_nplanes = pn;
+		if (!step1 && !la) {
+			float *pb = ctx->planes_b.as<float>(N_all);
+			hipLaunchKernelGGL(planes_kernel, dim3(nblk((int64_t)N_all, 256)), dim3(256), 0, ctx->stream, basic_w,
+			                   C, sc.WH, (int64_t)N_all, pb);
+			d_bplanes = pb;
+		}
+		d_gacc = ctx->gacc.as<float>(std::max(G, 1));
 		HIPCHK(hipGetLastError());
-		d_planes = pl_buf;
 	}
 
 	for (int g0 = 0; g0 < G; g0 += pl.chunk_groups) {
@@ -842,18 +854,21 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		if (la)
 			for (int c = 0; c < C; c++)   // channel planes, the first one sets the weights
 				hipLaunchKernelGGL(sc.sPx == 4 ? filter_learned_kernel<4> : filter_learned_kernel<8>, dim3(Gc),
-				                   dim3(64), 0, ctx->stream, d_planes + (size_t)c * WHF, d_idx, d_nsim, d_base,
+				                   dim3(64), 0, ctx->stream, d_nplanes + (size_t)c * WHF, d_idx, d_nsim, d_base,
 				                   dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, la->net, la->sigma,
 				                   la->gvar_thr, d_est, d_ew, d_etile, d_early, d_gw, d_gflag, la->nflag, C, c);
 		else if (pl.fast && step1)
-			hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
-			                   dim3(64), 0, ctx->stream, noisy_w, d_idx, d_nsim, d_base, dv.gtile, sc, g0,
-			                   Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt);
+			for (int c = 0; c < C; c++)   // channel planes; the last launch weights all
+				hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
+				                   dim3(64), 0, ctx->stream, d_nplanes, d_idx, d_nsim, d_base, dv.gtile, sc, g0,
+				                   Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt, C, c,
+				                   (int64_t)WHF, d_gacc);
 		else if (pl.fast)
-			hipLaunchKernelGGL(sc.sPx == 4 ? filter2_fast_kernel<4> : filter2_fast_kernel<8>, dim3(Gc),
-			                   dim3(64), 0, ctx->stream, noisy_w, basic_w, d_idx, d_nsim, d_base, dv.gtile,
-			                   sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early, d_gw,
-			                   d_cnt);
+			for (int c = 0; c < C; c++)
+				hipLaunchKernelGGL(sc.sPx == 4 ? filter2_fast_kernel<4> : filter2_fast_kernel<8>, dim3(Gc),
+				                   dim3(64), 0, ctx->stream, d_nplanes, d_bplanes, d_idx, d_nsim, d_base,
+				                   dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early,
+				                   d_gw, d_cnt, C, c, (int64_t)WHF, d_gacc);
 		else if (step1)
 			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), pl.filt_lds, ctx->stream, noisy_w,
 			                   d_idx, d_nsim, d_base, dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est,
@@ -1344,7 +1359,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
+	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
 	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
 	                  &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
 	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};

@@ -674,7 +674,7 @@ __global__ __launch_bounds__(64)
 void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
                    const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
                    const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
-                   StepConst sc, int rchunks, uint32_t *__restrict__ out_idx,
+                   StepConst sc, int rchunks, int nch, uint32_t *__restrict__ out_idx,
                    uint32_t *__restrict__ out_nsim, int G)
 {
 	constexpr int PP = PX * PX * PT;
@@ -707,56 +707,63 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	const int nframes = meta.x, nsrch = meta.y;
 	const int4 *gw = wins + (size_t)g * sWt;
 
-	// reference patch (channel 0), wave-uniform
-	float ref[PP];
-#pragma unroll
-	for (int ht = 0; ht < PT; ht++)
-#pragma unroll
-		for (int hy = 0; hy < PX; hy++)
-#pragma unroll
-			for (int hx = 0; hx < PX; hx++)
-				ref[(ht * PX + hy) * PX + hx] = img[(size_t)(gt + ht) * sc.WHC + (size_t)(gy + hy) * sc.W + gx + hx];
-
 	// items: (frame j, column cx, row chunk r), frame-major
 	int total = 0;
 	for (int j = 0; j < nframes; j++) total += (gw[j].z & 0xffff) * rchunks;
 	uint32_t lmin = 0xffffffffu;
-	for (int it = lane; it < total; it += 64) {
-		int j = 0, loc = it;
-		int4 f = gw[0];
-		for (;;) {
-			const int nit = (f.z & 0xffff) * rchunks;
-			if (loc < nit) break;
-			loc -= nit;
-			f = gw[++j];
-		}
-		const int nx = f.z & 0xffff, ny = f.z >> 16;
-		const int r = loc / nx, cx = loc - r * nx;
-		const int y0 = r * ny / rchunks, y1 = (r + 1) * ny / rchunks;
-		if (y0 >= y1) continue;
-		const float *p = img + (size_t)f.x * sc.WHC + (size_t)(tb.oy + (f.y >> 16)) * sc.W + (tb.ox + (f.y & 0xffff) + cx);
-		uint32_t *kout = skey + f.w + cx;
-		float w[PX][PT][PX];
+	// channels in order (step 1: channel 0; step 2: all): a channel's sums
+	// continue the previous channel's, which the same lane stored in skey
+	for (int c = 0; c < nch; c++) {
+		const bool last = c == nch - 1;
+		const float *imc = img + (size_t)c * sc.WH;
+		// reference patch of this channel, wave-uniform
+		float ref[PP];
 #pragma unroll
-		for (int k = 0; k < PX - 1; k++) bmr_load_row<PX, PT>(w[k], p + (size_t)(y0 + k) * sc.W, sc.WHC);
-		for (int y = y0; y < y1; y += PX) {
+		for (int ht = 0; ht < PT; ht++)
 #pragma unroll
-			for (int u = 0; u < PX; u++) {
-				if (y + u < y1) {
-					bmr_load_row<PX, PT>(w[(u + PX - 1) % PX], p + (size_t)(y + u + PX - 1) * sc.W, sc.WHC);
-					float d = 0.f;
+			for (int hy = 0; hy < PX; hy++)
 #pragma unroll
-					for (int ht = 0; ht < PT; ht++)
+				for (int hx = 0; hx < PX; hx++)
+					ref[(ht * PX + hy) * PX + hx] =
+					    imc[(size_t)(gt + ht) * sc.WHC + (size_t)(gy + hy) * sc.W + gx + hx];
+		for (int it = lane; it < total; it += 64) {
+			int j = 0, loc = it;
+			int4 f = gw[0];
+			for (;;) {
+				const int nit = (f.z & 0xffff) * rchunks;
+				if (loc < nit) break;
+				loc -= nit;
+				f = gw[++j];
+			}
+			const int nx = f.z & 0xffff, ny = f.z >> 16;
+			const int r = loc / nx, cx = loc - r * nx;
+			const int y0 = r * ny / rchunks, y1 = (r + 1) * ny / rchunks;
+			if (y0 >= y1) continue;
+			const float *p = imc + (size_t)f.x * sc.WHC + (size_t)(tb.oy + (f.y >> 16)) * sc.W +
+			                 (tb.ox + (f.y & 0xffff) + cx);
+			uint32_t *kout = skey + f.w + cx;
+			float w[PX][PT][PX];
 #pragma unroll
-						for (int hy = 0; hy < PX; hy++)
+			for (int k = 0; k < PX - 1; k++) bmr_load_row<PX, PT>(w[k], p + (size_t)(y0 + k) * sc.W, sc.WHC);
+			for (int y = y0; y < y1; y += PX) {
 #pragma unroll
-							for (int hx = 0; hx < PX; hx++) {
-								const float dif = ref[(ht * PX + hy) * PX + hx] - w[(u + hy) % PX][ht][hx];
-								d = d + dif * dif;
-							}
-					const uint32_t kb = __float_as_uint(d);
-					kout[(y + u) * nx] = kb;
-					lmin = min(lmin, kb);
+				for (int u = 0; u < PX; u++) {
+					if (y + u < y1) {
+						bmr_load_row<PX, PT>(w[(u + PX - 1) % PX], p + (size_t)(y + u + PX - 1) * sc.W, sc.WHC);
+						float d = c ? __uint_as_float(kout[(y + u) * nx]) : 0.f;
+#pragma unroll
+						for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+							for (int hy = 0; hy < PX; hy++)
+#pragma unroll
+								for (int hx = 0; hx < PX; hx++) {
+									const float dif = ref[(ht * PX + hy) * PX + hx] - w[(u + hy) % PX][ht][hx];
+									d = d + dif * dif;
+								}
+						const uint32_t kb = __float_as_uint(d);
+						kout[(y + u) * nx] = kb;
+						if (last) lmin = min(lmin, kb);
+					}
 				}
 			}
 		}
@@ -826,7 +833,7 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 }
 
 #define BMR_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
-	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+	const TileBox *, StepConst, int, int, uint32_t *, uint32_t *, int
 template __global__ void bm_reg_kernel<4, 4>(BMR_ARGS);
 template __global__ void bm_reg_kernel<8, 1>(BMR_ARGS);
 
@@ -1280,24 +1287,45 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 }
 
 // Equality early-out (nldct.cpp:1986-1999, 2222-2235) with lane = sample:
-// true when N > 1 and every patch equals patch 0; such a group is written
-// raw and flagged for the stale-weight fixup.
+// true when N > 1 and, in every one of the C channel planes (cstride
+// apart), every patch equals patch 0; such a group is written raw (rows of
+// C*64 samples) and flagged for the stale-weight fixup.
 template <int PX>
 __device__ __forceinline__ bool fast_early_out(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
-                                               int N, int W, int WH, int g, uint32_t row0,
-                                               float *__restrict__ est, uint8_t *__restrict__ early,
+                                               int N, int W, int WH, int C, int64_t cstride, int g,
+                                               uint32_t row0, float *__restrict__ est, uint8_t *__restrict__ early,
                                                float *__restrict__ gweight, uint32_t *__restrict__ n_early)
 {
 	if (N <= 1) return false;
 	const int l = threadIdx.x;
 	const int off = (l / (PX * PX)) * WH + ((l / PX) % PX) * W + (l % PX);
-	const float v0 = img[gidx[0] + off];
 	int differs = 0;
-	for (int n = 1; n < N; n++) differs |= (img[gidx[n] + off] != v0);
+	for (int c = 0; c < C; c++) {
+		const float *im = img + c * cstride;
+		const float v0 = im[gidx[0] + off];
+		for (int n = 1; n < N; n++) differs |= (im[gidx[n] + off] != v0);
+	}
 	if (__any(differs)) return false;
-	for (int n = 0; n < N; n++) est[(size_t)(row0 + n) * FAST_P + l] = img[gidx[n] + off];
+	for (int c = 0; c < C; c++)
+		for (int n = 0; n < N; n++)
+			est[((size_t)(row0 + n) * C + c) * FAST_P + l] = img[c * cstride + gidx[n] + off];
 	if (l == 0) { early[g] = 1; gweight[g] = 0.f; atomicAdd(n_early, 1u); }
 	return true;
+}
+
+// Multichannel groups are filtered one channel per launch (c = 0..C-1, on
+// channel planes cstride apart).  The group weight counts every channel
+// (nldct.cpp:1820-1936, 2038-2168), so gacc[g] carries the running count or
+// sum from launch to launch; launches before the last store the inverse
+// transform unweighted and the last one scales them by its weight: the same
+// products (w * window) * value as one pass over all channels.
+__device__ __forceinline__ void fast_scale_channels(int N, int C, float ww, uint32_t row0, float *__restrict__ est)
+{
+	for (int n = 0; n < N; n++)
+		for (int c = 0; c < C - 1; c++) {
+			float *e = est + ((size_t)(row0 + n) * C + c) * FAST_P + threadIdx.x;
+			*e = ww * *e;
+		}
 }
 
 template <int PX>
@@ -1309,7 +1337,8 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
                          const float *__restrict__ win, float *__restrict__ est,
                          float *__restrict__ entry_w, uint32_t *__restrict__ etile,
                          uint8_t *__restrict__ early, float *__restrict__ gweight,
-                         uint32_t *__restrict__ n_early)
+                         uint32_t *__restrict__ n_early, int C, int c, int64_t cstride,
+                         float *__restrict__ gacc)
 {
 	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
 	const int gl = blockIdx.x;
@@ -1319,14 +1348,18 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 	const int N = (int)nsim[g];
 	const uint32_t row0 = base[gl];
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
-	if (lane < N) etile[row0 + lane] = gtile[g];
-	if (fast_early_out<PX>(img, gidx, N, sc.W, sc.WH, g, row0, est, early, gweight, n_early)) return;
+	if (c == 0) {
+		if (lane < N) etile[row0 + lane] = gtile[g];
+		if (fast_early_out<PX>(img, gidx, N, sc.W, sc.WH, C, cstride, g, row0, est, early, gweight, n_early))
+			return;
+	} else if (early[g])
+		return;
 
 	{
 		float brow[FAST_P];
 #pragma unroll
 		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
-		fast_forward<PX>(img, gidx, N, sc.W, sc.WH, brow, zs);
+		fast_forward<PX>(img + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
 	}
 	__syncthreads();
 	float z[FAST_NMAX];
@@ -1344,13 +1377,20 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 	haar_lane(z, N, false);
 #pragma unroll
 	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
-	const int nnz_i = wave_sum_int(cnt);
+	const int nnz_i = wave_sum_int(cnt) + (c ? (int)gacc[g] : 0);
+	if (c < C - 1) {
+		if (lane == 0) { gacc[g] = (float)nnz_i; early[g] = 0; }
+		__syncthreads();
+		fast_inverse(zs, binv, N, 1.f, row0, est, C * FAST_P, c * FAST_P);
+		return;
+	}
 	const float nnz = (float)nnz_i;
 	const float w = 1 / sc.sigma2 / nnz;
 	if (lane == 0) { gweight[g] = w; early[g] = 0; }
 	if (lane < N) entry_w[row0 + lane] = w;
 	__syncthreads();
-	fast_inverse(zs, binv, N, w * win[lane], row0, est);
+	fast_inverse(zs, binv, N, w * win[lane], row0, est, C * FAST_P, c * FAST_P);
+	if (C > 1) fast_scale_channels(N, C, w * win[lane], row0, est);
 }
 
 template <int PX>
@@ -1363,7 +1403,8 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
                          const float *__restrict__ win, float *__restrict__ est,
                          float *__restrict__ entry_w, uint32_t *__restrict__ etile,
                          uint8_t *__restrict__ early, float *__restrict__ gweight,
-                         uint32_t *__restrict__ n_early)
+                         uint32_t *__restrict__ n_early, int C, int c, int64_t cstride,
+                         float *__restrict__ gacc)
 {
 	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
 	const int gl = blockIdx.x;
@@ -1373,20 +1414,24 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 	const int N = (int)nsim[g];
 	const uint32_t row0 = base[gl];
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
-	if (lane < N) etile[row0 + lane] = gtile[g];
-	if (fast_early_out<PX>(noisy, gidx, N, sc.W, sc.WH, g, row0, est, early, gweight, n_early)) return;
+	if (c == 0) {
+		if (lane < N) etile[row0 + lane] = gtile[g];
+		if (fast_early_out<PX>(noisy, gidx, N, sc.W, sc.WH, C, cstride, g, row0, est, early, gweight, n_early))
+			return;
+	} else if (early[g])
+		return;
 
 	float z[FAST_NMAX], y[FAST_NMAX];
 	{
 		float brow[FAST_P];
 #pragma unroll
 		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
-		fast_forward<PX>(basic, gidx, N, sc.W, sc.WH, brow, zs);
+		fast_forward<PX>(basic + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
 		__syncthreads();
 #pragma unroll
 		for (int n = 0; n < FAST_NMAX; n++) y[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
 		__syncthreads();
-		fast_forward<PX>(noisy, gidx, N, sc.W, sc.WH, brow, zs);
+		fast_forward<PX>(noisy + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
 	}
 	__syncthreads();
 #pragma unroll
@@ -1402,9 +1447,9 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 			y[n] = w * w;
 		}
 	}
-	// sum of w^2 in the reference's order, coefficient-major then patch:
+	// sum of w^2 in the reference's order, channel, coefficient, then patch:
 	// lane k folds its N terms into the running sum handed on by lane k-1.
-	float v = 0.f;
+	float v = c ? gacc[g] : 0.f;
 	for (int k = 0; k < FAST_P; k++) {
 		float t = v;
 #pragma unroll
@@ -1415,11 +1460,18 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 	__syncthreads();
 #pragma unroll
 	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
+	if (c < C - 1) {
+		if (lane == 0) { gacc[g] = v; early[g] = 0; }
+		__syncthreads();
+		fast_inverse(zs, binv, N, 1.f, row0, est, C * FAST_P, c * FAST_P);
+		return;
+	}
 	const float w = (float)(1. / (double)sc.sigma2 / (double)v);
 	if (lane == 0) { gweight[g] = w; early[g] = 0; }
 	if (lane < N) entry_w[row0 + lane] = w;
 	__syncthreads();
-	fast_inverse(zs, binv, N, w * win[lane], row0, est);
+	fast_inverse(zs, binv, N, w * win[lane], row0, est, C * FAST_P, c * FAST_P);
+	if (C > 1) fast_scale_channels(N, C, w * win[lane], row0, est);
 }
 
 
@@ -1648,10 +1700,10 @@ template __global__ void learned_pairs_kernel<8>(PAIRS_ARGS);
 
 #define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, StepConst, int, int, const float *, const float *, const float *, float *, \
-	float *, uint32_t *, uint8_t *, float *, uint32_t *
+	float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
 #define FILTER2_ARGS const float *, const float *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, const uint32_t *, StepConst, int, int, const float *, const float *, \
-	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *
+	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
 template __global__ void filter1_fast_kernel<4>(FILTER1_ARGS);
 template __global__ void filter1_fast_kernel<8>(FILTER1_ARGS);
 template __global__ void filter2_fast_kernel<4>(FILTER2_ARGS);

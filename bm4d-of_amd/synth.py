@@ -65,8 +65,10 @@ def _bilinear(layer, x, y):
 
 
 def video(width=960, height=540, frames=20, channels=1, n_shapes=20, seed=101,
-          max_shift=3.0, max_rot_deg=1.0):
-    """Moving textured scene, (F, C, H, W) float32 in [0, 255]-ish range."""
+          max_shift=3.0, max_rot_deg=1.0, threads=1):
+    """Moving textured scene, (F, C, H, W) float32 in [0, 255]-ish range.
+    Frames are rendered independently; `threads` > 1 renders them on a
+    thread pool with the same result."""
     rng = np.random.default_rng(seed)
     yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
     # smooth background gradient per channel
@@ -97,7 +99,8 @@ def video(width=960, height=540, frames=20, channels=1, n_shapes=20, seed=101,
             vx=rng.uniform(-max_shift, max_shift), vy=rng.uniform(-max_shift, max_shift),
             va=np.deg2rad(rng.uniform(-max_rot_deg, max_rot_deg))))
     out = np.zeros((frames, channels, height, width), np.float32)
-    for t in range(frames):
+
+    def render(t):
         img = bg.copy()
         for s in shapes:
             cx, cy = s["cx"] + s["vx"] * t, s["cy"] + s["vy"] * t
@@ -116,6 +119,14 @@ def video(width=960, height=540, frames=20, channels=1, n_shapes=20, seed=101,
             a = smp[..., -1:]
             img[y0:y1, x0:x1] = img[y0:y1, x0:x1] * (1 - a) + smp[..., :-1]
         out[t] = np.moveaxis(img, -1, 0).astype(np.float32)
+
+    if threads > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(render, range(frames)))
+    else:
+        for t in range(frames):
+            render(t)
     return out
 
 

@@ -225,6 +225,67 @@ def test_color_video_parity(ctx, channels, flow):
     assert ggr[0] < gr_gray[0] and ggr[1] < gr_gray[1]
 
 
+@pytest.mark.parametrize("channels,px,pt,flow,flat", [(3, 4, 4, True, False), (3, 4, 4, False, True),
+                                                      (4, 8, 1, False, True)])
+def test_color_register_paths_parity(ctx, channels, px, pt, flow, flat):
+    """Colour video through the register-resident filters (one launch per
+    channel, the group weight counting every channel) and the register-window
+    matching (step-2 distances summed over the channels): bitwise equal to
+    the CPU oracle and to the general kernels.  `flat` (RGB, no transform)
+    puts a region constant in every channel, which takes the equality
+    early-out, beside one constant in the first channel only, which must
+    not."""
+    f = 7 if pt > 1 else 2
+    clean, noisy = _video(48, 40, f, seed=60 + channels + px, channels=channels)
+    if flat:
+        for c in range(channels):
+            noisy[:, c, :, :14] = 40.0 + 30 * c
+        noisy[:, 0, 24:, 20:] = 90.0
+    ff = bf = None
+    if flow:
+        ff, bf = _smooth_flow(f, 40, 48, 3), _smooth_flow(f, 40, 48, 4)
+    a1, o1 = _params(noisy.shape, 1, px=px, pt=pt, wx=7, np_=16)
+    a2, o2 = _params(noisy.shape, 2, px=px, pt=pt, wx=7, np_=16)
+    if flat:
+        for p in (a1, o1, a2, o2):
+            p.colorSpace = 0
+    ob, of_, ogr = O.run(o1, o2, noisy, ff, bf, nparts=2, tie_mode=1)
+    gb, gf, ggr = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+    _same(gb, ob, "basic")
+    _same(gf, of_, "final")
+    try:
+        ctx.set_option("fast_filter", 0)
+        ctx.set_option("fast_bm", 0)
+        b0, f0, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+    finally:
+        ctx.set_option("fast_filter", 1)
+        ctx.set_option("fast_bm", 2)
+    assert np.array_equal(b0, gb) and np.array_equal(f0, gf)
+
+
+@pytest.mark.parametrize("channels,flow", [(3, True), (4, False)])
+def test_color_block_matching_kernels_agree(ctx, channels, flow):
+    """Step-2 matching on colour video sums the distances of all channels
+    in channel order: the register-window kernel's neighbour lists equal
+    the general kernel's."""
+    f = 8
+    clean, _ = _video(64, 48, f, seed=90 + channels, channels=channels)
+    clean[:, :, :20, :20] = 60.0
+    ff = bf = None
+    if flow:
+        ff, bf = _smooth_flow(f, 48, 64, 9), _smooth_flow(f, 48, 64, 10)
+    a = A.cli_params(2, 20, clean.shape, px=4, pt=4, wx=11, np_=32)
+    res = {}
+    try:
+        for mode in (0, 2):
+            ctx.set_option("fast_bm", mode)
+            res[mode] = ctx.bm(a, clean, ff, bf, nparts=2)
+    finally:
+        ctx.set_option("fast_bm", 2)
+    for u, v in zip(res[0], res[2]):
+        assert np.array_equal(u, v)
+
+
 def test_order_invariance_and_np1(ctx):
     clean, noisy = _video(40, 32, 6, seed=77)
     for kw in (dict(order_inv=True), dict(np_=1)):
