@@ -1584,9 +1584,10 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
 	const float cnt = (float)(64 * N);
 	const float gvar = s2 / cnt - (s1 / cnt) * (s1 / cnt);
-	// the group's weight comes from its first channel (as the reference
-	// aggregates colour groups, nldct.cpp:2295-2297); a group is flagged when
-	// any channel's gain variance exceeds the threshold
+	// one weight for all channels of a group, as the reference aggregates
+	// colour groups (nldct.cpp:2293-2299), here from the first channel's
+	// gains; a group is flagged when any channel's gain variance exceeds the
+	// threshold
 	const float w = c == 0 ? (float)(1. / ((double)sigma * sigma) / (double)s2) : gweight[g];
 	const uint8_t f = gvar > gvar_thr ? 1 : 0;
 	if (lane == 0) {
