@@ -45,10 +45,11 @@ __global__ void filter2_kernel(const float *, const float *, const uint32_t *, c
 template <int PX> __global__ void filter1_fast_kernel(FILTER1_ARGS);
 template <int PX> __global__ void filter2_fast_kernel(FILTER2_ARGS);
 #define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
-	StepConst, int, int, const float *, const float *, const float *, MlpInfer, float, float, float *, \
+	StepConst, int, int, const uint16_t *, const float *, MlpInfer, float, float, float *, \
 	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *, int, int
 #define PAIRS_ARGS const float *, const float *, const uint32_t *, const uint32_t *, StepConst, int, int, \
-	const float *, float, float, float *, float *, uint8_t *
+	const uint16_t *, float, float, float *, float *, uint8_t *
+__global__ void split_basis_kernel(const float *, const float *, uint16_t *);
 template <int PX> __global__ void filter_learned_kernel(LEARNED_ARGS);
 template <int PX> __global__ void learned_pairs_kernel(PAIRS_ARGS);
 __global__ void fallback_mark_kernel(const uint32_t *, const uint32_t *, const uint8_t *, StepConst, int,
@@ -698,6 +699,7 @@ StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F,
 // group per tile) in buffers that outlive the step's launches.
 struct StepDev {
 	float *basis = nullptr, *binv = nullptr, *win = nullptr;
+	uint16_t *bsp = nullptr;   // fp16 split tables of 64-sample bases (matrix-core transforms)
 	TileBox *tiles = nullptr;
 	uint32_t *gpos = nullptr, *gtile = nullptr, *tile_first = nullptr;
 };
@@ -710,7 +712,8 @@ StepDev upload_plan(bm4dof_ctx *ctx, const StepPlan &pl, DevBuf &geo_buf)
 	auto al = [](size_t b) { return (b + 255) / 256 * 256; };
 	const size_t o_basis = 0, o_binv = o_basis + al(nb * 4), o_win = o_binv + al(nbi * 4),
 	             o_tiles = o_win + al(nw * 4), o_gpos = o_tiles + al(nt * sizeof(TileBox)),
-	             o_gtile = o_gpos + al(G * 4), o_tf = o_gtile + al(G * 4), total = o_tf + al(std::max<size_t>(ntf, 1) * 4);
+	             o_gtile = o_gpos + al(G * 4), o_tf = o_gtile + al(G * 4), o_bsp = o_tf + al(std::max<size_t>(ntf, 1) * 4),
+	             total = o_bsp + al(nb == 4096 ? 4 * nb * 2 : 0);
 	char *b = (char *)geo_buf.get(total);
 	StepDev d;
 	d.basis = (float *)(b + o_basis);
@@ -720,6 +723,7 @@ StepDev upload_plan(bm4dof_ctx *ctx, const StepPlan &pl, DevBuf &geo_buf)
 	d.gpos = (uint32_t *)(b + o_gpos);
 	d.gtile = (uint32_t *)(b + o_gtile);
 	d.tile_first = (uint32_t *)(b + o_tf);
+	d.bsp = nb == 4096 ? (uint16_t *)(b + o_bsp) : nullptr;
 	HIPCHK(hipMemcpy(d.basis, pl.hb.data(), nb * 4, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d.binv, pl.hbi.data(), nbi * 4, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d.win, pl.hw.data(), nw * 4, hipMemcpyHostToDevice));
@@ -729,6 +733,11 @@ StepDev upload_plan(bm4dof_ctx *ctx, const StepPlan &pl, DevBuf &geo_buf)
 		HIPCHK(hipMemcpy(d.gtile, pl.geo.gtile.data(), (size_t)pl.G * 4, hipMemcpyHostToDevice));
 	}
 	if (ntf) HIPCHK(hipMemcpy(d.tile_first, pl.geo.tile_first.data(), ntf * 4, hipMemcpyHostToDevice));
+	if (d.bsp) {
+		hipLaunchKernelGGL(split_basis_kernel, dim3(16), dim3(256), 0, ctx->stream, d.basis, d.binv, d.bsp);
+		HIPCHK(hipGetLastError());
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+	}
 	return d;
 }
 
@@ -855,7 +864,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 			for (int c = 0; c < C; c++)   // channel planes, the first one sets the weights
 				hipLaunchKernelGGL(sc.sPx == 4 ? filter_learned_kernel<4> : filter_learned_kernel<8>, dim3(Gc),
 				                   dim3(64), 0, ctx->stream, d_nplanes + (size_t)c * WHF, d_idx, d_nsim, d_base,
-				                   dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, la->net, la->sigma,
+				                   dv.gtile, sc, g0, Gc, dv.bsp, dv.win, la->net, la->sigma,
 				                   la->gvar_thr, d_est, d_ew, d_etile, d_early, d_gw, d_gflag, la->nflag, C, c);
 		else if (pl.fast && step1)
 			for (int c = 0; c < C; c++)   // channel planes; the last launch weights all
@@ -1167,15 +1176,19 @@ int64_t learned_pairs_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const float 
 	build_bases(p, hb, hbi, hw);
 	StepGeom geo = make_geometry(p, W, H, F, nparts);
 	StepConst sc = make_const(p, W, H, F, 1, use_flow, geo.til);
-	if (!(sc.P == 64 && sc.npmax <= 32 && ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1))))
-		throw ArgError("learned pass: 4x4x4 or 8x8x1 patches and at most 32 neighbours", BM4DOF_EUNSUPP);
+	if (!(sc.P == 64 && sc.npmax < 64 && ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1))))
+		throw ArgError("learned pass: 4x4x4 or 8x8x1 patches and fewer than 64 similar patches", BM4DOF_EUNSUPP);
 	const int G = (int)geo.gpos.size();
 	const int nslot = (G + stride - 1) / stride;
 	const int64_t rows = (int64_t)nslot * 32;
 	if (rows > max_rows) throw ArgError("output holds fewer rows than the sampled groups need");
 	if (G == 0) return 0;
 	float *d_basis = ctx->basis.as<float>(hb.size());
+	float *d_binv = ctx->binv.as<float>(hbi.size());
+	uint16_t *d_bsp = ctx->bsplit.as<uint16_t>(4 * hb.size());
 	HIPCHK(hipMemcpyAsync(d_basis, hb.data(), hb.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+	HIPCHK(hipMemcpyAsync(d_binv, hbi.data(), hbi.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+	hipLaunchKernelGGL(split_basis_kernel, dim3(16), dim3(256), 0, ctx->stream, d_basis, d_binv, d_bsp);
 	TileBox *d_tiles = ctx->tiles.as<TileBox>(geo.til.crop.size());
 	HIPCHK(hipMemcpyAsync(d_tiles, geo.til.crop.data(), geo.til.crop.size() * sizeof(TileBox),
 	                      hipMemcpyHostToDevice, ctx->stream));
@@ -1189,7 +1202,7 @@ int64_t learned_pairs_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const float 
 	       d_idx, d_nsim);
 	const float bs = beta2 * p.sigma;
 	hipLaunchKernelGGL(sc.sPx == 4 ? learned_pairs_kernel<4> : learned_pairs_kernel<8>, dim3(nslot), dim3(64), 0,
-	                   ctx->stream, noisy, basic, d_idx, d_nsim, sc, G, stride, d_basis, p.sigma, bs * bs, xout,
+	                   ctx->stream, noisy, basic, d_idx, d_nsim, sc, G, stride, d_bsp, p.sigma, bs * bs, xout,
 	                   tout, rflag);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1359,7 +1372,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
+	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bsplit, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
 	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
 	                  &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
 	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};

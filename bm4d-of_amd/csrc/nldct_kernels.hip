@@ -1485,23 +1485,142 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 // flagged (gflag) for the plain two-step fallback.
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Patch transforms of the learned pass on the matrix cores.  The learned
+// pass has no reference order to keep, so its two dense 64 x 64 transforms
+// run as 32 x 64 x 64 products on v_mfma_f32_32x32x8_f16 with each fp32
+// operand split into fp16 hi + lo (x = hi + lo to 2^-22 relative) and three
+// products per step (hi*hi + hi*lo + lo*hi), accumulated in fp32.
+// Split tables (per plan, 64 x 64 each, fp16 bits): [0] basis hi [k][p],
+// [1] basis lo, [2] inverse basis transposed hi [p][k], [3] its lo.
+// Lane layout as the network's (mlp_dev.hpp): lane l, h = l / 32, j = l % 32.
+// ---------------------------------------------------------------------------
+
+typedef _Float16 tr_f16x4 __attribute__((ext_vector_type(4)));
+constexpr int TR_ZS = 68;   // padded LDS row stride of the inverse's operand
+
+__device__ __forceinline__ void tr_split(float4 x, tr_f16x4 &hi, tr_f16x4 &lo)
+{
+	hi[0] = (_Float16)x.x; lo[0] = (_Float16)(x.x - (float)hi[0]);
+	hi[1] = (_Float16)x.y; lo[1] = (_Float16)(x.y - (float)hi[1]);
+	hi[2] = (_Float16)x.z; lo[2] = (_Float16)(x.z - (float)hi[2]);
+	hi[3] = (_Float16)x.w; lo[3] = (_Float16)(x.w - (float)hi[3]);
+}
+
+// Power of two that brings the wave's largest |value| to [2^13, 2^14): the
+// scaled operands stay inside fp16 whatever the input range, and scaling by
+// a power of two is exact both ways.
+__device__ __forceinline__ int tr_scale_exp(const float4 (&x)[8])
+{
+	float m = 0.f;
+#pragma unroll
+	for (int s = 0; s < 8; s++)
+		m = fmaxf(m, fmaxf(fmaxf(fabsf(x[s].x), fabsf(x[s].y)), fmaxf(fabsf(x[s].z), fabsf(x[s].w))));
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+	int e;
+	(void)frexpf(m, &e);
+	return 14 - e;
+}
+
+__device__ __forceinline__ void tr_split_scaled(float4 x, int sh, tr_f16x4 &hi, tr_f16x4 &lo)
+{
+	tr_split(make_float4(ldexpf(x.x, sh), ldexpf(x.y, sh), ldexpf(x.z, sh), ldexpf(x.w, sh)), hi, lo);
+}
+
+__device__ __forceinline__ tr_f16x4 tr_load4(const uint16_t *p)
+{
+	tr_f16x4 r;
+	__builtin_memcpy(&r, p, sizeof r);
+	return r;
+}
+
+// zs[n*64 + k] = sum_p X[n][p] B[k][p] for the N patches at gidx (rows
+// n >= N zero).  One wave; every lane must call it.
+template <int PX>
+__device__ __forceinline__ void mfma_forward(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
+                                             int N, int W, int WH, const uint16_t *__restrict__ bsp,
+                                             float *__restrict__ zs)
+{
+	const int l = threadIdx.x, h = l >> 5, j = l & 31;
+	// A operand: patch j, samples p = 8s + 4h .. +3 (one row of a 4 x 4 x 4
+	// patch, half a row of an 8 x 8 x 1 patch)
+	float4 x[8];
+#pragma unroll
+	for (int s = 0; s < 8; s++) {
+		x[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+		if (j < N) {
+			const int off = PX == 4 ? (s >> 1) * WH + ((2 * s + h) & 3) * W : s * W + 4 * h;
+			__builtin_memcpy(&x[s], img + gidx[j] + off, sizeof x[s]);
+		}
+	}
+	const int sh = tr_scale_exp(x);
+	tr_f16x4 ah[8], al[8];
+#pragma unroll
+	for (int s = 0; s < 8; s++) tr_split_scaled(x[s], sh, ah[s], al[s]);
+#pragma unroll
+	for (int t = 0; t < 2; t++) {
+		mlp_f32x16 c = {};
+		const uint16_t *bh = bsp + (32 * t + j) * FAST_P + 4 * h, *bl = bh + FAST_P * FAST_P;
+#pragma unroll
+		for (int s = 0; s < 8; s++) {
+			const tr_f16x4 b_hi = tr_load4(bh + 8 * s), b_lo = tr_load4(bl + 8 * s);
+			c = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[s], b_hi, c, 0, 0, 0);
+			c = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[s], b_lo, c, 0, 0, 0);
+			c = __builtin_amdgcn_mfma_f32_32x32x8f16(al[s], b_hi, c, 0, 0, 0);
+		}
+#pragma unroll
+		for (int r = 0; r < 16; r++) zs[(8 * (r >> 2) + 4 * h + (r & 3)) * FAST_P + 32 * t + j] = ldexpf(c[r], -sh);
+	}
+}
+
+// est[(row0+n)*rs + ro + p] = (w * win[p]) * sum_k Z[n][k] Bi[k][p] for n < N,
+// Z from LDS rows of stride TR_ZS.  One wave; every lane must call it.
+__device__ __forceinline__ void mfma_inverse(const float *__restrict__ zp, const uint16_t *__restrict__ bsp,
+                                             int N, float w, const float *__restrict__ win, uint32_t row0,
+                                             float *__restrict__ est, int rs, int ro)
+{
+	const int l = threadIdx.x, h = l >> 5, j = l & 31;
+	float4 z[8];
+#pragma unroll
+	for (int s = 0; s < 8; s++) z[s] = *reinterpret_cast<const float4 *>(zp + j * TR_ZS + 8 * s + 4 * h);
+	const int sh = tr_scale_exp(z);
+	tr_f16x4 ah[8], al[8];
+#pragma unroll
+	for (int s = 0; s < 8; s++) tr_split_scaled(z[s], sh, ah[s], al[s]);
+#pragma unroll
+	for (int t = 0; t < 2; t++) {
+		mlp_f32x16 c = {};
+		const uint16_t *bh = bsp + 2 * FAST_P * FAST_P + (32 * t + j) * FAST_P + 4 * h;
+		const uint16_t *bl = bh + FAST_P * FAST_P;
+#pragma unroll
+		for (int s = 0; s < 8; s++) {
+			const tr_f16x4 b_hi = tr_load4(bh + 8 * s), b_lo = tr_load4(bl + 8 * s);
+			c = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[s], b_hi, c, 0, 0, 0);
+			c = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[s], b_lo, c, 0, 0, 0);
+			c = __builtin_amdgcn_mfma_f32_32x32x8f16(al[s], b_hi, c, 0, 0, 0);
+		}
+		const float ww = w * win[32 * t + j];
+#pragma unroll
+		for (int r = 0; r < 16; r++) {
+			const int n = 8 * (r >> 2) + 4 * h + (r & 3);
+			if (n < N) est[(size_t)(row0 + n) * rs + ro + 32 * t + j] = ww * ldexpf(c[r], -sh);
+		}
+	}
+}
+
 // Haar-domain coefficients of a group into the lane registers z[n] (lane k),
 // and each row's distance statistic: log1p of its energy over 64 sigma^2
 // (for a detail row, the squared differences between the patch sub-means
 // it compares; for the first row, the group mean's energy) into rstat[n].
 template <int PX>
 __device__ __forceinline__ void learned_coeffs(const float *__restrict__ img, const uint32_t *__restrict__ gidx,
-                                               int N, const StepConst &sc, const float *__restrict__ basis,
+                                               int N, const StepConst &sc, const uint16_t *__restrict__ bsp,
                                                float sigma, float *__restrict__ zs, float (&z)[FAST_NMAX],
                                                float *__restrict__ rstat)
 {
 	const int lane = threadIdx.x;
-	{
-		float brow[FAST_P];
-#pragma unroll
-		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
-		fast_forward<PX, false>(img, gidx, N, sc.W, sc.WH, brow, zs);
-	}
+	mfma_forward<PX>(img, gidx, N, sc.W, sc.WH, bsp, zs);
 	__syncthreads();
 #pragma unroll
 	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
@@ -1539,13 +1658,14 @@ __global__ __launch_bounds__(64)
 void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__restrict__ idx,
                            const uint32_t *__restrict__ nsim, const uint32_t *__restrict__ base,
                            const uint32_t *__restrict__ gtile, StepConst sc, int g0, int Gc,
-                           const float *__restrict__ basis, const float *__restrict__ binv,
-                           const float *__restrict__ win, MlpInfer net, float sigma, float gvar_thr,
+                           const uint16_t *__restrict__ bsp, const float *__restrict__ win, MlpInfer net,
+                           float sigma, float gvar_thr,
                            float *__restrict__ est, float *__restrict__ entry_w, uint32_t *__restrict__ etile,
                            uint8_t *__restrict__ early, float *__restrict__ gweight,
                            uint8_t *__restrict__ gflag, uint32_t *__restrict__ n_flag, int C, int c)
 {
-	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];   // coefficients, then gains
+	// coefficients, then gains (row stride 64), then the filtered rows (TR_ZS)
+	__shared__ __align__(16) float zs[FAST_NMAX * TR_ZS];
 	__shared__ __align__(16) uint16_t xb[32 * MLP_XS];
 	__shared__ __align__(16) uint16_t hb[32 * MLP_HS];
 	const int gl = blockIdx.x;
@@ -1559,7 +1679,7 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 
 	__shared__ float rstat[FAST_NMAX];
 	float z[FAST_NMAX];
-	learned_coeffs<PX>(img, gidx, N, sc, basis, sigma, zs, z, rstat);
+	learned_coeffs<PX>(img, gidx, N, sc, bsp, sigma, zs, z, rstat);
 	learned_features_lds(z, N, sigma, rstat, xb);
 	__syncthreads();
 	mlp_rows32(net, xb, hb, zs, FAST_P);
@@ -1580,8 +1700,9 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 	}
 	haar_lane(z, N, false);
 	__syncthreads();
+	// filtered rows for the inverse (rows n >= N are zero)
 #pragma unroll
-	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
+	for (int n = 0; n < FAST_NMAX; n++) zs[n * TR_ZS + lane] = z[n];
 	const float cnt = (float)(64 * N);
 	const float gvar = s2 / cnt - (s1 / cnt) * (s1 / cnt);
 	// one weight for all channels of a group, as the reference aggregates
@@ -1603,7 +1724,24 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
 	}
 	if (c == 0 && lane < N) entry_w[row0 + lane] = w;
 	__syncthreads();
-	fast_inverse<false>(zs, binv, N, w * win[lane], row0, est, C * FAST_P, c * FAST_P);
+	mfma_inverse(zs, bsp, N, w, win, row0, est, C * FAST_P, c * FAST_P);
+}
+
+// Split tables of the matrix-core transforms from the fp32 bases
+// (basis[k*64 + p], binv[k*64 + p]); 4096 threads.
+__global__ void split_basis_kernel(const float *__restrict__ basis, const float *__restrict__ binv,
+                                   uint16_t *__restrict__ bsp)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= FAST_P * FAST_P) return;
+	const int k = i / FAST_P, p = i - k * FAST_P;
+	const float b = basis[i], bi = binv[i];
+	const _Float16 bh = (_Float16)b, bl = (_Float16)(b - (float)bh);
+	const _Float16 ih = (_Float16)bi, il = (_Float16)(bi - (float)ih);
+	__builtin_memcpy(bsp + i, &bh, 2);
+	__builtin_memcpy(bsp + FAST_P * FAST_P + i, &bl, 2);
+	__builtin_memcpy(bsp + 2 * FAST_P * FAST_P + p * FAST_P + k, &ih, 2);
+	__builtin_memcpy(bsp + 3 * FAST_P * FAST_P + p * FAST_P + k, &il, 2);
 }
 
 // Training pairs: for every `stride`-th group, the features of its rows
@@ -1614,7 +1752,7 @@ template <int PX>
 __global__ __launch_bounds__(64)
 void learned_pairs_kernel(const float *__restrict__ noisy, const float *__restrict__ basic,
                           const uint32_t *__restrict__ idx, const uint32_t *__restrict__ nsim, StepConst sc,
-                          int G, int stride, const float *__restrict__ basis, float sigma, float beta_sigma2,
+                          int G, int stride, const uint16_t *__restrict__ bsp, float sigma, float beta_sigma2,
                           float *__restrict__ xout, float *__restrict__ tout, uint8_t *__restrict__ rflag)
 {
 	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
@@ -1626,7 +1764,7 @@ void learned_pairs_kernel(const float *__restrict__ noisy, const float *__restri
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
 	__shared__ float rstat[FAST_NMAX];
 	float z[FAST_NMAX];
-	learned_coeffs<PX>(noisy, gidx, N, sc, basis, sigma, zs, z, rstat);
+	learned_coeffs<PX>(noisy, gidx, N, sc, bsp, sigma, zs, z, rstat);
 	const float is = 1.f / sigma, ls = logf(sigma / 20.f);
 	const size_t r0 = (size_t)slot * FAST_NMAX;
 #pragma unroll
@@ -1638,7 +1776,7 @@ void learned_pairs_kernel(const float *__restrict__ noisy, const float *__restri
 	}
 	__syncthreads();
 	float y[FAST_NMAX];
-	learned_coeffs<PX>(basic, gidx, N, sc, basis, sigma, zs, y, rstat);
+	learned_coeffs<PX>(basic, gidx, N, sc, bsp, sigma, zs, y, rstat);
 #pragma unroll
 	for (int n = 0; n < FAST_NMAX; n++) {
 		const float wgt = y[n] * y[n] / (y[n] * y[n] + beta_sigma2);
@@ -1689,10 +1827,10 @@ __global__ void fallback_blend_kernel(const uint8_t *__restrict__ mask, const fl
 }
 
 #define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
-	StepConst, int, int, const float *, const float *, const float *, MlpInfer, float, float, float *, \
+	StepConst, int, int, const uint16_t *, const float *, MlpInfer, float, float, float *, \
 	float *, uint32_t *, uint8_t *, float *, uint8_t *, uint32_t *, int, int
 #define PAIRS_ARGS const float *, const float *, const uint32_t *, const uint32_t *, StepConst, int, int, \
-	const float *, float, float, float *, float *, uint8_t *
+	const uint16_t *, float, float, float *, float *, uint8_t *
 template __global__ void filter_learned_kernel<4>(LEARNED_ARGS);
 template __global__ void filter_learned_kernel<8>(LEARNED_ARGS);
 template __global__ void learned_pairs_kernel<4>(PAIRS_ARGS);
