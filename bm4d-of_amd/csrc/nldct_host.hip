@@ -24,7 +24,7 @@ __global__ void bm_window_kernel(const float *, const float *, const uint32_t *,
                                  const TileBox *, StepConst, int, int4 *, int2 *);
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
-template <int PX, int PT> __global__ void bm_reg_kernel(const float *, const int4 *, const int2 *,
+template <int PX, int PT, bool MC> __global__ void bm_reg_kernel(const float *, const int4 *, const int2 *,
                                                         const uint32_t *, const uint32_t *, const TileBox *,
                                                         StepConst, int, int, uint32_t *, uint32_t *, int);
 template <int PX, int PT, int NF> __global__ void bm_fast_kernel(BMF_ARGS);
@@ -81,7 +81,7 @@ template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
 #define AGG2_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, \
 	StepConst, int, int, float *, float *, uint32_t *
 __global__ void pack_rows_kernel(const uint32_t *, const uint32_t *, const float *, int64_t, uint4 *);
-template <int CC, int CAP> __global__ void agg_chunk_kernel(AGG2_ARGS);
+template <int CC, int CAP, int WP> __global__ void agg_chunk_kernel(AGG2_ARGS);
 __global__ void band_bounds_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
 __global__ void normalize_kernel(const float *, const float *, const float *, StepConst,
                                  float *);
@@ -515,9 +515,11 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 		HIPCHK(hipGetLastError());
 		// row chunks per candidate column: enough work items for a wave
 		const int rchunks = std::max(1, std::min(sc.sWx, 64 / std::max(1, sWt * sc.sWx)));
-		auto kern = PX == 4 ? bm_reg_kernel<4, 4> : bm_reg_kernel<8, 1>;
+		const int nch = step == 1 ? 1 : sc.C;
+		auto kern = nch == 1 ? (PX == 4 ? bm_reg_kernel<4, 4, false> : bm_reg_kernel<8, 1, false>)
+		                     : (PX == 4 ? bm_reg_kernel<4, 4, true> : bm_reg_kernel<8, 1, true>);
 		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_reg, ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile,
-		                   d_tiles, sc, rchunks, step == 1 ? 1 : sc.C, d_idx, d_nsim, G);
+		                   d_tiles, sc, rchunks, nch, d_idx, d_nsim, G);
 		HIPCHK(hipGetLastError());
 		return;
 	}
@@ -594,13 +596,17 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 		                   d_erec);
 		HIPCHK(hipGetLastError());
 		const dim3 cgrid((unsigned)(((size_t)(t1c - t0c) * ((sc.W + 7) / 8) * ((sc.H + 7) / 8) + 7) / 8 * 8));
-		// rows held in LDS per e-range: 512 / 768 / 1024 / 1536 (one channel)
-		auto kern = agg_chunk_kernel<1, 1024>;
-		if (sc.C == 1 && ctx->agg_cap == 512) kern = agg_chunk_kernel<1, 512>;
-		else if (sc.C == 1 && ctx->agg_cap == 768) kern = agg_chunk_kernel<1, 768>;
-		else if (sc.C == 1 && ctx->agg_cap == 1536) kern = agg_chunk_kernel<1, 1536>;
-		else if (sc.C == 3) kern = agg_chunk_kernel<3, 1024>;
-		else if (sc.C == 4) kern = agg_chunk_kernel<4, 1024>;
+		// rows held in LDS per e-range: 512 / 768 / 1024 / 1536 (one channel,
+		// patches of up to 64 samples); window table of 64 or 256 samples
+		auto kern = agg_chunk_kernel<1, 1024, 64>;
+		if (sc.P > 64)
+			kern = sc.C == 1 ? agg_chunk_kernel<1, 1024, 256>
+			                 : sc.C == 3 ? agg_chunk_kernel<3, 1024, 256> : agg_chunk_kernel<4, 1024, 256>;
+		else if (sc.C == 1 && ctx->agg_cap == 512) kern = agg_chunk_kernel<1, 512, 64>;
+		else if (sc.C == 1 && ctx->agg_cap == 768) kern = agg_chunk_kernel<1, 768, 64>;
+		else if (sc.C == 1 && ctx->agg_cap == 1536) kern = agg_chunk_kernel<1, 1536, 64>;
+		else if (sc.C == 3) kern = agg_chunk_kernel<3, 1024, 64>;
+		else if (sc.C == 4) kern = agg_chunk_kernel<4, 1024, 64>;
 		hipLaunchKernelGGL(kern, cgrid, dim3(64), 0, ctx->stream, kstart, svals, d_erec, est, win, sc, t0c,
 		                   t1c - t0c, num, wsum, d_nsplit);
 		HIPCHK(hipGetLastError());

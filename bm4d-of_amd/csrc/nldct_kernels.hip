@@ -669,7 +669,7 @@ __device__ __forceinline__ void bmr_load_row(float (&dst)[PT][PX], const float *
 		}
 }
 
-template <int PX, int PT>
+template <int PX, int PT, bool MC>
 __global__ __launch_bounds__(64)
 void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
                    const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
@@ -713,8 +713,10 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	uint32_t lmin = 0xffffffffu;
 	// channels in order (step 1: channel 0; step 2: all): a channel's sums
 	// continue the previous channel's, which the same lane stored in skey
-	for (int c = 0; c < nch; c++) {
-		const bool last = c == nch - 1;
+	// (MC: more than one channel; the one-channel form keeps no such state)
+	const int nc = MC ? nch : 1;
+	for (int c = 0; c < nc; c++) {
+		const bool last = c == nc - 1;
 		const float *imc = img + (size_t)c * sc.WH;
 		// reference patch of this channel, wave-uniform
 		float ref[PP];
@@ -750,7 +752,7 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 				for (int u = 0; u < PX; u++) {
 					if (y + u < y1) {
 						bmr_load_row<PX, PT>(w[(u + PX - 1) % PX], p + (size_t)(y + u + PX - 1) * sc.W, sc.WHC);
-						float d = c ? __uint_as_float(kout[(y + u) * nx]) : 0.f;
+						float d = MC && c ? __uint_as_float(kout[(y + u) * nx]) : 0.f;
 #pragma unroll
 						for (int ht = 0; ht < PT; ht++)
 #pragma unroll
@@ -762,7 +764,7 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 								}
 						const uint32_t kb = __float_as_uint(d);
 						kout[(y + u) * nx] = kb;
-						if (last) lmin = min(lmin, kb);
+						if (!MC || last) lmin = min(lmin, kb);
 					}
 				}
 			}
@@ -834,8 +836,10 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 
 #define BMR_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, int, uint32_t *, uint32_t *, int
-template __global__ void bm_reg_kernel<4, 4>(BMR_ARGS);
-template __global__ void bm_reg_kernel<8, 1>(BMR_ARGS);
+template __global__ void bm_reg_kernel<4, 4, false>(BMR_ARGS);
+template __global__ void bm_reg_kernel<8, 1, false>(BMR_ARGS);
+template __global__ void bm_reg_kernel<4, 4, true>(BMR_ARGS);
+template __global__ void bm_reg_kernel<8, 1, true>(BMR_ARGS);
 
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
 	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
@@ -2217,8 +2221,12 @@ __device__ __forceinline__ uint32_t span8(int a, int b)
 	return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
 }
 
-template <int CC, int CAP>
-__global__ __launch_bounds__(64)
+// CAP: rows sorted per e-range; WP: window table size (>= patch samples).
+// The LDS budget (about 15 KB at CAP 1024, WP 64) and the 168-register cap
+// keep 10 waves per CU: the kernel waits on gathers, and fewer waves (the
+// unconstrained 178-register form, or 27 KB of LDS) cost 45 % more time.
+template <int CC, int CAP, int WP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__restrict__ svals,
                       const uint4 *__restrict__ erec, const float *__restrict__ est,
                       const float *__restrict__ win, StepConst sc, int tframe0, int nframes,
@@ -2230,17 +2238,18 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	__shared__ __align__(16) uint32_t ubuf[USZ];
 	uint32_t *keysB = ubuf, *hist = ubuf + CAP;
 	float *stage = reinterpret_cast<float *>(ubuf);
-	__shared__ uint32_t pcur[AGG_MAXPOS], pend[AGG_MAXPOS];
+	__shared__ uint32_t pcur[AGG_MAXPOS];
 	__shared__ uint16_t pcnt[AGG_MAXPOS];   // rows of the current e-range (<= CAP)
-	__shared__ float swin[256];
+	__shared__ float swin[WP];
 	__shared__ uint32_t m_pos2[2][64];   // (dx0 + 16) | (dy0 + 16) << 8 | dt << 16
-	__shared__ uint32_t m_e2[2][64];
-	__shared__ float m_w2[2][64];
 	// per row of a chunk: stage index and window index of the block pixel
-	// (0, 0) (offsets of the row's patch folded in) and the weight bits
+	// (0, 0) (offsets of the row's patch folded in), the weight bits and the
+	// row index e
 	__shared__ int4 m_rec2[2][64];
-	uint32_t *m_pos = m_pos2[0], *m_e = m_e2[0];
-	float *m_w = m_w2[0];
+	// the unpipelined form (slices wider than the wave) keeps its row index
+	// and weight in the record space
+	uint32_t *m_pos = m_pos2[0], *m_e = reinterpret_cast<uint32_t *>(&m_rec2[0][0]);
+	float *m_w = reinterpret_cast<float *>(&m_rec2[1][0]);
 	const int lane = threadIdx.x;
 	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
 	const int SL = sPx * sPx;
@@ -2263,18 +2272,26 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 	const int nqx = qx1 - qx0 + 1, nqy = qy1 - qy0 + 1;
 	const int npos = nqx * nqy * (qt1 - qt0 + 1);
 
-	// position runs, total row count and e-range of the block
+	// position runs (ends in registers: position lane + 64 i), total row
+	// count and e-range of the block
+	constexpr int PPL = AGG_MAXPOS / 64;
+	uint32_t pend[PPL];
 	uint32_t ksum = 0, emin = 0xffffffffu, emax = 0u;
-	for (int p = lane; p < npos; p += 64) {
-		const int qx = qx0 + p % nqx, qy = qy0 + (p / nqx) % nqy, qt = qt0 + p / (nqx * nqy);
-		const uint32_t key = (uint32_t)qt * sc.WH + (uint32_t)qy * sc.W + qx;
-		const uint32_t a = kstart[key], b = kstart[key + 1];
-		pcur[p] = a;
-		pend[p] = b;
-		if (b > a) {
-			ksum += b - a;
-			emin = min(emin, svals[a]);
-			emax = max(emax, svals[b - 1]);
+#pragma unroll
+	for (int i = 0; i < PPL; i++) {
+		const int p = lane + 64 * i;
+		pend[i] = 0;
+		if (p < npos) {
+			const int qx = qx0 + p % nqx, qy = qy0 + (p / nqx) % nqy, qt = qt0 + p / (nqx * nqy);
+			const uint32_t key = (uint32_t)qt * sc.WH + (uint32_t)qy * sc.W + qx;
+			const uint32_t a = kstart[key], b = kstart[key + 1];
+			pcur[p] = a;
+			pend[i] = b;
+			if (b > a) {
+				ksum += b - a;
+				emin = min(emin, svals[a]);
+				emax = max(emax, svals[b - 1]);
+			}
 		}
 	}
 	ksum = wave_sum_u32(ksum);
@@ -2333,15 +2350,20 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 		uint32_t K = 0;
 		for (;;) {
 			uint32_t kl = 0;
-			for (int p = lane; p < npos; p += 64) {
-				const uint32_t a = pcur[p], b = pend[p];
-				uint32_t l = a, r = b;   // first index in [a, b) with svals >= hi
-				while (l < r) {
-					const uint32_t mid = (l + r) >> 1;
-					if (svals[mid] < hi) l = mid + 1; else r = mid;
+#pragma unroll
+			for (int i = 0; i < PPL; i++) {
+				const int p = lane + 64 * i;
+				if (p < npos) {
+					const uint32_t a = pcur[p], b = pend[i];
+					uint32_t l = a, r = b;   // first index in [a, b) with svals >= hi
+					if (hi > emax) l = b;    // the last range takes every remaining row
+					while (l < r) {
+						const uint32_t mid = (l + r) >> 1;
+						if (svals[mid] < hi) l = mid + 1; else r = mid;
+					}
+					pcnt[p] = (uint16_t)min<uint32_t>(l - a, 65535u);
+					kl += l - a;
 				}
-				pcnt[p] = (uint16_t)min<uint32_t>(l - a, 65535u);
-				kl += l - a;
 			}
 			K = wave_sum_u32(kl);
 			if (K <= CAP) break;
@@ -2408,10 +2430,8 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 						for (int i = 0; i < 4; i++) if (own_t[i] == (int)tN) om = own_m[i];
 						mrow &= om;
 						m_pos2[buf][lane] = (uint32_t)(dx0 + 16) | ((uint32_t)(dy0 + 16) << 8) | ((uint32_t)dt << 16);
-						m_e2[buf][lane] = eN;
-						m_w2[buf][lane] = wN;
 						const int off = dy0 * sPx + dx0;
-						m_rec2[buf][lane] = make_int4(lane * RS - off, dt * SL - off, (int)__float_as_uint(wN), 0);
+						m_rec2[buf][lane] = make_int4(lane * RS - off, dt * SL - off, (int)__float_as_uint(wN), (int)eN);
 					}
 					return mrow;
 				};
@@ -2424,7 +2444,8 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 						const int j = jl + i * rpp;
 						if (stage_lane && j < nbc) {
 							const int dt = (int)((m_pos2[buf][j] >> 16) & 255u);
-							sv[i] = est[((size_t)m_e2[buf][j] * CC + cl) * P + dt * SL + sl];
+							const uint32_t ej = reinterpret_cast<const uint32_t *>(&m_rec2[buf][j])[3];
+							sv[i] = est[((size_t)ej * CC + cl) * P + dt * SL + sl];
 						}
 					}
 				};
@@ -2556,12 +2577,15 @@ void agg_chunk_kernel(const uint32_t *__restrict__ kstart, const uint32_t *__res
 
 #define AGG2_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, \
 	StepConst, int, int, float *, float *, uint32_t *
-template __global__ void agg_chunk_kernel<1, 512>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<1, 768>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<1, 1024>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<1, 1536>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<3, 1024>(AGG2_ARGS);
-template __global__ void agg_chunk_kernel<4, 1024>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 512, 64>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 768, 64>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 1024, 64>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 1536, 64>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<3, 1024, 64>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<4, 1024, 64>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<1, 1024, 256>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<3, 1024, 256>(AGG2_ARGS);
+template __global__ void agg_chunk_kernel<4, 1024, 256>(AGG2_ARGS);
 
 #define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
