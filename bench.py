@@ -234,6 +234,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned single pass")
     ap.add_argument("--batch32", action="store_true",
                     help="also time the batched workload: 32 uneven gray sequences in one hipGraph")
+    ap.add_argument("--learned-eval", type=int, default=0, metavar="N",
+                    help="also evaluate the shipped network on N held-out 960x540x20 sequences")
     ap.add_argument("--no-color", action="store_true",
                     help="skip the colour 1920x1080x30 learned pass with plain fallback")
     ap.add_argument("--color-gvar", type=float, default=0.02,
@@ -393,6 +395,10 @@ def main():
         print(f"[bench] learned single pass: {lv:.2f} Mpix/s", file=sys.stderr, flush=True)
     if args.batch32:
         out["batch32"] = batch32(A, S, ctx, torch, args.nparts)
+    if args.learned_eval:
+        L = _load("bm4dof_amd.learned", os.path.join("bm4d-of_amd", "learned.py"))
+        _, out["learned_heldout"] = L.evaluate(ctx, A, S, L.load(), n=args.learned_eval,
+                                               threads=min(16, len(os.sched_getaffinity(0))))
     if not args.no_color:
         L = _load("bm4dof_amd.learned", os.path.join("bm4d-of_amd", "learned.py"))
         out["color_fallback"] = color_fallback(A, S, L, ctx, torch, args.nparts, args.color_gvar,

@@ -108,8 +108,64 @@ def train(ctx, x, t, steps=3000, batch=4096, lr=1e-3, seed=1, theta=None, log=No
     return np.array(losses)
 
 
+def evaluate(ctx, A, S, theta, n=20, size=(960, 540, 20), sigmas=(10.0, 20.0, 30.0, 40.0), seed0=900000,
+             nparts=2, threads=1, log=None):
+    """Held-out evaluation of a network: n seeded sequences (seeds from
+    seed0, outside the training range) of size W x H x F at the given noise
+    levels in turn, each with TV-L1 flows (5 scales, 10 warps, lambda 0.15)
+    and the plain parameters px = pt = 4, np 32, 11 x 11 search.  Per
+    sequence: PSNR of the plain two-step output and of the learned single
+    pass, and the device time of each (inputs resident).  Returns
+    (per-sequence list, summary)."""
+    import torch
+    W, H, F = size
+    dev = torch.device("cuda", ctx.device)
+    ctx.mlp_set(theta)
+    psnr = lambda v, c: float(10 * np.log10(255.0 ** 2 / np.mean((v.astype(np.float64) - c) ** 2)))
+    res = []
+    for i in range(n):
+        sigma = float(sigmas[i % len(sigmas)])
+        clean = S.video(W, H, F, seed=seed0 + i, threads=threads)
+        noisy = S.add_noise(clean, sigma, seed0 + 50000 + i)
+        dn = torch.from_numpy(noisy).to(dev)
+        ff = torch.empty((F, 2, H, W), dtype=torch.float32, device=dev)
+        bf = torch.empty_like(ff)
+        ctx.tvl1_sequence_dev(dn.data_ptr(), W, H, F, ff.data_ptr(), bf.data_ptr())
+        p1 = A.cli_params(1, sigma, noisy.shape, px=4, pt=4, wx=11, np_=32)
+        p2 = A.cli_params(2, sigma, noisy.shape, px=4, pt=4, wx=11, np_=32)
+        db, dfin, dl = torch.empty_like(dn), torch.empty_like(dn), torch.empty_like(dn)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.run_dev(p1, p2, dn.data_ptr(), noisy.shape, ff.data_ptr(), bf.data_ptr(), db.data_ptr(),
+                    dfin.data_ptr(), nparts=nparts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.run_learned_dev(p1, p2, dn.data_ptr(), noisy.shape, ff.data_ptr(), bf.data_ptr(), dl.data_ptr(),
+                            nparts=nparts)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        r = {"seed": seed0 + i, "sigma": sigma, "psnr_noisy": psnr(noisy, clean),
+             "psnr_plain": psnr(dfin.cpu().numpy(), clean), "psnr_learned": psnr(dl.cpu().numpy(), clean),
+             "ms_plain": (t1 - t0) * 1e3, "ms_learned": (t2 - t1) * 1e3}
+        res.append(r)
+        if log:
+            log(f"held-out {i + 1}/{n} sigma {sigma:g}: plain {r['psnr_plain']:.2f} dB, learned "
+                f"{r['psnr_learned']:.2f} dB, {r['ms_plain']:.1f} / {r['ms_learned']:.1f} ms")
+    # the first sequence also plans and allocates: speeds from the others
+    timed = res[1:] if len(res) > 1 else res
+    summary = {
+        "sequences": n, "size": f"{W}x{H}x{F}", "sigmas": list(sigmas),
+        "dpsnr_mean_db": float(np.mean([r["psnr_learned"] - r["psnr_plain"] for r in res])),
+        "dpsnr_by_sigma_db": {str(sg): float(np.mean([r["psnr_learned"] - r["psnr_plain"] for r in res
+                                                      if r["sigma"] == sg])) for sg in sigmas},
+        "speedup_mean": float(np.mean([r["ms_plain"] / r["ms_learned"] for r in timed])),
+    }
+    return res, summary
+
+
 def main():
     import importlib.util
+    import json
     import sys
     import torch
     ap = argparse.ArgumentParser(description="train the gain network of the learned single pass")
@@ -122,6 +178,10 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--stride", type=int, default=4)
     ap.add_argument("--seed0", type=int, default=1000)
+    ap.add_argument("--eval", type=int, default=0, metavar="N",
+                    help="after training (or with --steps 0, for the weights in --out), evaluate on N "
+                         "held-out 960x540x20 sequences against the plain two-step path")
+    ap.add_argument("--threads", type=int, default=1, help="threads of the scene generator")
     args = ap.parse_args()
     pkg = sys.modules.get(__package__) if __package__ else None
     if pkg is None:
@@ -136,12 +196,17 @@ def main():
     W, H, F = (int(v) for v in args.size.lower().split("x"))
     sigmas = [float(s) for s in args.sigmas.split(",")]
     ctx = pkg.Context(0)
+    if args.steps == 0:
+        res, summary = evaluate(ctx, pkg, S, load(args.out), n=args.eval, threads=args.threads,
+                                log=lambda m: print("[eval] " + m, flush=True))
+        print(json.dumps(summary), flush=True)
+        return
     xs, ts = [], []
     t0 = time.time()
     for i in range(args.sequences):
         seed = args.seed0 + i
         sigma = sigmas[i % len(sigmas)]
-        clean = S.video(W, H, F, seed=seed)
+        clean = S.video(W, H, F, seed=seed, threads=args.threads)
         noisy = S.add_noise(clean, sigma, seed + 50000)
         x, t = training_pairs(ctx, pkg, noisy, sigma, stride=args.stride)
         xs.append(x)
@@ -153,8 +218,14 @@ def main():
     del xs, ts
     losses = train(ctx, x, t, steps=args.steps, batch=args.batch, lr=args.lr,
                    log=lambda m: print("[train] " + m, flush=True))
-    save(ctx.mlp_get(), args.out)
+    theta = ctx.mlp_get()
+    save(theta, args.out)
     print(f"[train] {x.shape[0]} rows, final loss {losses[-100:].mean():.5f}, saved {args.out}", flush=True)
+    if args.eval:
+        del x, t
+        res, summary = evaluate(ctx, pkg, S, theta, n=args.eval, threads=args.threads,
+                                log=lambda m: print("[eval] " + m, flush=True))
+        print(json.dumps(summary), flush=True)
 
 
 if __name__ == "__main__":

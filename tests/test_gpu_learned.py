@@ -202,3 +202,16 @@ def test_colour_fallback_takes_the_plain_result(ctx, trained, flow):
     print("colour psnr noisy %.2f learned %.2f plain %.2f blend %.2f (flagged %.3f)" %
           (ps(noisy), ps(learned), ps(plain), ps(some), fs))
     assert ps(learned) > ps(noisy) + 5.0
+
+
+def test_held_out_evaluation_reports_both_paths(ctx, trained):
+    """learned.evaluate: PSNR of the plain two-step path and of the learned
+    pass on held-out seeded sequences, per noise level, with device times."""
+    L, theta, _, _ = trained
+    res, summary = L.evaluate(ctx, A, synth, theta, n=2, size=(96, 80, 8), sigmas=(20.0, 40.0))
+    assert len(res) == 2 and summary["sequences"] == 2
+    for r in res:
+        assert r["psnr_plain"] > r["psnr_noisy"] + 3 and r["psnr_learned"] > r["psnr_noisy"] + 3
+        assert r["ms_plain"] > 0 and r["ms_learned"] > 0
+    assert set(summary["dpsnr_by_sigma_db"]) == {"20.0", "40.0"}
+    assert np.isfinite(summary["dpsnr_mean_db"])
