@@ -543,10 +543,10 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	HIPCHK(hipGetLastError());
 }
 
-void fill_u32(bm4dof_ctx *ctx, uint32_t *p, uint32_t v, int64_t n)
+void fill_u32(bm4dof_ctx *ctx, uint32_t *p, uint32_t v, int64_t n, hipStream_t st = nullptr)
 {
 	if (n <= 0) return;
-	hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, p, v, n);
+	hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(n, 256)), dim3(256), 0, st ? st : ctx->stream, p, v, n);
 	HIPCHK(hipGetLastError());
 }
 void fill_f32(bm4dof_ctx *ctx, float *p, float v, int64_t n)
@@ -558,20 +558,27 @@ void fill_f32(bm4dof_ctx *ctx, float *p, float v, int64_t n)
 
 // out[0..n) = scan of in[0..n_in) (identity past n_in): op 0 exclusive sum,
 // op 1 inclusive maximum.  Needs ctx->scan_tmp reserved (reserve_scan).
-void reserve_scan(bm4dof_ctx *ctx, int64_t n) { (void)ctx->scan_tmp.as<uint32_t>((size_t)(n / 1024 + 2)); }
-void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n, uint32_t *out)
+void reserve_scan(bm4dof_ctx *ctx, int64_t n)
+{
+	(void)ctx->scan_tmp.as<uint32_t>((size_t)(n / 1024 + 2));
+	(void)ctx->scan_tmp2.as<uint32_t>((size_t)(n / 1024 + 2));
+}
+// On the side stream (side = true) it uses its own block-sum buffer.
+void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n, uint32_t *out,
+           bool side = false)
 {
 	if (n <= 0) return;
 	const int64_t nb = (n + 1023) / 1024;
-	uint32_t *bsum = ctx->scan_tmp.as<uint32_t>((size_t)nb + 2);
+	uint32_t *bsum = (side ? ctx->scan_tmp2 : ctx->scan_tmp).as<uint32_t>((size_t)nb + 2);
+	hipStream_t st = side ? ctx->side : ctx->stream;
 	if (op == 0) {
-		hipLaunchKernelGGL(scan_block_kernel<0>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, in, n_in, n, out, bsum);
-		hipLaunchKernelGGL(scan_top_kernel<0>, dim3(1), dim3(256), 0, ctx->stream, bsum, nb);
-		hipLaunchKernelGGL(scan_add_kernel<0>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, out, n, bsum);
+		hipLaunchKernelGGL(scan_block_kernel<0>, dim3((unsigned)nb), dim3(256), 0, st, in, n_in, n, out, bsum);
+		hipLaunchKernelGGL(scan_top_kernel<0>, dim3(1), dim3(256), 0, st, bsum, nb);
+		hipLaunchKernelGGL(scan_add_kernel<0>, dim3(nblk(n, 256)), dim3(256), 0, st, out, n, bsum);
 	} else {
-		hipLaunchKernelGGL(scan_block_kernel<1>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, in, n_in, n, out, bsum);
-		hipLaunchKernelGGL(scan_top_kernel<1>, dim3(1), dim3(256), 0, ctx->stream, bsum, nb);
-		hipLaunchKernelGGL(scan_add_kernel<1>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, out, n, bsum);
+		hipLaunchKernelGGL(scan_block_kernel<1>, dim3((unsigned)nb), dim3(256), 0, st, in, n_in, n, out, bsum);
+		hipLaunchKernelGGL(scan_top_kernel<1>, dim3(1), dim3(256), 0, st, bsum, nb);
+		hipLaunchKernelGGL(scan_add_kernel<1>, dim3(nblk(n, 256)), dim3(256), 0, st, out, n, bsum);
 	}
 	HIPCHK(hipGetLastError());
 }
@@ -849,21 +856,25 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		uint32_t *d_etile = ctx->etile.as<uint32_t>(Mub);
 		float *d_ew = ctx->entry_w.as<float>(Mub);
 		float *d_est = ctx->est.as<float>((size_t)Mub * C * sc.P);
-		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, d_idx, d_nsim,
+		// rows listed by patch position, in row order within a position: on
+		// the side stream, beside the filter (only the aggregation reads them)
+		HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
+		HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_idx, d_nsim,
 		                   d_base, np, g0, Gc, d_epos);
 		HIPCHK(hipGetLastError());
-		hipLaunchKernelGGL(add_rows_kernel, dim3(1), dim3(64), 0, ctx->stream, d_M, d_rows);
-		// rows listed by patch position, in row order within a position
-		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1);
-		hipLaunchKernelGGL(count_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, d_epos, d_M, Mub,
+		hipLaunchKernelGGL(add_rows_kernel, dim3(1), dim3(64), 0, ctx->side, d_M, d_rows);
+		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1, ctx->side);
+		hipLaunchKernelGGL(count_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_epos, d_M, Mub,
 		                   d_pcnt);
-		dscan(ctx, 0, d_pcnt, (int64_t)WHF, (int64_t)WHF + 1, d_kstart);
-		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1);
-		hipLaunchKernelGGL(scatter_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, d_epos, d_M, Mub,
+		dscan(ctx, 0, d_pcnt, (int64_t)WHF, (int64_t)WHF + 1, d_kstart, true);
+		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1, ctx->side);
+		hipLaunchKernelGGL(scatter_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_epos, d_M, Mub,
 		                   d_kstart, d_pcnt, d_svals);
-		hipLaunchKernelGGL(sort_runs_kernel, dim3(nblk((int64_t)WHF, 256)), dim3(256), 0, ctx->stream, d_kstart,
+		hipLaunchKernelGGL(sort_runs_kernel, dim3(nblk((int64_t)WHF, 256)), dim3(256), 0, ctx->side, d_kstart,
 		                   (int64_t)WHF, d_svals);
 		HIPCHK(hipGetLastError());
+		HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
 		mark("l_sort", "s1_sort", "s2_sort");
 
 		if (la)
@@ -906,6 +917,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 			HIPCHK(hipGetLastError());
 		}
 		mark("l_filter", "s1_filter", "s2_filter");
+		HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
 
 		std::vector<uint32_t> band_groups;
 		for (uint32_t gf : pl.geo.frame_first)
@@ -1354,6 +1366,9 @@ int bm4dof_ctx_create(bm4dof_ctx **out, int device)
 		if (device < 0 || device >= n) throw ArgError("no HIP device " + std::to_string(device));
 		HIPCHK(hipSetDevice(device));
 		HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+		HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+		HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+		HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
 	});
 	if (rc) {
 		g_global_err = ctx->err;
@@ -1370,6 +1385,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	if (!ctx) return;
 	(void)hipSetDevice(ctx->device);
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+	if (ctx->side) (void)hipStreamSynchronize(ctx->side);
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
 	ctx->plan_cache.reset();
 	DevBuf *bufs[] = {&ctx->gpos, &ctx->gtile, &ctx->tiles, &ctx->idx, &ctx->nsim, &ctx->base,
@@ -1380,12 +1396,15 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
 	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bsplit, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
 	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
-	                  &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
+	                  &ctx->scan_tmp2, &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
 	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};
 	for (DevBuf *b : bufs) b->release();
 	DevBuf *tvb[] = {&ctx->tv_frames, &ctx->tv_pairs, &ctx->tv_part, &ctx->tv_mm, &ctx->tv_state,
 	                 &ctx->tv_work, &ctx->tv_pyr, &ctx->tv_taps};
 	for (DevBuf *b : tvb) b->release();
+	if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+	if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+	if (ctx->side) (void)hipStreamDestroy(ctx->side);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 }
