@@ -1290,6 +1290,74 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 	}
 }
 
+// The inverse over the columns k set in kmask only, for groups whose other
+// columns are zero in every row (step 1 after the hard threshold): a
+// skipped term is (+0) * Bi = +-0, and adding +-0 to a sequential sum that
+// starts at +0 leaves it unchanged bit for bit (it never becomes -0), so the
+// result equals fast_inverse's.  Lane k holds z[0..N) of column k on entry
+// (registers); the compacted rows are written over zs.
+__device__ __forceinline__ void fast_inverse_sparse(const float (&z)[FAST_NMAX], float *__restrict__ zs,
+                                                    const float *__restrict__ binv, int N, float ww,
+                                                    uint32_t row0, float *__restrict__ est, uint64_t kmask,
+                                                    int rs = FAST_P, int ro = 0)
+{
+	const int lane = threadIdx.x;
+	const int kc = __popcll(kmask);
+	// compacted column index of this lane's column
+	if ((kmask >> lane) & 1ull) {
+		const int j = __popcll(kmask & ((1ull << lane) - 1ull));
+#pragma unroll
+		for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + j] = z[n];
+	}
+	// inverse basis rows of the kept columns, in column order
+	float bsel[FAST_P];
+	uint64_t m = kmask;
+#pragma unroll
+	for (int j = 0; j < FAST_P; j++) {
+		bsel[j] = 0.f;
+		if (j < kc) {
+			const int k = __ffsll((unsigned long long)m) - 1;
+			m &= m - 1ull;
+			bsel[j] = binv[k * FAST_P + lane];
+		}
+	}
+	__syncthreads();
+	int n = 0;
+	for (; n + 1 < N; n += 2) {
+		const float *z0 = zs + n * FAST_P, *z1 = zs + (n + 1) * FAST_P;
+		float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+		for (int j = 0; j < FAST_P; j += 4) {
+			if (j < kc) {
+				const float4 a = *reinterpret_cast<const float4 *>(z0 + j);
+				const float4 b = *reinterpret_cast<const float4 *>(z1 + j);
+				acc0 = acc0 + a.x * bsel[j];
+				acc1 = acc1 + b.x * bsel[j];
+				if (j + 1 < kc) { acc0 = acc0 + a.y * bsel[j + 1]; acc1 = acc1 + b.y * bsel[j + 1]; }
+				if (j + 2 < kc) { acc0 = acc0 + a.z * bsel[j + 2]; acc1 = acc1 + b.z * bsel[j + 2]; }
+				if (j + 3 < kc) { acc0 = acc0 + a.w * bsel[j + 3]; acc1 = acc1 + b.w * bsel[j + 3]; }
+			}
+		}
+		est[(size_t)(row0 + n) * rs + ro + lane] = ww * acc0;
+		est[(size_t)(row0 + n + 1) * rs + ro + lane] = ww * acc1;
+	}
+	for (; n < N; n++) {
+		const float *z0 = zs + n * FAST_P;
+		float acc = 0.f;
+#pragma unroll
+		for (int j = 0; j < FAST_P; j += 4) {
+			if (j < kc) {
+				const float4 a = *reinterpret_cast<const float4 *>(z0 + j);
+				acc = acc + a.x * bsel[j];
+				if (j + 1 < kc) acc = acc + a.y * bsel[j + 1];
+				if (j + 2 < kc) acc = acc + a.z * bsel[j + 2];
+				if (j + 3 < kc) acc = acc + a.w * bsel[j + 3];
+			}
+		}
+		est[(size_t)(row0 + n) * rs + ro + lane] = ww * acc;
+	}
+}
+
 // Equality early-out (nldct.cpp:1986-1999, 2222-2235) with lane = sample:
 // true when N > 1 and, in every one of the C channel planes (cstride
 // apart), every patch equals patch 0; such a group is written raw (rows of
@@ -1379,22 +1447,25 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 		}
 	}
 	haar_lane(z, N, false);
+	// columns with a coefficient left in any row (the others are +0 throughout)
+	bool nzc = false;
 #pragma unroll
-	for (int n = 0; n < FAST_NMAX; n++) if (n < N) zs[n * FAST_P + lane] = z[n];
+	for (int n = 0; n < FAST_NMAX; n++) if (n < N) nzc |= z[n] != 0.f;
+	const uint64_t kmask = __ballot(nzc);
 	const int nnz_i = wave_sum_int(cnt) + (c ? (int)gacc[g] : 0);
-	if (c < C - 1) {
-		if (lane == 0) { gacc[g] = (float)nnz_i; early[g] = 0; }
-		__syncthreads();
-		fast_inverse(zs, binv, N, 1.f, row0, est, C * FAST_P, c * FAST_P);
-		return;
-	}
 	const float nnz = (float)nnz_i;
 	const float w = 1 / sc.sigma2 / nnz;
-	if (lane == 0) { gweight[g] = w; early[g] = 0; }
-	if (lane < N) entry_w[row0 + lane] = w;
-	__syncthreads();
-	fast_inverse(zs, binv, N, w * win[lane], row0, est, C * FAST_P, c * FAST_P);
-	if (C > 1) fast_scale_channels(N, C, w * win[lane], row0, est);
+	const bool last = c == C - 1;
+	if (lane == 0) {
+		if (!last) gacc[g] = (float)nnz_i;
+		else gweight[g] = w;
+		early[g] = 0;
+	}
+	if (last && lane < N) entry_w[row0 + lane] = w;
+	// launches before the last store unweighted rows
+	const float ww = last ? w * win[lane] : 1.f;
+	fast_inverse_sparse(z, zs, binv, N, ww, row0, est, kmask, C * FAST_P, c * FAST_P);
+	if (last && C > 1) fast_scale_channels(N, C, w * win[lane], row0, est);
 }
 
 template <int PX>
