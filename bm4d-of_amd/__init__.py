@@ -283,9 +283,9 @@ class Context:
         self._check(self._L.bm4dof_ctx_set_option(self.h, key.encode(), int(value)))
 
     def timings(self):
-        ms = (C.c_float * 64)()
-        names = (C.c_char_p * 64)()
-        n = self._L.bm4dof_last_timings(self.h, ms, names, 64)
+        ms = (C.c_float * 4096)()
+        names = (C.c_char_p * 4096)()
+        n = self._L.bm4dof_last_timings(self.h, ms, names, 4096)
         return [(names[i].decode(), ms[i]) for i in range(n)]
 
     def stats(self):

@@ -104,14 +104,14 @@ const char *bm4dof_global_error(void);
  * execution is fixed).  They select between kernels that compute the same
  * result; outputs do not depend on them.  Keys:
  *   "fast_filter"  1 (default): register-resident group filter where the
- *                  group shape allows it (one channel, 64-sample patches,
- *                  at most 32 neighbours, no orderInvariance); 0: the
- *                  general LDS filter for every group.
- *   "fast_bm"      block matching for patches 4x4x4 or 8x8x1 with one
- *                  channel in the distance: 2 (default): one wave per group,
- *                  candidate windows in registers; 1: one wave per group on
- *                  LDS-staged search windows; 0: the general kernel for
- *                  every shape.
+ *                  group shape allows it (64-sample patches, fewer than 64
+ *                  similar patches, no orderInvariance; colour one launch
+ *                  per channel); 0: the general LDS filter for every group.
+ *   "fast_bm"      block matching for patches 4x4x4 or 8x8x1: 2 (default):
+ *                  one wave per group, candidate windows in registers (any
+ *                  channel count); 1: one wave per group on LDS-staged
+ *                  search windows (one channel in the distance); 0: the
+ *                  general kernel for every shape.
  *   "bm_frames"    frames of search window staged in LDS per round of the
  *                  LDS-staged block matching: 1, 2 (default) or 4.
  *   "agg_chunked"  1 (default): ordered aggregation by sorted row chunks
