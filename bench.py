@@ -81,9 +81,22 @@ def _traffic(kernel):
     return None
 
 
-def cpu_baseline(noisy, ff, bf, crop, threads):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(noisy, clean, ff, bf, crop, threads, gpu_run):
     """Time the oracle (C restatement of nldct.cpp, scalar per thread) on a
-    crop of the workload; returns the bench's cpu_baseline object."""
+    crop of the workload; returns the bench's cpu_baseline object and the
+    comparison of the GPU output on the same crop and tiling (gpu_run) with
+    the oracle's: max-abs difference and PSNR difference of the final
+    estimates."""
     import numpy as np
     sys.path.insert(0, REPO)
     import oracle as O
@@ -97,12 +110,20 @@ def cpu_baseline(noisy, ff, bf, crop, threads):
     o2 = O.cli_params(2, SIGMA, F, px=4, pt=4, wx=11, np_=32)
     O._omp_threads(threads)
     t0 = time.perf_counter()
-    O.run(o1, o2, n, f1, b1, nparts=2 * threads)
+    _, cpu_final, _ = O.run(o1, o2, n, f1, b1, nparts=2 * threads)
     dt = time.perf_counter() - t0
+    gpu_final = gpu_run(n, f1, b1, 2 * threads)
+    c = clean[sl]
+    parity = {"sample": f"{cw}x{ch}x{F} crop, nParts={2 * threads}",
+              "max_abs_diff": float(np.abs(gpu_final.astype(np.float64) - cpu_final).max()),
+              "identical": bool(np.array_equal(gpu_final, cpu_final)),
+              "dpsnr_db": round(_psnr(gpu_final, c) - _psnr(cpu_final, c), 6),
+              "psnr_cpu_db": round(_psnr(cpu_final, c), 3)}
     return {"value": round(cw * ch * F / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads,
             "kind": "port",
             "sample": f"{cw}x{ch}x{F} centre crop of the same video and flows, both steps, "
-                      f"{threads} OpenMP threads, nParts={2 * threads}, {dt:.1f} s"}
+                      f"{threads} OpenMP threads of {os.cpu_count()} ({_cpu_model()}), "
+                      f"nParts={2 * threads}, {dt:.1f} s"}, parity
 
 
 def batch32(A, S, ctx, torch, nparts):
@@ -403,16 +424,22 @@ def main():
         L = _load("bm4dof_amd.learned", os.path.join("bm4d-of_amd", "learned.py"))
         out["color_fallback"] = color_fallback(A, S, L, ctx, torch, args.nparts, args.color_gvar,
                                                min(16, len(os.sched_getaffinity(0))))
-    ctx.close()
     if not args.no_cpu:
         cw, ch = (int(v) for v in args.cpu_crop.lower().split("x"))
         thr = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         print(f"[bench] CPU baseline: oracle on {cw}x{ch}x{F}, {thr} threads", file=sys.stderr,
               flush=True)
-        out["cpu_baseline"] = cpu_baseline(noisy, d_ff.cpu().numpy(), d_bf.cpu().numpy(),
-                                           (cw, ch), thr)
+
+        def gpu_run(n, f1, b1, nparts):
+            q1 = A.cli_params(1, SIGMA, n.shape, px=4, pt=4, wx=11, np_=32)
+            q2 = A.cli_params(2, SIGMA, n.shape, px=4, pt=4, wx=11, np_=32)
+            return ctx.run(q1, q2, n, f1, b1, nparts=nparts)[1]
+
+        out["cpu_baseline"], out["parity_vs_cpu"] = cpu_baseline(
+            noisy, clean, d_ff.cpu().numpy(), d_bf.cpu().numpy(), (cw, ch), thr, gpu_run)
     else:
         out["cpu_baseline"] = None
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 

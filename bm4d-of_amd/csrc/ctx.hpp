@@ -68,7 +68,7 @@ struct bm4dof_ctx {
 	bm4dof::DevBuf gpos, gtile, tiles, idx, nsim, base, keys, vals, skeys, svals, sort_tmp, lstart, lend,
 	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
 	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
-	    io_final, planes, planes_b, gacc, bsplit, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx, stat64, lastf, fidx, geo, scan_tmp,
+	    io_final, planes, planes_b, gacc, bsplit, ties, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx, stat64, lastf, fidx, geo, scan_tmp,
 	    scan_tmp2;
 	// timings of the last run
 	std::vector<std::pair<const char *, float>> timings;

@@ -17,16 +17,20 @@ namespace bm4dof {
 
 // Kernels (nldct_kernels.hip).
 __global__ void color_kernel(float *, int, int, int, int, float, float, float);
-__global__ void bm_kernel(const float *, const float *, const float *, const uint32_t *,
-                          const uint32_t *, const TileBox *, StepConst, int, int, uint32_t *,
+__global__ void bm_kernel(const float *, const float *, const float *, const uint32_t *, const uint32_t *,
+                          const TileBox *, StepConst, int, int, uint32_t *, uint32_t *, int, uint32_t *,
                           uint32_t *, int);
+__global__ void bm_tie_kernel(const float *, const float *, const float *, const uint32_t *, const uint32_t *,
+                              const TileBox *, StepConst, int, int, uint32_t *, uint32_t *, int, const uint32_t *,
+                              const uint32_t *, int);
 __global__ void bm_window_kernel(const float *, const float *, const uint32_t *, const uint32_t *,
                                  const TileBox *, StepConst, int, int4 *, int2 *);
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
-	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int, uint32_t *, uint32_t *, int
 template <int PX, int PT, bool MC> __global__ void bm_reg_kernel(const float *, const int4 *, const int2 *,
                                                         const uint32_t *, const uint32_t *, const TileBox *,
-                                                        StepConst, int, int, uint32_t *, uint32_t *, int);
+                                                        StepConst, int, int, uint32_t *, uint32_t *, int,
+                                                        uint32_t *, uint32_t *, int);
 template <int PX, int PT, int NF> __global__ void bm_fast_kernel(BMF_ARGS);
 __global__ void filter1_kernel(const float *, const uint32_t *, const uint32_t *, const uint32_t *,
                                const uint32_t *, StepConst, int, int, const float *, const float *,
@@ -74,6 +78,9 @@ __global__ void early_fix_kernel(const uint8_t *, const uint32_t *, const float 
                                  const uint32_t *, const uint32_t *, const uint32_t *, int, int, const float *,
                                  int, int, float *, float *);
 __global__ void filtered_index_kernel(const uint8_t *, int, uint32_t *);
+__global__ void first_moved_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
+__global__ void drop_groups_kernel(const uint32_t *, int, uint32_t *);
+__global__ void first_index_kernel(const uint32_t *, int, int, uint32_t *);
 #define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const float *, const float *, const float *, StepConst, const uint32_t *, int, int, float *, \
 	float *, uint32_t *
@@ -487,6 +494,19 @@ void color(bm4dof_ctx *ctx, float *v, int W, int H, int F, int C, bool forward)
 	HIPCHK(hipGetLastError());
 }
 
+void fill_u32(bm4dof_ctx *ctx, uint32_t *p, uint32_t v, int64_t n, hipStream_t st = nullptr);
+
+// The listed groups matched again with the reference's selection; a fixed
+// grid that loops over the device-side count (nothing to read back).
+static void bm_ties(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, const float *ff,
+                    const float *bf, const uint32_t *d_gpos, const uint32_t *d_gtile, const TileBox *d_tiles, int G,
+                    uint32_t *d_idx, uint32_t *d_nsim, const uint32_t *d_ties, int maxCand, size_t lds)
+{
+	hipLaunchKernelGGL(bm_tie_kernel, dim3((unsigned)std::min(G, 2048)), dim3(256), lds, ctx->stream, img, ff, bf,
+	                   d_gpos, d_gtile, d_tiles, sc, step, maxCand, d_idx, d_nsim, G, d_ties, d_ties + 1, G);
+	HIPCHK(hipGetLastError());
+}
+
 // Block matching of all groups of a step.
 void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, const float *ff,
             const float *bf, const uint32_t *d_gpos, const uint32_t *d_gtile, const TileBox *d_tiles,
@@ -497,6 +517,11 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	const size_t lds = (size_t)maxCand * 12 + (size_t)sc.C * sc.P * 4;
 	if (lds > 60 * 1024) throw ArgError("search region too large for the block-matching kernel", BM4DOF_EUNSUPP);
 	if (G == 0) return;
+	// groups whose list depends on the order of equal distances: counted in
+	// ties[0], listed in ties[1..G], then matched again in the reference's
+	// own order (bm_tie_kernel)
+	uint32_t *d_ties = ctx->ties.as<uint32_t>((size_t)G + 1);
+	fill_u32(ctx, d_ties, 0u, 1);
 	// one-wave kernels, patches 4x4x4 or 8x8x1: step 1 matches on channel 0,
 	// step 2 sums all channels (the register kernel takes any channel count,
 	// the LDS-staged one a single channel)
@@ -519,8 +544,9 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 		auto kern = nch == 1 ? (PX == 4 ? bm_reg_kernel<4, 4, false> : bm_reg_kernel<8, 1, false>)
 		                     : (PX == 4 ? bm_reg_kernel<4, 4, true> : bm_reg_kernel<8, 1, true>);
 		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_reg, ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile,
-		                   d_tiles, sc, rchunks, nch, d_idx, d_nsim, G);
+		                   d_tiles, sc, rchunks, nch, d_idx, d_nsim, G, d_ties, d_ties + 1, G);
 		HIPCHK(hipGetLastError());
+		bm_ties(ctx, sc, step, img, ff, bf, d_gpos, d_gtile, d_tiles, G, d_idx, d_nsim, d_ties, maxCand, lds);
 		return;
 	}
 	if (ctx->fast_bm && shape_ok && (step == 1 || sc.C == 1) && lds_fast <= 56 * 1024) {
@@ -534,16 +560,18 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 		else if (NF == 4) kern = PX == 4 ? bm_fast_kernel<4, 4, 4> : bm_fast_kernel<8, 1, 4>;
 		hipLaunchKernelGGL(kern, dim3(G), dim3(64), lds_fast,
 		                   ctx->stream, img, d_wins, d_wmeta, d_gpos, d_gtile, d_tiles, sc, (int)stage_floats,
-		                   d_idx, d_nsim, G);
+		                   d_idx, d_nsim, G, d_ties, d_ties + 1, G);
 		HIPCHK(hipGetLastError());
+		bm_ties(ctx, sc, step, img, ff, bf, d_gpos, d_gtile, d_tiles, G, d_idx, d_nsim, d_ties, maxCand, lds);
 		return;
 	}
 	hipLaunchKernelGGL(bm_kernel, dim3(G), dim3(256), lds, ctx->stream, img, ff, bf, d_gpos, d_gtile,
-	                   d_tiles, sc, step, maxCand, d_idx, d_nsim, G);
+	                   d_tiles, sc, step, maxCand, d_idx, d_nsim, G, d_ties, d_ties + 1, G);
 	HIPCHK(hipGetLastError());
+	bm_ties(ctx, sc, step, img, ff, bf, d_gpos, d_gtile, d_tiles, G, d_idx, d_nsim, d_ties, maxCand, lds);
 }
 
-void fill_u32(bm4dof_ctx *ctx, uint32_t *p, uint32_t v, int64_t n, hipStream_t st = nullptr)
+void fill_u32(bm4dof_ctx *ctx, uint32_t *p, uint32_t v, int64_t n, hipStream_t st)
 {
 	if (n <= 0) return;
 	hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(n, 256)), dim3(256), 0, st ? st : ctx->stream, p, v, n);
@@ -675,7 +703,7 @@ struct StepPlan {
 };
 
 StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C, bool use_flow, int nparts,
-                   bool learned)
+                   bool learned, bool full_grid = false)
 {
 	StepPlan pl;
 	pl.p = p;
@@ -685,7 +713,7 @@ StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F,
 	pl.learned = learned;
 	check_step_params(p, (unsigned)C);
 	build_bases(p, pl.hb, pl.hbi, pl.hw);
-	pl.geo = make_geometry(p, W, H, F, nparts, C);
+	pl.geo = make_geometry(p, W, H, F, nparts, C, !full_grid);
 	pl.sc = make_const(p, W, H, F, C, use_flow, pl.geo.til);
 	pl.G = (int)pl.geo.gpos.size();
 	pl.np = pl.sc.npmax;
@@ -781,6 +809,7 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 	(void)ctx->lastf.as<uint32_t>(G);
 	(void)ctx->fidx.as<uint32_t>(G);
 	(void)ctx->bm_wins.as<int4>(G * sWt);
+	(void)ctx->ties.as<uint32_t>(G + 1);
 	(void)ctx->bm_wmeta.as<int2>(G);
 	if (pl.learned) {
 		(void)ctx->gflag.as<uint8_t>(G);
@@ -796,9 +825,11 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 
 // Launches one step on ctx->stream.  With `tm` it records stage events; a
 // captured enqueue passes none.  Needs reserve_step(plan) first.
+// parts: 1 set-up and block matching, 2 the rest (filter, aggregation,
+// normalisation), 3 both.
 void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const float *noisy_w,
                   const float *basic_w, const float *ff, const float *bf, float *out_w, Timer *tm,
-                  const LearnedArgs *la)
+                  const LearnedArgs *la, int parts = 3)
 {
 	const bool step1 = pl.step1;
 	const StepConst &sc = pl.sc;
@@ -817,14 +848,16 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	float *d_wsum = ctx->wsum.as<float>(WHF);
 	uint32_t *d_kstart = ctx->lstart.as<uint32_t>(WHF + 1);
 	uint32_t *d_pcnt = ctx->lend.as<uint32_t>(WHF + 1);
-	fill_u32(ctx, d_cnt, 0u, 4);   // early-out count (slots 4.. are run totals)
-	fill_f32(ctx, d_num, 0.f, (int64_t)N_all);
-	fill_f32(ctx, d_wsum, 0.f, (int64_t)WHF);
-	mark("l_setup", "s1_setup", "s2_setup");
-
-	run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, dv.gpos, dv.gtile, dv.tiles, G, d_idx,
-	       d_nsim);
-	mark("l_bm", "s1_bm", "s2_bm");
+	if (parts & 1) {
+		fill_u32(ctx, d_cnt, 0u, 4);   // early-out count (slots 4.. are run totals)
+		fill_f32(ctx, d_num, 0.f, (int64_t)N_all);
+		fill_f32(ctx, d_wsum, 0.f, (int64_t)WHF);
+		mark("l_setup", "s1_setup", "s2_setup");
+		run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, dv.gpos, dv.gtile, dv.tiles, G, d_idx,
+		       d_nsim);
+		mark("l_bm", "s1_bm", "s2_bm");
+	}
+	if (!(parts & 2)) return;
 	uint8_t *d_gflag = la ? ctx->gflag.as<uint8_t>(std::max(G, 1)) : nullptr;
 	// the register-resident filters read one-channel planes
 	const float *d_nplanes = noisy_w, *d_bplanes = basic_w;
@@ -957,20 +990,21 @@ struct PlanCache {
 constexpr size_t kPlanCacheSize = 4;
 
 std::string plan_key(const bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C, bool use_flow,
-                     int nparts, bool learned)
+                     int nparts, bool learned, bool full_grid)
 {
-	const long long f[] = {W, H, F, C, use_flow, nparts, learned, ctx->fast_filter, (long long)ctx->est_budget};
+	const long long f[] = {W, H, F, C, use_flow, nparts, learned, ctx->fast_filter, (long long)ctx->est_budget,
+	                       full_grid};
 	std::string k(reinterpret_cast<const char *>(&p), sizeof p);
 	k.append(reinterpret_cast<const char *>(f), sizeof f);
 	return k;
 }
 
 const CachedPlan &cached_plan(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, int C,
-                              bool use_flow, int nparts, bool learned)
+                              bool use_flow, int nparts, bool learned, bool full_grid = false)
 {
 	if (!ctx->plan_cache) ctx->plan_cache = std::make_shared<PlanCache>();
 	auto &v = static_cast<PlanCache *>(ctx->plan_cache.get())->v;
-	const std::string key = plan_key(ctx, p, W, H, F, C, use_flow, nparts, learned);
+	const std::string key = plan_key(ctx, p, W, H, F, C, use_flow, nparts, learned, full_grid);
 	for (size_t i = 0; i < v.size(); i++)
 		if (v[i]->key == key) {
 			std::rotate(v.begin(), v.begin() + i, v.begin() + i + 1);
@@ -978,7 +1012,7 @@ const CachedPlan &cached_plan(bm4dof_ctx *ctx, const bm4dof_params &p, int W, in
 		}
 	auto e = std::make_unique<CachedPlan>();
 	e->key = key;
-	e->pl = plan_step(ctx, p, W, H, F, C, use_flow, nparts, learned);
+	e->pl = plan_step(ctx, p, W, H, F, C, use_flow, nparts, learned, full_grid);
 	e->dv = upload_plan(ctx, e->pl, e->geo);
 	if (v.size() >= kPlanCacheSize) {
 		HIPCHK(hipStreamSynchronize(ctx->stream));   // the evicted plan may still be in use
@@ -986,6 +1020,59 @@ const CachedPlan &cached_plan(bm4dof_ctx *ctx, const bm4dof_params &p, int W, in
 	}
 	v.insert(v.begin(), std::move(e));
 	return *v[0];
+}
+
+// The reference's scan of a step (processNlBayes, nldct.cpp:753-790): per
+// tile, grid positions in raster order are processed while their mask is
+// set, and each processed group clears the mask at its first neighbour's
+// index taken in the C-channel layout (nldct.cpp:2325-2329; for C > 1 that
+// index points to frame C*t of the one-channel mask, and past the mask it is
+// dropped, as in make_geometry).  Groups the reference skips get no rows;
+// per_tile receives the processed count of every tile.  Synchronous.
+void replay_grid(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, std::vector<uint32_t> &per_tile)
+{
+	const int G = pl.G, np = pl.np;
+	uint32_t *d_first = ctx->fidx.as<uint32_t>((size_t)G);
+	hipLaunchKernelGGL(first_index_kernel, dim3(nblk(G, 256)), dim3(256), 0, ctx->stream, ctx->idx.as<uint32_t>(1),
+	                   np, G, d_first);
+	HIPCHK(hipGetLastError());
+	std::vector<uint32_t> first((size_t)G);
+	HIPCHK(hipMemcpyAsync(first.data(), d_first, (size_t)G * 4, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	const uint32_t W = (uint32_t)pl.W, WH = (uint32_t)pl.W * (uint32_t)pl.H;
+	const int C = pl.C, nparts = (int)pl.geo.til.crop.size();
+	std::vector<uint32_t> drop;
+	std::vector<uint8_t> mask;
+	per_tile.assign(nparts, 0);
+	for (int n = 0; n < nparts; n++) {
+		const TileBox &c = pl.geo.til.crop[n];
+		const size_t wc = (size_t)(c.ex - c.ox), hc = (size_t)(c.ey - c.oy), whc = wc * hc, whf = whc * pl.F;
+		const uint32_t g0 = pl.geo.tile_first[n];
+		const uint32_t g1 = n + 1 < nparts ? pl.geo.tile_first[n + 1] : (uint32_t)G;
+		auto local = [&](uint32_t q, size_t cs) {   // tile-local index of a position, channel stride cs
+			const uint32_t t = q / WH, r = q - t * WH, y = r / W, x = r - y * W;
+			return (size_t)t * cs + (size_t)(y - (uint32_t)c.oy) * wc + (size_t)(x - (uint32_t)c.ox);
+		};
+		mask.assign(whf, 0);
+		for (uint32_t g = g0; g < g1; g++) mask[local(pl.geo.gpos[g], whc)] = 1;
+		for (uint32_t g = g0; g < g1; g++) {
+			if (!mask[local(pl.geo.gpos[g], whc)]) {
+				drop.push_back(g);
+				continue;
+			}
+			per_tile[n]++;
+			const size_t i0 = local(first[g], whc * (size_t)C);
+			if (i0 < whf) mask[i0] = 0;
+		}
+	}
+	if (!drop.empty()) {
+		uint32_t *d_drop = ctx->lastf.as<uint32_t>(drop.size());
+		HIPCHK(hipMemcpyAsync(d_drop, drop.data(), drop.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+		hipLaunchKernelGGL(drop_groups_kernel, dim3(nblk((int64_t)drop.size(), 256)), dim3(256), 0, ctx->stream,
+		                   d_drop, (int)drop.size(), ctx->nsim.as<uint32_t>(1));
+		HIPCHK(hipGetLastError());
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+	}
 }
 
 // One step, eager: (cached) plan, reserve and enqueue.
@@ -997,23 +1084,48 @@ uint64_t run_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F, 
                   float *out_w, int nparts, Timer &tm, const LearnedArgs *la = nullptr, float *ratio = nullptr)
 {
 	const bool use_flow = ff != nullptr && bf != nullptr;
-	const CachedPlan &cp = cached_plan(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr);
-	const StepPlan &pl = cp.pl;
-	if (ratio) {
-		const float whf = (float)((size_t)W * H * F);
-		float r = 0.f;
-		for (int t = 0; t < nparts; t++) {
-			const uint32_t g0 = pl.geo.tile_first[t];
-			const uint32_t g1 = t + 1 < nparts ? pl.geo.tile_first[t + 1] : (uint32_t)pl.geo.gpos.size();
-			r += 100.f * (float)(g1 - g0) / whf;
+	const float *ffu = use_flow ? ff : nullptr, *bfu = use_flow ? bf : nullptr;
+	const CachedPlan *cp = &cached_plan(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr);
+	reserve_step(ctx, cp->pl);
+	enqueue_step(ctx, cp->pl, cp->dv, noisy_w, basic_w, ffu, bfu, out_w, &tm, la, 1);
+	// The grid was laid out with every group's first neighbour being its own
+	// patch.  Equal distances in heap order can put another patch first; the
+	// reference then clears that position of its processing mask.  Replay
+	// its scan on the full grid when that happens.
+	std::vector<uint32_t> per_tile;
+	if (cp->pl.np > 1 && cp->pl.G > 0) {
+		uint32_t *d_moved = ctx->counters.as<uint32_t>(16) + 15;
+		fill_u32(ctx, d_moved, 0u, 1);
+		hipLaunchKernelGGL(first_moved_kernel, dim3(nblk(cp->pl.G, 256)), dim3(256), 0, ctx->stream,
+		                   ctx->idx.as<uint32_t>(1), cp->dv.gpos, cp->pl.np, cp->pl.G, d_moved);
+		HIPCHK(hipGetLastError());
+		uint32_t moved = 0;
+		HIPCHK(hipMemcpyAsync(&moved, d_moved, 4, hipMemcpyDeviceToHost, ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		if (moved > 0) {
+			if (C > 1) {
+				// the laid-out grid already left out the positions the colour
+				// clears remove: match the full grid
+				cp = &cached_plan(ctx, p, W, H, F, C, use_flow, nparts, la != nullptr, true);
+				reserve_step(ctx, cp->pl);
+				enqueue_step(ctx, cp->pl, cp->dv, noisy_w, basic_w, ffu, bfu, out_w, &tm, la, 1);
+			}
+			replay_grid(ctx, cp->pl, cp->dv, per_tile);
 		}
-		*ratio = r;
 	}
-	reserve_step(ctx, pl);
-	ctx->step_groups[pl.step1 ? 0 : 1] += (uint64_t)pl.G;
-	enqueue_step(ctx, pl, cp.dv, noisy_w, basic_w, use_flow ? ff : nullptr, use_flow ? bf : nullptr, out_w, &tm,
-	             la);
-	return (uint64_t)pl.G;
+	const StepPlan &pl = cp->pl;
+	uint64_t groups = 0;
+	if (ratio) *ratio = 0.f;
+	for (int t = 0; t < nparts; t++) {
+		const uint32_t g0 = pl.geo.tile_first[t];
+		const uint32_t g1 = t + 1 < nparts ? pl.geo.tile_first[t + 1] : (uint32_t)pl.geo.gpos.size();
+		const uint32_t n = per_tile.empty() ? g1 - g0 : per_tile[t];
+		groups += n;
+		if (ratio) *ratio += 100.f * (float)n / (float)((size_t)W * H * F);
+	}
+	ctx->step_groups[pl.step1 ? 0 : 1] += groups;
+	enqueue_step(ctx, pl, cp->dv, noisy_w, basic_w, ffu, bfu, out_w, &tm, la, 2);
+	return groups;
 }
 
 // Run statistics: zeroed on the device at a run's start, read at its end.
@@ -1394,7 +1506,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bsplit, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
+	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bsplit, &ctx->ties, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
 	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
 	                  &ctx->scan_tmp2, &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
 	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};

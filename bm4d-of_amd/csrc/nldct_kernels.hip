@@ -80,10 +80,98 @@ __global__ void color_kernel(float *__restrict__ v, int WH, int C, int F, int fo
 // pt-1.., then raster qy, qx).  Selection keeps the kk = largest power of
 // two <= min(np, #candidates) smallest distances in the total order
 // (distance, reference patch first, s), which equals the reference's
-// std::partial_sort order whenever distances are distinct, then applies the
-// power-of-two tau cut (nldct.cpp:1285-1289).
+// std::partial_sort order whenever the distances the list depends on are
+// distinct, then applies the power-of-two tau cut (nldct.cpp:1285-1289).
+// Groups where they are not (bm_has_tie) are listed and matched again with
+// the selection of the reference itself: libstdc++'s heap select and
+// sort_heap on the scan order (bm_tie_kernel).
 // ---------------------------------------------------------------------------
 struct BmFrame { int qt, rx0, ry0, nx, ny, base; };
+
+// True when the kept list could come out otherwise under std::partial_sort,
+// which leaves equal distances in heap order: two kept distances are equal,
+// or a candidate left out has the distance of the last kept one.  n is the
+// count kept after the tau cut; out_key holds the kk smallest, sorted.  One
+// wave, every lane.
+__device__ __forceinline__ bool bm_has_tie(const uint32_t *__restrict__ skey, int nsrch,
+                                           const uint32_t *__restrict__ out_key, int nSimP, float tau2)
+{
+	const int lane = threadIdx.x & 63;
+	int n = 1;
+	while (2 * n <= nSimP && __uint_as_float(out_key[n]) < tau2) n *= 2;
+	const uint32_t last = out_key[n - 1];
+	const bool eq = lane + 1 < n && out_key[lane] == out_key[lane + 1];
+	int le = 0;
+	for (int s = lane; s < nsrch; s += 64) le += skey[s] <= last ? 1 : 0;
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) le += __shfl_xor(le, o, 64);
+	return __any(eq) || le > n;
+}
+
+__device__ __forceinline__ void bm_tie_append(int g, uint32_t *__restrict__ tie_count,
+                                              uint32_t *__restrict__ tie_list, int tie_cap)
+{
+	const uint32_t i = atomicAdd(tie_count, 1u);
+	if (i < (uint32_t)tie_cap) tie_list[i] = (uint32_t)g;
+}
+
+// libstdc++'s std::partial_sort(first, first + middle, first + len) under
+// `<` on the key (__heap_select, then __sort_heap), on parallel arrays; one
+// thread.  Same steps as the library's __adjust_heap / __push_heap /
+// __make_heap / __pop_heap.
+__device__ void ls_push_heap(uint32_t *key, int *val, int hole, int top, uint32_t vk, int vv)
+{
+	int parent = (hole - 1) / 2;
+	while (hole > top && __uint_as_float(key[parent]) < __uint_as_float(vk)) {
+		key[hole] = key[parent];
+		val[hole] = val[parent];
+		hole = parent;
+		parent = (hole - 1) / 2;
+	}
+	key[hole] = vk;
+	val[hole] = vv;
+}
+__device__ void ls_adjust_heap(uint32_t *key, int *val, int hole, int len, uint32_t vk, int vv)
+{
+	const int top = hole;
+	int second = hole;
+	while (second < (len - 1) / 2) {
+		second = 2 * (second + 1);
+		if (__uint_as_float(key[second]) < __uint_as_float(key[second - 1])) second--;
+		key[hole] = key[second];
+		val[hole] = val[second];
+		hole = second;
+	}
+	if ((len & 1) == 0 && second == (len - 2) / 2) {
+		second = 2 * (second + 1);
+		key[hole] = key[second - 1];
+		val[hole] = val[second - 1];
+		hole = second - 1;
+	}
+	ls_push_heap(key, val, hole, top, vk, vv);
+}
+__device__ void ls_pop_heap(uint32_t *key, int *val, int len, int result)
+{
+	const uint32_t vk = key[result];
+	const int vv = val[result];
+	key[result] = key[0];
+	val[result] = val[0];
+	ls_adjust_heap(key, val, 0, len, vk, vv);
+}
+__device__ void ls_partial_sort(uint32_t *key, int *val, int middle, int len)
+{
+	if (middle >= 2)
+		for (int parent = (middle - 2) / 2;; parent--) {
+			ls_adjust_heap(key, val, parent, middle, key[parent], val[parent]);
+			if (parent == 0) break;
+		}
+	for (int i = middle; i < len; ++i)
+		if (__uint_as_float(key[i]) < __uint_as_float(key[0])) ls_pop_heap(key, val, middle, i);
+	for (int l = middle; l > 1;) {
+		--l;
+		ls_pop_heap(key, val, l, l);
+	}
+}
 
 __device__ __forceinline__ int wave_incl_scan(int v)
 {
@@ -96,12 +184,16 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 	return v;
 }
 
-__global__ __launch_bounds__(BM_THREADS)
-void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
-               const float *__restrict__ bflow, const uint32_t *__restrict__ gpos,
-               const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
-               StepConst sc, int step, int maxCand, uint32_t *__restrict__ out_idx,
-               uint32_t *__restrict__ out_nsim, int G)
+// One group, one workgroup.  LIBSTD: the selection is libstdc++'s
+// partial_sort on the scan order (for the listed groups of bm_tie_kernel);
+// else the total order, listing the groups bm_has_tie finds (tie_list).
+template <bool LIBSTD>
+__device__ void bm_group(int g, const float *__restrict__ img, const float *__restrict__ fflow,
+                         const float *__restrict__ bflow, const uint32_t *__restrict__ gpos,
+                         const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
+                         StepConst sc, int step, int maxCand, uint32_t *__restrict__ out_idx,
+                         uint32_t *__restrict__ out_nsim, int G, uint32_t *__restrict__ tie_count,
+                         uint32_t *__restrict__ tie_list, int tie_cap)
 {
 	extern __shared__ unsigned char smem[];
 	uint32_t *skey = (uint32_t *)smem;               // maxCand
@@ -120,7 +212,6 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 	__shared__ int out_rank[64];
 	__shared__ int s_na, s_nb, s_nfin;
 
-	const int g = blockIdx.x;
 	if (g >= G) return;
 	const int tid = threadIdx.x;
 	const uint32_t pos = gpos[g];
@@ -235,7 +326,7 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 					}
 		}
 		skey[s] = __float_as_uint(d);
-		srank[s] = (qx == px && qy == py && qt == pt) ? -1 : s;
+		srank[s] = (!LIBSTD && qx == px && qy == py && qt == pt) ? -1 : s;
 	}
 	__syncthreads();
 
@@ -243,6 +334,15 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 	int kk = 1;
 	while (kk * 2 <= nSimP) kk *= 2;
 
+	if (LIBSTD) {
+		if (tid == 0) ls_partial_sort(skey, srank, nSimP, nsrch);
+		__syncthreads();
+		if (tid < kk) {
+			out_key[tid] = skey[tid];
+			out_rank[tid] = srank[tid];
+		}
+		__syncthreads();
+	} else {
 	// Radix select of the kk-th smallest key, 8 bits per pass.
 	uint32_t prefix = 0, pmask = 0;
 	uint32_t remaining = (uint32_t)kk;
@@ -313,6 +413,11 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 		out_rank[p] = r;
 	}
 	__syncthreads();
+	if (tie_list && tid < 64) {
+		const bool tie = bm_has_tie(skey, nsrch, out_key, nSimP, sc.tau2);
+		if (tie && tid == 0) bm_tie_append(g, tie_count, tie_list, tie_cap);
+	}
+	}
 	if (tid == 0) {
 		int n = 1;
 		while (2 * n <= nSimP && __uint_as_float(out_key[n]) < sc.tau2) n *= 2;
@@ -334,6 +439,36 @@ void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
 			q = (uint32_t)f.qt * (uint32_t)sc.WH + (uint32_t)(qy + tb.oy) * (uint32_t)sc.W + (uint32_t)(qx + tb.ox);
 		}
 		oidx[tid] = q;
+	}
+}
+
+__global__ __launch_bounds__(BM_THREADS)
+void bm_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
+               const float *__restrict__ bflow, const uint32_t *__restrict__ gpos,
+               const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
+               StepConst sc, int step, int maxCand, uint32_t *__restrict__ out_idx,
+               uint32_t *__restrict__ out_nsim, int G, uint32_t *__restrict__ tie_count,
+               uint32_t *__restrict__ tie_list, int tie_cap)
+{
+	bm_group<false>((int)blockIdx.x, img, fflow, bflow, gpos, gtile, tiles, sc, step, maxCand, out_idx, out_nsim,
+	                G, tie_count, tie_list, tie_cap);
+}
+
+// The listed groups again, with the reference's own selection (their
+// earlier lists are overwritten).
+__global__ __launch_bounds__(BM_THREADS)
+void bm_tie_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
+                   const float *__restrict__ bflow, const uint32_t *__restrict__ gpos,
+                   const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
+                   StepConst sc, int step, int maxCand, uint32_t *__restrict__ out_idx,
+                   uint32_t *__restrict__ out_nsim, int G, const uint32_t *__restrict__ tie_count,
+                   const uint32_t *__restrict__ tie_list, int tie_cap)
+{
+	const uint32_t n = min(*tie_count, (uint32_t)tie_cap);
+	for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+		bm_group<true>((int)tie_list[i], img, fflow, bflow, gpos, gtile, tiles, sc, step, maxCand, out_idx,
+		               out_nsim, G, nullptr, nullptr, 0);
+		__syncthreads();
 	}
 }
 
@@ -528,7 +663,8 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
                     const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
                     const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
                     StepConst sc, int stage_floats, uint32_t *__restrict__ out_idx,
-                    uint32_t *__restrict__ out_nsim, int G)
+                    uint32_t *__restrict__ out_nsim, int G, uint32_t *__restrict__ tie_count,
+                    uint32_t *__restrict__ tie_list, int tie_cap)
 {
 	constexpr int PP = PX * PX * PT;
 	extern __shared__ unsigned char smem[];
@@ -631,6 +767,9 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 	while (kk * 2 <= nSimP) kk *= 2;
 	__syncthreads();
 	bm_radix_select(skey, stie, nsrch, kk, s_self, out_key, out_rank);
+	__syncthreads();
+	const bool tie = tie_list ? bm_has_tie(skey, nsrch, out_key, nSimP, sc.tau2) : false;
+	if (tie && threadIdx.x == 0) bm_tie_append(g, tie_count, tie_list, tie_cap);
 	bm_emit(out_key, out_rank, nSimP, sc, gw, nframes, pos, tb, oidx, out_nsim + g);
 }
 
@@ -675,7 +814,8 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
                    const int2 *__restrict__ wmeta, const uint32_t *__restrict__ gpos,
                    const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles,
                    StepConst sc, int rchunks, int nch, uint32_t *__restrict__ out_idx,
-                   uint32_t *__restrict__ out_nsim, int G)
+                   uint32_t *__restrict__ out_nsim, int G, uint32_t *__restrict__ tie_count,
+                   uint32_t *__restrict__ tie_list, int tie_cap)
 {
 	constexpr int PP = PX * PX * PT;
 	extern __shared__ unsigned char smem[];
@@ -831,18 +971,21 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	} else {
 		bm_radix_select(skey, stie, nsrch, kk, s_self, out_key, out_rank);
 	}
+	__syncthreads();
+	const bool tie = tie_list ? bm_has_tie(skey, nsrch, out_key, nSimP, sc.tau2) : false;
+	if (tie && lane == 0) bm_tie_append(g, tie_count, tie_list, tie_cap);
 	bm_emit(out_key, out_rank, nSimP, sc, gw, nframes, pos, tb, oidx, out_nsim + g);
 }
 
 #define BMR_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
-	const TileBox *, StepConst, int, int, uint32_t *, uint32_t *, int
+	const TileBox *, StepConst, int, int, uint32_t *, uint32_t *, int, uint32_t *, uint32_t *, int
 template __global__ void bm_reg_kernel<4, 4, false>(BMR_ARGS);
 template __global__ void bm_reg_kernel<8, 1, false>(BMR_ARGS);
 template __global__ void bm_reg_kernel<4, 4, true>(BMR_ARGS);
 template __global__ void bm_reg_kernel<8, 1, true>(BMR_ARGS);
 
 #define BMF_ARGS const float *, const int4 *, const int2 *, const uint32_t *, const uint32_t *, \
-	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int
+	const TileBox *, StepConst, int, uint32_t *, uint32_t *, int, uint32_t *, uint32_t *, int
 template __global__ void bm_fast_kernel<4, 4, 1>(BMF_ARGS);
 template __global__ void bm_fast_kernel<4, 4, 2>(BMF_ARGS);
 template __global__ void bm_fast_kernel<4, 4, 4>(BMF_ARGS);

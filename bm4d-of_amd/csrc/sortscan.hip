@@ -201,4 +201,28 @@ __global__ void filtered_index_kernel(const uint8_t *__restrict__ early, int n, 
 	if (g < n) v[g] = early[g] ? 0u : (uint32_t)g + 1u;
 }
 
+// Counts the groups whose first neighbour is not the reference patch itself
+// (equal distances in heap order): the reference then clears another
+// position of its processing mask (nldct.cpp:2325-2329).
+__global__ void first_moved_kernel(const uint32_t *__restrict__ idx, const uint32_t *__restrict__ gpos, int np,
+                                   int G, uint32_t *__restrict__ count)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g < G && idx[(size_t)g * np] != gpos[g]) atomicAdd(count, 1u);
+}
+
+// First neighbour of every group.
+__global__ void first_index_kernel(const uint32_t *__restrict__ idx, int np, int G, uint32_t *__restrict__ first)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g < G) first[g] = idx[(size_t)g * np];
+}
+
+// Groups the reference does not process: no rows (listed in drop[0..n)).
+__global__ void drop_groups_kernel(const uint32_t *__restrict__ drop, int n, uint32_t *__restrict__ nsim)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) nsim[drop[i]] = 0u;
+}
+
 } // namespace bm4dof

@@ -108,7 +108,7 @@ def train(ctx, x, t, steps=3000, batch=4096, lr=1e-3, seed=1, theta=None, log=No
     return np.array(losses)
 
 
-def evaluate(ctx, A, S, theta, n=20, size=(960, 540, 20), sigmas=(10.0, 20.0, 30.0, 40.0), seed0=900000,
+def evaluate(ctx, A, S, theta, n=20, size=(960, 540, 20), sigmas=(10.0, 20.0, 30.0, 40.0), seed0=9000,
              nparts=2, threads=1, log=None):
     """Held-out evaluation of a network: n seeded sequences (seeds from
     seed0, outside the training range) of size W x H x F at the given noise
