@@ -200,7 +200,11 @@ int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
  * noise level as one launch set: planned and uploaded at creation, then
  * either enqueued on every run or captured once into a hipGraph
  * (BM4DOF_BATCH_GRAPH) that every run replays.  Each sequence's output is
- * bitwise the output of bm4dof_run_dev on it alone (same nparts).  With
+ * bitwise the output of bm4dof_run_dev on it alone (same nparts), except
+ * where a run of identical patches puts a patch other than the reference
+ * one first in a neighbour list: bm4dof_run_dev then replays the
+ * reference's processing-mask scan (a host step between block matching and
+ * filtering), which a batch does not.  With
  * BM4DOF_BATCH_AUTOTUNE the execution options that cannot change results
  * ("fast_bm", "bm_frames", "agg_cap") are timed on the first sequence of every shape
  * class and the fastest set is used for that class.  Buffers are device
