@@ -91,8 +91,9 @@ struct BmFrame { int qt, rx0, ry0, nx, ny, base; };
 // True when the kept list could come out otherwise under std::partial_sort,
 // which leaves equal distances in heap order: two kept distances are equal,
 // or a candidate left out has the distance of the last kept one.  n is the
-// count kept after the tau cut; out_key holds the kk smallest, sorted.  One
-// wave, every lane.
+// count kept after the tau cut; out_key holds the kk smallest, sorted; skey
+// the nsrch distances (all of them, or every one up to a bound at least the
+// kk-th smallest).  One wave, every lane.
 __device__ __forceinline__ bool bm_has_tie(const uint32_t *__restrict__ skey, int nsrch,
                                            const uint32_t *__restrict__ out_key, int nSimP, float tau2)
 {
@@ -972,7 +973,10 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 		bm_radix_select(skey, stie, nsrch, kk, s_self, out_key, out_rank);
 	}
 	__syncthreads();
-	const bool tie = tie_list ? bm_has_tie(skey, nsrch, out_key, nSimP, sc.tau2) : false;
+	// every distance up to the bound is among the ranked candidates
+	const bool tie = tie_list ? (fits ? bm_has_tie(cand_key, cnt, out_key, nSimP, sc.tau2)
+	                                  : bm_has_tie(skey, nsrch, out_key, nSimP, sc.tau2))
+	                          : false;
 	if (tie && lane == 0) bm_tie_append(g, tie_count, tie_list, tie_cap);
 	bm_emit(out_key, out_rank, nSimP, sc, gw, nframes, pos, tb, oidx, out_nsim + g);
 }
@@ -1882,9 +1886,11 @@ void filter_learned_kernel(const float *__restrict__ img, const uint32_t *__rest
                            uint8_t *__restrict__ early, float *__restrict__ gweight,
                            uint8_t *__restrict__ gflag, uint32_t *__restrict__ n_flag, int C, int c)
 {
-	// coefficients, then gains (row stride 64), then the filtered rows (TR_ZS)
+	// coefficients, then the bf16 feature rows (read by the network's first
+	// layer only), then gains (row stride 64), then the filtered rows (TR_ZS)
 	__shared__ __align__(16) float zs[FAST_NMAX * TR_ZS];
-	__shared__ __align__(16) uint16_t xb[32 * MLP_XS];
+	uint16_t *xb = reinterpret_cast<uint16_t *>(zs);
+	static_assert(32 * MLP_XS * 2 <= FAST_NMAX * TR_ZS * 4, "feature rows fit the coefficient buffer");
 	__shared__ __align__(16) uint16_t hb[32 * MLP_HS];
 	const int gl = blockIdx.x;
 	if (gl >= Gc) return;
