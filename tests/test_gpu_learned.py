@@ -215,3 +215,21 @@ def test_held_out_evaluation_reports_both_paths(ctx, trained):
         assert r["ms_plain"] > 0 and r["ms_learned"] > 0
     assert set(summary["dpsnr_by_sigma_db"]) == {"20.0", "40.0"}
     assert np.isfinite(summary["dpsnr_mean_db"])
+
+
+def test_learned_pass_with_8x8x1_patches(ctx, trained):
+    """The learned kernel's 8 x 8 x 1 form (one frame per patch): finite,
+    well above the noisy input, and with every group flagged it gives the
+    plain two-step result bitwise."""
+    _, theta, _, _ = trained
+    ctx.mlp_set(theta)
+    clean = synth.video(72, 56, 4, seed=404)
+    noisy = synth.add_noise(clean, 25.0, 405)
+    p1 = A.cli_params(1, 25, noisy.shape, px=8, pt=1, wx=15, np_=16)
+    p2 = A.cli_params(2, 25, noisy.shape, px=8, pt=1, wx=15, np_=16)
+    out, frac = ctx.run_learned(p1, p2, noisy)
+    ps = lambda v: 10 * np.log10(255.0 ** 2 / np.mean((v.astype(np.float64) - clean) ** 2))
+    assert np.isfinite(out).all() and frac == 0.0
+    assert ps(out) > ps(noisy) + 5.0
+    allf, fa = ctx.run_learned(p1, p2, noisy, gvar_thr=0.0)
+    assert fa == 1.0 and np.array_equal(allf, ctx.run(p1, p2, noisy)[1])
