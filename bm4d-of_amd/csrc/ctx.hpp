@@ -68,12 +68,14 @@ struct bm4dof_ctx {
 	bm4dof::DevBuf gpos, gtile, tiles, idx, nsim, base, keys, vals, skeys, svals, sort_tmp, lstart, lend,
 	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
 	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
-	    io_final, planes, planes_b, gacc, bsplit, ties, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx, stat64, lastf, fidx, geo, scan_tmp,
+	    io_final, planes, planes_b, coef_n, coef_b, gacc, bsplit, ties, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx, stat64, lastf, fidx, geo, scan_tmp,
 	    scan_tmp2;
 	// timings of the last run
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
 	size_t est_budget = (size_t)24 << 30;
+	// bytes of per-anchor coefficient tables a step may hold (0: none)
+	size_t coef_budget = (size_t)16 << 30;
 	// ordered-aggregation statistics of the last run
 	uint64_t agg_blocks = 0, agg_overflow_blocks = 0;
 	int agg_region[3] = {0, 0, 0};

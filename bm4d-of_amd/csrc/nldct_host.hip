@@ -42,11 +42,13 @@ __global__ void filter2_kernel(const float *, const float *, const uint32_t *, c
                                uint32_t *, uint8_t *, float *, uint32_t *);
 #define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, StepConst, int, int, const float *, const float *, const float *, float *, \
-	float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
+	float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *, const float *
 #define FILTER2_ARGS const float *, const float *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, const uint32_t *, StepConst, int, int, const float *, const float *, \
-	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
+	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *, \
+	const float *, const float *
 template <int PX> __global__ void filter1_fast_kernel(FILTER1_ARGS);
+template <int PX> __global__ void coef_kernel(const float *, int, int, int, const float *, float *);
 template <int PX> __global__ void filter2_fast_kernel(FILTER2_ARGS);
 #define LEARNED_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	StepConst, int, int, const uint16_t *, const float *, MlpInfer, float, float, float *, \
@@ -876,6 +878,30 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		d_gacc = ctx->gacc.as<float>(std::max(G, 1));
 		HIPCHK(hipGetLastError());
 	}
+	// forward transforms of every patch anchor, once per image, when the
+	// tables fit coef_budget: the register filters then read a group row's
+	// coefficients instead of transforming the patch again in every group
+	// that holds it (bit for bit the same sums)
+	const float *d_coef_n = nullptr, *d_coef_b = nullptr;
+	if (pl.fast && !la) {
+		const int nimg = step1 ? 1 : 2;
+		const size_t per = (size_t)C * WHF * sc.P;
+		const int nx = sc.W - sc.sPx + 1, ny = sc.H - sc.sPx + 1, nt = sc.F - sc.sPt + 1;
+		const int64_t waves = nx > 0 && ny > 0 && nt > 0 ? (int64_t)((nx + 15) / 16) * ny * nt : 0;
+		if (per * sizeof(float) * nimg <= ctx->coef_budget && waves > 0 && waves * 64 < ((int64_t)1 << 32)) {
+			for (int i = 0; i < nimg; i++) {
+				float *tab = (i ? ctx->coef_b : ctx->coef_n).as<float>(per);
+				const float *src = i ? d_bplanes : d_nplanes;
+				for (int c = 0; c < C; c++)
+					hipLaunchKernelGGL(sc.sPx == 4 ? coef_kernel<4> : coef_kernel<8>, dim3((unsigned)waves), dim3(64),
+					                   0, ctx->stream, src + (size_t)c * WHF, sc.W, sc.H, sc.F, dv.basis,
+					                   tab + (size_t)c * WHF * sc.P);
+				(i ? d_coef_b : d_coef_n) = tab;
+			}
+			HIPCHK(hipGetLastError());
+		}
+		mark("l_coef", "s1_coef", "s2_coef");
+	}
 
 	for (int g0 = 0; g0 < G; g0 += pl.chunk_groups) {
 		const int Gc = std::min(pl.chunk_groups, G - g0);
@@ -921,13 +947,13 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 				hipLaunchKernelGGL(sc.sPx == 4 ? filter1_fast_kernel<4> : filter1_fast_kernel<8>, dim3(Gc),
 				                   dim3(64), 0, ctx->stream, d_nplanes, d_idx, d_nsim, d_base, dv.gtile, sc, g0,
 				                   Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early, d_gw, d_cnt, C, c,
-				                   (int64_t)WHF, d_gacc);
+				                   (int64_t)WHF, d_gacc, d_coef_n);
 		else if (pl.fast)
 			for (int c = 0; c < C; c++)
 				hipLaunchKernelGGL(sc.sPx == 4 ? filter2_fast_kernel<4> : filter2_fast_kernel<8>, dim3(Gc),
 				                   dim3(64), 0, ctx->stream, d_nplanes, d_bplanes, d_idx, d_nsim, d_base,
 				                   dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est, d_ew, d_etile, d_early,
-				                   d_gw, d_cnt, C, c, (int64_t)WHF, d_gacc);
+				                   d_gw, d_cnt, C, c, (int64_t)WHF, d_gacc, d_coef_n, d_coef_b);
 		else if (step1)
 			hipLaunchKernelGGL(filter1_kernel, dim3(Gc), dim3(256), pl.filt_lds, ctx->stream, noisy_w,
 			                   d_idx, d_nsim, d_base, dv.gtile, sc, g0, Gc, dv.basis, dv.binv, dv.win, d_est,
@@ -1506,7 +1532,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
 	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
 	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->planes, &ctx->planes_b, &ctx->gacc, &ctx->bsplit, &ctx->ties, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
+	                  &ctx->planes, &ctx->planes_b, &ctx->coef_n, &ctx->coef_b, &ctx->gacc, &ctx->bsplit, &ctx->ties, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
 	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
 	                  &ctx->scan_tmp2, &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
 	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};
@@ -1539,6 +1565,9 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		else if (k == "bm_frames") {
 			if (value != 1 && value != 2 && value != 4) throw ArgError("bm_frames must be 1, 2 or 4");
 			ctx->bm_frames = (int)value;
+		} else if (k == "coef_budget") {
+			if (value < 0) throw ArgError("coef_budget must be >= 0");
+			ctx->coef_budget = (size_t)value;
 		} else if (k == "agg_cap") {
 			if (value != 512 && value != 768 && value != 1024 && value != 1536)
 				throw ArgError("agg_cap must be 512, 768, 1024 or 1536");

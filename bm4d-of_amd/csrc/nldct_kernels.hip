@@ -1505,6 +1505,77 @@ __device__ __forceinline__ void fast_inverse_sparse(const float (&z)[FAST_NMAX],
 	}
 }
 
+// Forward transform of every patch anchor of a one-channel plane, once:
+// coef[pos*64 + k] = sum_p B[k][p] x[pos + off(p)] in the order of
+// fast_forward (the same dot product whatever group the patch is in, so the
+// filters' rows read it bit for bit).  One wave per run of anchors along x,
+// lane = coefficient, basis row in registers, two anchors per round.
+template <int PX>
+__global__ __launch_bounds__(64)
+void coef_kernel(const float *__restrict__ img, int W, int H, int F, const float *__restrict__ basis,
+                 float *__restrict__ coef)
+{
+	constexpr int PT = FAST_P / (PX * PX);
+	constexpr int RUN = 16;   // anchors per wave
+	const int nx = W - PX + 1, ny = H - PX + 1, nt = F - PT + 1;
+	const int runs_x = (nx + RUN - 1) / RUN;
+	const int64_t wid = blockIdx.x;
+	if (wid >= (int64_t)runs_x * ny * nt) return;
+	const int rx = (int)(wid % runs_x);
+	const int64_t rest = wid / runs_x;
+	const int y = (int)(rest % ny), t = (int)(rest / ny);
+	const int x0 = rx * RUN, x1 = min(x0 + RUN, nx);
+	const int lane = threadIdx.x;
+	const int64_t WH = (int64_t)W * H;
+	float brow[FAST_P];
+#pragma unroll
+	for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
+	const int64_t pos0 = (int64_t)t * WH + (int64_t)y * W;
+	int x = x0;
+	for (; x + 1 < x1; x += 2) {
+		const float *s0 = img + pos0 + x, *s1 = s0 + 1;
+		float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+		for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+			for (int hy = 0; hy < PX; hy++) {
+				const float *r0 = s0 + ht * WH + hy * W, *r1 = s1 + ht * WH + hy * W;
+#pragma unroll
+				for (int hx = 0; hx < PX; hx++) {
+					const float b = brow[(ht * PX + hy) * PX + hx];
+					acc0 = acc0 + r0[hx] * b;
+					acc1 = acc1 + r1[hx] * b;
+				}
+			}
+		coef[(pos0 + x) * FAST_P + lane] = acc0;
+		coef[(pos0 + x + 1) * FAST_P + lane] = acc1;
+	}
+	if (x < x1) {
+		const float *s0 = img + pos0 + x;
+		float acc = 0.f;
+#pragma unroll
+		for (int ht = 0; ht < PT; ht++)
+#pragma unroll
+			for (int hy = 0; hy < PX; hy++) {
+				const float *r = s0 + ht * WH + hy * W;
+#pragma unroll
+				for (int hx = 0; hx < PX; hx++) acc = acc + r[hx] * brow[(ht * PX + hy) * PX + hx];
+			}
+		coef[(pos0 + x) * FAST_P + lane] = acc;
+	}
+}
+template __global__ void coef_kernel<4>(const float *, int, int, int, const float *, float *);
+template __global__ void coef_kernel<8>(const float *, int, int, int, const float *, float *);
+
+// Coefficient rows of the N patches at anchors gidx[0..N) from a
+// coef_kernel table: lane k gets z[n] = coef[gidx[n]*64 + k] (rows n >= N zero).
+__device__ __forceinline__ void fast_rows(const float *__restrict__ coef, const uint32_t *__restrict__ gidx, int N,
+                                          float (&z)[FAST_NMAX])
+{
+#pragma unroll
+	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? coef[(size_t)gidx[n] * FAST_P + threadIdx.x] : 0.f;
+}
+
 // Equality early-out (nldct.cpp:1986-1999, 2222-2235) with lane = sample:
 // true when N > 1 and, in every one of the C channel planes (cstride
 // apart), every patch equals patch 0; such a group is written raw (rows of
@@ -1557,7 +1628,7 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
                          float *__restrict__ entry_w, uint32_t *__restrict__ etile,
                          uint8_t *__restrict__ early, float *__restrict__ gweight,
                          uint32_t *__restrict__ n_early, int C, int c, int64_t cstride,
-                         float *__restrict__ gacc)
+                         float *__restrict__ gacc, const float *__restrict__ coef)
 {
 	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
 	const int gl = blockIdx.x;
@@ -1574,16 +1645,20 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 	} else if (early[g])
 		return;
 
-	{
-		float brow[FAST_P];
-#pragma unroll
-		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
-		fast_forward<PX>(img + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
-	}
-	__syncthreads();
 	float z[FAST_NMAX];
+	if (coef)
+		fast_rows(coef + c * cstride * FAST_P, gidx, N, z);
+	else {
+		{
+			float brow[FAST_P];
 #pragma unroll
-	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+			for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
+			fast_forward<PX>(img + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
+		}
+		__syncthreads();
+#pragma unroll
+		for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+	}
 	haar_lane(z, N, true);
 	int cnt = 0;
 #pragma unroll
@@ -1626,7 +1701,8 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
                          float *__restrict__ entry_w, uint32_t *__restrict__ etile,
                          uint8_t *__restrict__ early, float *__restrict__ gweight,
                          uint32_t *__restrict__ n_early, int C, int c, int64_t cstride,
-                         float *__restrict__ gacc)
+                         float *__restrict__ gacc, const float *__restrict__ coef_n,
+                         const float *__restrict__ coef_b)
 {
 	__shared__ __align__(16) float zs[FAST_NMAX * FAST_P];
 	const int gl = blockIdx.x;
@@ -1644,20 +1720,25 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 		return;
 
 	float z[FAST_NMAX], y[FAST_NMAX];
-	{
-		float brow[FAST_P];
+	if (coef_n) {
+		fast_rows(coef_b + c * cstride * FAST_P, gidx, N, y);
+		fast_rows(coef_n + c * cstride * FAST_P, gidx, N, z);
+	} else {
+		{
+			float brow[FAST_P];
 #pragma unroll
-		for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
-		fast_forward<PX>(basic + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
+			for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
+			fast_forward<PX>(basic + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
+			__syncthreads();
+#pragma unroll
+			for (int n = 0; n < FAST_NMAX; n++) y[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
+			__syncthreads();
+			fast_forward<PX>(noisy + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
+		}
 		__syncthreads();
 #pragma unroll
-		for (int n = 0; n < FAST_NMAX; n++) y[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
-		__syncthreads();
-		fast_forward<PX>(noisy + c * cstride, gidx, N, sc.W, sc.WH, brow, zs);
+		for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
 	}
-	__syncthreads();
-#pragma unroll
-	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
 	haar_lane(z, N, true);
 	haar_lane(y, N, true);
 	// Wiener shrinkage; y[n] becomes the energy term w^2
@@ -2063,10 +2144,11 @@ template __global__ void learned_pairs_kernel<8>(PAIRS_ARGS);
 
 #define FILTER1_ARGS const float *, const uint32_t *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, StepConst, int, int, const float *, const float *, const float *, float *, \
-	float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
+	float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *, const float *
 #define FILTER2_ARGS const float *, const float *, const uint32_t *, const uint32_t *, \
 	const uint32_t *, const uint32_t *, StepConst, int, int, const float *, const float *, \
-	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *
+	const float *, float *, float *, uint32_t *, uint8_t *, float *, uint32_t *, int, int, int64_t, float *, \
+	const float *, const float *
 template __global__ void filter1_fast_kernel<4>(FILTER1_ARGS);
 template __global__ void filter1_fast_kernel<8>(FILTER1_ARGS);
 template __global__ void filter2_fast_kernel<4>(FILTER2_ARGS);

@@ -119,6 +119,13 @@ const char *bm4dof_global_error(void);
  *                  high); 0: the banded aggregation for every block.
  *   "agg_cap"      rows sorted per step of the chunked aggregation, for one
  *                  channel: 512, 768, 1024 (default) or 1536.
+ *   "coef_budget"  bytes of device memory a step of the register filters may
+ *                  hold in per-anchor coefficient tables (the forward
+ *                  transform of every patch position, computed once per
+ *                  image; step 1 one table, step 2 two, each C*W*H*F*64
+ *                  floats; default 16 GiB).  Steps whose tables do not fit
+ *                  transform the patches group by group; the output is the
+ *                  same either way.  0 disables the tables.
  * Returns BM4DOF_EINVAL for an unknown key.
  */
 int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value);

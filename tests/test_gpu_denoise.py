@@ -139,7 +139,9 @@ def test_video_two_step_parity(ctx, w, h, f, nparts, flow):
 @pytest.mark.parametrize("px,pt,np_,frames", [(4, 4, 32, 8), (8, 1, 16, 1), (4, 4, 8, 6), (8, 1, 32, 3)])
 def test_register_and_lds_filters_agree(ctx, px, pt, np_, frames):
     """The register-resident filter (one channel, 64-sample patches) and the
-    general LDS filter give bitwise equal basic and final estimates."""
+    general LDS filter give bitwise equal basic and final estimates; so does
+    the register filter without the per-anchor coefficient tables
+    (coef_budget 0: every group transforms its patches itself)."""
     clean, noisy = _video(80, 56, frames, seed=10 * px + np_)
     a1 = A.cli_params(1, 20, noisy.shape, px=px, pt=pt, wx=9, np_=np_)
     a2 = A.cli_params(2, 20, noisy.shape, px=px, pt=pt, wx=9, np_=np_)
@@ -150,6 +152,12 @@ def test_register_and_lds_filters_agree(ctx, px, pt, np_, frames):
         ctx.set_option("fast_filter", 1)
     b1, f1, _ = ctx.run(a1, a2, noisy, nparts=2)
     assert np.array_equal(b0, b1) and np.array_equal(f0, f1)
+    try:
+        ctx.set_option("coef_budget", 0)
+        b2, f2, _ = ctx.run(a1, a2, noisy, nparts=2)
+    finally:
+        ctx.set_option("coef_budget", 16 << 30)
+    assert np.array_equal(b2, b1) and np.array_equal(f2, f1)
     with pytest.raises(A.BM4DOFError, match="unknown option"):
         ctx.set_option("no_such_option", 1)
 
@@ -269,7 +277,8 @@ def test_color_register_paths_parity(ctx, channels, px, pt, flow, flat):
     """Colour video through the register-resident filters (one launch per
     channel, the group weight counting every channel) and the register-window
     matching (step-2 distances summed over the channels): bitwise equal to
-    the CPU oracle and to the general kernels.  `flat` (RGB, no transform)
+    the CPU oracle, to the general kernels and to the register filters
+    without coefficient tables (coef_budget 0).  `flat` (RGB, no transform)
     puts a region constant in every channel, which takes the equality
     early-out, beside one constant in the first channel only, which must
     not."""
@@ -299,6 +308,12 @@ def test_color_register_paths_parity(ctx, channels, px, pt, flow, flat):
         ctx.set_option("fast_filter", 1)
         ctx.set_option("fast_bm", 2)
     assert np.array_equal(b0, gb, equal_nan=True) and np.array_equal(f0, gf, equal_nan=True)
+    try:
+        ctx.set_option("coef_budget", 0)
+        b2, f2, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+    finally:
+        ctx.set_option("coef_budget", 16 << 30)
+    assert np.array_equal(b2, gb, equal_nan=True) and np.array_equal(f2, gf, equal_nan=True)
 
 
 @pytest.mark.parametrize("channels,flow", [(3, True), (4, False)])
