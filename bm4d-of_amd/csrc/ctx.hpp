@@ -76,6 +76,14 @@ struct bm4dof_ctx {
 	size_t est_budget = (size_t)24 << 30;
 	// bytes of per-anchor coefficient tables a step may hold (0: none)
 	size_t coef_budget = (size_t)16 << 30;
+	// the noisy coefficient table of the last step that filled it (patch
+	// shape, channels, size, basis), and whether the next step of the same
+	// run may read it: set around the second step when both steps see the
+	// same noisy planes
+	bool coef_n_ok = false, coef_n_keep = false;
+	int coef_n_shape[4] = {0, 0, 0, 0};
+	size_t coef_n_whf = 0;
+	std::vector<float> coef_n_basis;
 	// ordered-aggregation statistics of the last run
 	uint64_t agg_blocks = 0, agg_overflow_blocks = 0;
 	int agg_region[3] = {0, 0, 0};

@@ -889,16 +889,27 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		const int nx = sc.W - sc.sPx + 1, ny = sc.H - sc.sPx + 1, nt = sc.F - sc.sPt + 1;
 		const int64_t waves = nx > 0 && ny > 0 && nt > 0 ? (int64_t)((nx + 15) / 16) * ny * nt : 0;
 		if (per * sizeof(float) * nimg <= ctx->coef_budget && waves > 0 && waves * 64 < ((int64_t)1 << 32)) {
+			const bool have_n = !step1 && ctx->coef_n_keep && ctx->coef_n_ok && ctx->coef_n_whf == WHF &&
+			                    ctx->coef_n_shape[0] == sc.sPx && ctx->coef_n_shape[1] == sc.sPt &&
+			                    ctx->coef_n_shape[2] == C && ctx->coef_n_basis == pl.hb;
+			ctx->coef_n_ok = false;
 			for (int i = 0; i < nimg; i++) {
 				float *tab = (i ? ctx->coef_b : ctx->coef_n).as<float>(per);
 				const float *src = i ? d_bplanes : d_nplanes;
+				(i ? d_coef_b : d_coef_n) = tab;
+				if (i == 0 && have_n) continue;   // the first step's table of the same planes
 				for (int c = 0; c < C; c++)
 					hipLaunchKernelGGL(sc.sPx == 4 ? coef_kernel<4> : coef_kernel<8>, dim3((unsigned)waves), dim3(64),
 					                   0, ctx->stream, src + (size_t)c * WHF, sc.W, sc.H, sc.F, dv.basis,
 					                   tab + (size_t)c * WHF * sc.P);
-				(i ? d_coef_b : d_coef_n) = tab;
 			}
 			HIPCHK(hipGetLastError());
+			ctx->coef_n_ok = true;
+			ctx->coef_n_whf = WHF;
+			ctx->coef_n_shape[0] = sc.sPx;
+			ctx->coef_n_shape[1] = sc.sPt;
+			ctx->coef_n_shape[2] = C;
+			ctx->coef_n_basis = pl.hb;
 		}
 		mark("l_coef", "s1_coef", "s2_coef");
 	}
@@ -1182,6 +1193,9 @@ static void two_step(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
                      float *basic, float *final, int nparts, Timer &tm, float *gr)
 {
 	const size_t n = (size_t)W * H * F * C;
+	// the second step reads the first one's noisy coefficient table when both
+	// steps filter the same noisy planes (same colour space)
+	ctx->coef_n_ok = ctx->coef_n_keep = false;
 	if (step != 2) {
 		const float *nw = noisy;
 		if (C > 1 && p1->colorSpace == BM4DOF_YUV) {
@@ -1202,7 +1216,9 @@ static void two_step(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_para
 			color(ctx, basic, W, H, F, C, true);
 			nw = t;
 		}
+		ctx->coef_n_keep = step == 0 && (C == 1 || p1->colorSpace == p2->colorSpace);
 		run_step(ctx, *p2, W, H, F, C, nw, basic, ff, bf, final, nparts, tm, nullptr, &gr[1]);
+		ctx->coef_n_ok = ctx->coef_n_keep = false;
 		if (C > 1 && p2->colorSpace == BM4DOF_YUV) {
 			color(ctx, basic, W, H, F, C, false);
 			color(ctx, final, W, H, F, C, false);

@@ -1509,7 +1509,8 @@ __device__ __forceinline__ void fast_inverse_sparse(const float (&z)[FAST_NMAX],
 // coef[pos*64 + k] = sum_p B[k][p] x[pos + off(p)] in the order of
 // fast_forward (the same dot product whatever group the patch is in, so the
 // filters' rows read it bit for bit).  One wave per run of anchors along x,
-// lane = coefficient, basis row in registers, two anchors per round.
+// lane = coefficient, basis row in registers, four anchors per round (their
+// patches overlap: one load of a row segment serves all four).
 template <int PX>
 __global__ __launch_bounds__(64)
 void coef_kernel(const float *__restrict__ img, int W, int H, int F, const float *__restrict__ basis,
@@ -1532,25 +1533,28 @@ void coef_kernel(const float *__restrict__ img, int W, int H, int F, const float
 	for (int p = 0; p < FAST_P; p++) brow[p] = basis[lane * FAST_P + p];
 	const int64_t pos0 = (int64_t)t * WH + (int64_t)y * W;
 	int x = x0;
-	for (; x + 1 < x1; x += 2) {
-		const float *s0 = img + pos0 + x, *s1 = s0 + 1;
-		float acc0 = 0.f, acc1 = 0.f;
+	for (; x + 3 < x1; x += 4) {
+		const float *s0 = img + pos0 + x;
+		float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
 		for (int ht = 0; ht < PT; ht++)
 #pragma unroll
 			for (int hy = 0; hy < PX; hy++) {
-				const float *r0 = s0 + ht * WH + hy * W, *r1 = s1 + ht * WH + hy * W;
+				const float *r = s0 + ht * WH + hy * W;
+				float seg[PX + 3];
+#pragma unroll
+				for (int i = 0; i < PX + 3; i++) seg[i] = r[i];
 #pragma unroll
 				for (int hx = 0; hx < PX; hx++) {
 					const float b = brow[(ht * PX + hy) * PX + hx];
-					acc0 = acc0 + r0[hx] * b;
-					acc1 = acc1 + r1[hx] * b;
+#pragma unroll
+					for (int a = 0; a < 4; a++) acc[a] = acc[a] + seg[hx + a] * b;
 				}
 			}
-		coef[(pos0 + x) * FAST_P + lane] = acc0;
-		coef[(pos0 + x + 1) * FAST_P + lane] = acc1;
+#pragma unroll
+		for (int a = 0; a < 4; a++) coef[(pos0 + x + a) * FAST_P + lane] = acc[a];
 	}
-	if (x < x1) {
+	for (; x < x1; x++) {
 		const float *s0 = img + pos0 + x;
 		float acc = 0.f;
 #pragma unroll
