@@ -156,14 +156,15 @@ def test_register_and_lds_filters_agree(ctx, px, pt, np_, frames):
         ctx.set_option("coef_budget", 0)
         b2, f2, _ = ctx.run(a1, a2, noisy, nparts=2)
     finally:
-        ctx.set_option("coef_budget", 16 << 30)
+        ctx.set_option("coef_budget", 32 << 30)
     assert np.array_equal(b2, b1) and np.array_equal(f2, f1)
     with pytest.raises(A.BM4DOFError, match="unknown option"):
         ctx.set_option("no_such_option", 1)
 
 
 @pytest.mark.parametrize("w,h,f,nparts,px,pt", [(96, 64, 8, 4, 4, 4), (64, 48, 6, 2, 4, 4),
-                                                 (80, 72, 1, 4, 8, 1), (50, 38, 7, 6, 4, 4)])
+                                                 (80, 72, 1, 4, 8, 1), (50, 38, 7, 6, 4, 4),
+                                                 (72, 56, 9, 2, 4, 2)])
 def test_chunked_and_banded_aggregation_agree(ctx, w, h, f, nparts, px, pt):
     """The chunked aggregation (per-pixel masks of sorted row chunks) and the
     banded one fold the same rows in the same order: bitwise equal output,
@@ -312,7 +313,7 @@ def test_color_register_paths_parity(ctx, channels, px, pt, flow, flat):
         ctx.set_option("coef_budget", 0)
         b2, f2, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
     finally:
-        ctx.set_option("coef_budget", 16 << 30)
+        ctx.set_option("coef_budget", 32 << 30)
     assert np.array_equal(b2, gb, equal_nan=True) and np.array_equal(f2, gf, equal_nan=True)
 
 
