@@ -75,7 +75,7 @@ struct bm4dof_ctx {
 	std::vector<hipEvent_t> ev;
 	size_t est_budget = (size_t)24 << 30;
 	// bytes of per-anchor coefficient tables a step may hold (0: none)
-	size_t coef_budget = (size_t)32 << 30;
+	size_t coef_budget = (size_t)96 << 30;
 	// the noisy coefficient table of the last step that filled it (patch
 	// shape, channels, size, basis), and whether the next step of the same
 	// run may read it: set around the second step when both steps see the

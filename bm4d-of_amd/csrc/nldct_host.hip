@@ -885,6 +885,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	const float *d_coef_n = nullptr, *d_coef_b = nullptr;
 	if (pl.fast && !la) {
 		const int nimg = step1 ? 1 : 2;
+		bool nimg_ok = true;
 		const size_t per = (size_t)C * WHF * sc.P;
 		const int nx = sc.W - sc.sPx + 1, ny = sc.H - sc.sPx + 1, nt = sc.F - sc.sPt + 1;
 		const int64_t waves = nx > 0 && ny > 0 && nt > 0 ? (int64_t)((nx + 15) / 16) * ny * nt : 0;
@@ -893,8 +894,19 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 			                    ctx->coef_n_shape[0] == sc.sPx && ctx->coef_n_shape[1] == sc.sPt &&
 			                    ctx->coef_n_shape[2] == C && ctx->coef_n_basis == pl.hb;
 			ctx->coef_n_ok = false;
-			for (int i = 0; i < nimg; i++) {
-				float *tab = (i ? ctx->coef_b : ctx->coef_n).as<float>(per);
+			// tables the device cannot hold are left out (the filters then
+			// transform group by group)
+			float *tabs[2] = {nullptr, nullptr};
+			try {
+				for (int i = 0; i < nimg; i++) tabs[i] = (i ? ctx->coef_b : ctx->coef_n).as<float>(per);
+			} catch (const ArgError &e) {
+				if (e.code != BM4DOF_ENOMEM) throw;
+				ctx->coef_n.release();
+				ctx->coef_b.release();
+				nimg_ok = false;
+			}
+			for (int i = 0; nimg_ok && i < nimg; i++) {
+				float *tab = tabs[i];
 				const float *src = i ? d_bplanes : d_nplanes;
 				(i ? d_coef_b : d_coef_n) = tab;
 				if (i == 0 && have_n) continue;   // the first step's table of the same planes
@@ -904,7 +916,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 					                   tab + (size_t)c * WHF * sc.P);
 			}
 			HIPCHK(hipGetLastError());
-			ctx->coef_n_ok = true;
+			ctx->coef_n_ok = nimg_ok;
 			ctx->coef_n_whf = WHF;
 			ctx->coef_n_shape[0] = sc.sPx;
 			ctx->coef_n_shape[1] = sc.sPt;

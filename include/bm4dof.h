@@ -123,9 +123,10 @@ const char *bm4dof_global_error(void);
  *                  hold in per-anchor coefficient tables (the forward
  *                  transform of every patch position, computed once per
  *                  image; step 1 one table, step 2 two, each C*W*H*F*64
- *                  floats; default 32 GiB).  Steps whose tables do not fit
- *                  transform the patches group by group; the output is the
- *                  same either way.  0 disables the tables.
+ *                  floats; default 96 GiB).  Steps whose tables do not fit
+ *                  the budget or the free device memory transform the
+ *                  patches group by group; the output is the same either
+ *                  way.  0 disables the tables.
  * Returns BM4DOF_EINVAL for an unknown key.
  */
 int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value);

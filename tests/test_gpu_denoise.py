@@ -156,7 +156,7 @@ def test_register_and_lds_filters_agree(ctx, px, pt, np_, frames):
         ctx.set_option("coef_budget", 0)
         b2, f2, _ = ctx.run(a1, a2, noisy, nparts=2)
     finally:
-        ctx.set_option("coef_budget", 32 << 30)
+        ctx.set_option("coef_budget", 96 << 30)
     assert np.array_equal(b2, b1) and np.array_equal(f2, f1)
     with pytest.raises(A.BM4DOFError, match="unknown option"):
         ctx.set_option("no_such_option", 1)
@@ -313,7 +313,7 @@ def test_color_register_paths_parity(ctx, channels, px, pt, flow, flat):
         ctx.set_option("coef_budget", 0)
         b2, f2, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
     finally:
-        ctx.set_option("coef_budget", 32 << 30)
+        ctx.set_option("coef_budget", 96 << 30)
     assert np.array_equal(b2, gb, equal_nan=True) and np.array_equal(f2, gf, equal_nan=True)
 
 
