@@ -1320,6 +1320,10 @@ void filter2_kernel(const float *__restrict__ noisy, const float *__restrict__ b
 #define FAST_P 64
 #define FAST_NMAX 32
 
+// Haar transform across the rows of lane-held columns, levels up to N.
+// Rows n >= N must be +0 on entry and stay so: the butterflies that start
+// at such a row pair two of them (n + h > n >= N) and give +0 again, so
+// they run unguarded, without a select per row.
 __device__ __forceinline__ void haar_lane(float (&z)[FAST_NMAX], int N, bool forward)
 {
 	const float isqrt2 = 0.70710678118654752440f;
@@ -1330,12 +1334,10 @@ __device__ __forceinline__ void haar_lane(float (&z)[FAST_NMAX], int N, bool for
 		if (s <= N) {
 #pragma unroll
 			for (int n = 0; n < FAST_NMAX; n += s) {
-				if (n < N) {
-					const float z0 = z[n] * isqrt2;
-					const float z1 = z[n + h] * isqrt2;
-					z[n] = z0 + z1;
-					z[n + h] = z0 - z1;
-				}
+				const float z0 = z[n] * isqrt2;
+				const float z1 = z[n + h] * isqrt2;
+				z[n] = z0 + z1;
+				z[n + h] = z0 - z1;
 			}
 		}
 	}
@@ -1756,11 +1758,13 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 	}
 	// sum of w^2 in the reference's order, channel, coefficient, then patch:
 	// lane k folds its N terms into the running sum handed on by lane k-1.
+	// Rows n >= N hold +0 and the sum is >= +0 (or NaN), so adding them
+	// leaves it unchanged bit for bit: no select per term.
 	float v = c ? gacc[g] : 0.f;
 	for (int k = 0; k < FAST_P; k++) {
 		float t = v;
 #pragma unroll
-		for (int n = 0; n < FAST_NMAX; n++) if (n < N) t = t + y[n];
+		for (int n = 0; n < FAST_NMAX; n++) t = t + y[n];
 		v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t), k));
 	}
 	haar_lane(z, N, false);
