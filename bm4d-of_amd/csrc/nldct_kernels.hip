@@ -1666,19 +1666,19 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 		for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? zs[n * FAST_P + lane] : 0.f;
 	}
 	haar_lane(z, N, true);
+	// rows n >= N are +0: 0 > (beta sigma)^2 fails, so they count nothing
+	// and stay +0 without a guard
 	int cnt = 0;
 #pragma unroll
 	for (int n = 0; n < FAST_NMAX; n++) {
-		if (n < N) {
-			if (z[n] * z[n] > sc.beta_sigma2 || (lane == 0 && n == 0)) cnt++;
-			else z[n] = 0.f;
-		}
+		if (z[n] * z[n] > sc.beta_sigma2 || (lane == 0 && n == 0)) cnt++;
+		else z[n] = 0.f;
 	}
 	haar_lane(z, N, false);
 	// columns with a coefficient left in any row (the others are +0 throughout)
 	bool nzc = false;
 #pragma unroll
-	for (int n = 0; n < FAST_NMAX; n++) if (n < N) nzc |= z[n] != 0.f;
+	for (int n = 0; n < FAST_NMAX; n++) nzc |= z[n] != 0.f;
 	const uint64_t kmask = __ballot(nzc);
 	const int nnz_i = wave_sum_int(cnt) + (c ? (int)gacc[g] : 0);
 	const float nnz = (float)nnz_i;
