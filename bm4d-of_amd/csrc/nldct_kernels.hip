@@ -1582,6 +1582,20 @@ __device__ __forceinline__ void fast_rows(const float *__restrict__ coef, const 
 	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? coef[(size_t)gidx[n] * FAST_P + threadIdx.x] : 0.f;
 }
 
+// Whether all N coefficient rows are bitwise equal (every lane must call
+// it).  Equal patches give equal rows: the same sums in the same order, and
+// a +-0 term cannot change a sum that starts at +0 (it never becomes -0).
+// So rows that differ rule out the equality early-out without reading the
+// patches.
+__device__ __forceinline__ bool fast_rows_equal(const float (&z)[FAST_NMAX], int N)
+{
+	const uint32_t z0 = __float_as_uint(z[0]);
+	bool d = false;
+#pragma unroll
+	for (int n = 1; n < FAST_NMAX; n++) d |= n < N && __float_as_uint(z[n]) != z0;
+	return !__any(d);
+}
+
 // Equality early-out (nldct.cpp:1986-1999, 2222-2235) with lane = sample:
 // true when N > 1 and, in every one of the C channel planes (cstride
 // apart), every patch equals patch 0; such a group is written raw (rows of
@@ -1644,17 +1658,17 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 	const int N = (int)nsim[g];
 	const uint32_t row0 = base[gl];
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	float z[FAST_NMAX];
+	if (coef) fast_rows(coef + c * cstride * FAST_P, gidx, N, z);
 	if (c == 0) {
 		if (lane < N) etile[row0 + lane] = gtile[g];
-		if (fast_early_out<PX>(img, gidx, N, sc.W, sc.WH, C, cstride, g, row0, est, early, gweight, n_early))
+		if ((!coef || fast_rows_equal(z, N)) &&
+		    fast_early_out<PX>(img, gidx, N, sc.W, sc.WH, C, cstride, g, row0, est, early, gweight, n_early))
 			return;
 	} else if (early[g])
 		return;
 
-	float z[FAST_NMAX];
-	if (coef)
-		fast_rows(coef + c * cstride * FAST_P, gidx, N, z);
-	else {
+	if (!coef) {
 		{
 			float brow[FAST_P];
 #pragma unroll
@@ -1718,18 +1732,19 @@ void filter2_fast_kernel(const float *__restrict__ noisy, const float *__restric
 	const int N = (int)nsim[g];
 	const uint32_t row0 = base[gl];
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
+	float z[FAST_NMAX], y[FAST_NMAX];
+	if (coef_n) fast_rows(coef_n + c * cstride * FAST_P, gidx, N, z);
 	if (c == 0) {
 		if (lane < N) etile[row0 + lane] = gtile[g];
-		if (fast_early_out<PX>(noisy, gidx, N, sc.W, sc.WH, C, cstride, g, row0, est, early, gweight, n_early))
+		if ((!coef_n || fast_rows_equal(z, N)) &&
+		    fast_early_out<PX>(noisy, gidx, N, sc.W, sc.WH, C, cstride, g, row0, est, early, gweight, n_early))
 			return;
 	} else if (early[g])
 		return;
 
-	float z[FAST_NMAX], y[FAST_NMAX];
-	if (coef_n) {
+	if (coef_n)
 		fast_rows(coef_b + c * cstride * FAST_P, gidx, N, y);
-		fast_rows(coef_n + c * cstride * FAST_P, gidx, N, z);
-	} else {
+	else {
 		{
 			float brow[FAST_P];
 #pragma unroll
