@@ -5,6 +5,7 @@
 #include "common.hpp"
 #include "../../include/bm4dof.h"
 
+#include <atomic>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -28,13 +29,19 @@ struct ArgError : std::runtime_error {
 			throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));               \
 	} while (0)
 
-// Grow-only device buffer.
+// Grow-only device buffer.  `moves` counts, over the process, the times a
+// buffer's memory was freed or replaced: launches captured into a graph
+// hold raw addresses and are replayed only while it is unchanged.
 struct DevBuf {
 	void *p = nullptr;
 	size_t n = 0;
+	static inline std::atomic<uint64_t> moves{0};
 	void *get(size_t bytes) {
 		if (bytes <= n && p) return p;
-		if (p) HIPCHK(hipFree(p));
+		if (p) {
+			moves++;
+			HIPCHK(hipFree(p));
+		}
 		p = nullptr;
 		n = 0;
 		if (bytes == 0) bytes = 256;
@@ -48,7 +55,10 @@ struct DevBuf {
 	}
 	template <class T> T *as(size_t count) { return (T *)get(count * sizeof(T)); }
 	void release() {
-		if (p) (void)hipFree(p);
+		if (p) {
+			moves++;
+			(void)hipFree(p);
+		}
 		p = nullptr;
 		n = 0;
 	}

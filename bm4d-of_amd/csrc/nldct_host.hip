@@ -786,6 +786,24 @@ StepDev upload_plan(bm4dof_ctx *ctx, const StepPlan &pl, DevBuf &geo_buf)
 
 // Grows every shared work buffer a plan's launches use, so that enqueueing
 // the step allocates nothing (required while a stream is being captured).
+// Threads of coef_kernel over one channel plane (one wave per 16 anchors
+// along x), 0 when the patch does not fit the video.
+int64_t coef_waves(const StepConst &sc)
+{
+	const int nx = sc.W - sc.sPx + 1, ny = sc.H - sc.sPx + 1, nt = sc.F - sc.sPt + 1;
+	return nx > 0 && ny > 0 && nt > 0 ? (int64_t)((nx + 15) / 16) * ny * nt : 0;
+}
+
+// Floats of one image's coefficient table for a step that uses them (the
+// register filters of a plain step, within coef_budget), else 0.
+size_t coef_floats(const StepPlan &pl, size_t budget)
+{
+	const int64_t waves = coef_waves(pl.sc);
+	if (!pl.fast || pl.learned || waves <= 0 || waves * 64 >= ((int64_t)1 << 32)) return 0;
+	const size_t per = (size_t)pl.C * pl.WHF * pl.sc.P;
+	return per * sizeof(float) * (pl.step1 ? 1 : 2) <= budget ? per : 0;
+}
+
 void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 {
 	const size_t G = (size_t)std::max(pl.G, 1), np = (size_t)pl.np;
@@ -821,6 +839,19 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 		(void)ctx->planes.as<float>(pl.N_all);
 		(void)ctx->gacc.as<float>(G);
 		if (!pl.step1) (void)ctx->planes_b.as<float>(pl.N_all);
+	}
+	// per-anchor coefficient tables within coef_budget; left out when the
+	// device cannot hold them (the filters then transform group by group)
+	const size_t per = coef_floats(pl, ctx->coef_budget);
+	if (per > 0) {
+		try {
+			(void)ctx->coef_n.as<float>(per);
+			if (!pl.step1) (void)ctx->coef_b.as<float>(per);
+		} catch (const ArgError &e) {
+			if (e.code != BM4DOF_ENOMEM) throw;
+			ctx->coef_n.release();
+			ctx->coef_b.release();
+		}
 	}
 	reserve_scan(ctx, (int64_t)std::max(pl.WHF + 1, G + 1));
 }
@@ -884,29 +915,17 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	// that holds it (bit for bit the same sums)
 	const float *d_coef_n = nullptr, *d_coef_b = nullptr;
 	if (pl.fast && !la) {
+		// the tables reserve_step allocated (nothing allocates here)
 		const int nimg = step1 ? 1 : 2;
-		bool nimg_ok = true;
-		const size_t per = (size_t)C * WHF * sc.P;
-		const int nx = sc.W - sc.sPx + 1, ny = sc.H - sc.sPx + 1, nt = sc.F - sc.sPt + 1;
-		const int64_t waves = nx > 0 && ny > 0 && nt > 0 ? (int64_t)((nx + 15) / 16) * ny * nt : 0;
-		if (per * sizeof(float) * nimg <= ctx->coef_budget && waves > 0 && waves * 64 < ((int64_t)1 << 32)) {
+		const size_t per = coef_floats(pl, ctx->coef_budget);
+		const int64_t waves = coef_waves(sc);
+		if (per > 0 && ctx->coef_n.n >= per * sizeof(float) && (step1 || ctx->coef_b.n >= per * sizeof(float))) {
 			const bool have_n = !step1 && ctx->coef_n_keep && ctx->coef_n_ok && ctx->coef_n_whf == WHF &&
 			                    ctx->coef_n_shape[0] == sc.sPx && ctx->coef_n_shape[1] == sc.sPt &&
 			                    ctx->coef_n_shape[2] == C && ctx->coef_n_basis == pl.hb;
 			ctx->coef_n_ok = false;
-			// tables the device cannot hold are left out (the filters then
-			// transform group by group)
-			float *tabs[2] = {nullptr, nullptr};
-			try {
-				for (int i = 0; i < nimg; i++) tabs[i] = (i ? ctx->coef_b : ctx->coef_n).as<float>(per);
-			} catch (const ArgError &e) {
-				if (e.code != BM4DOF_ENOMEM) throw;
-				ctx->coef_n.release();
-				ctx->coef_b.release();
-				nimg_ok = false;
-			}
-			for (int i = 0; nimg_ok && i < nimg; i++) {
-				float *tab = tabs[i];
+			for (int i = 0; i < nimg; i++) {
+				float *tab = (float *)(i ? ctx->coef_b : ctx->coef_n).p;
 				const float *src = i ? d_bplanes : d_nplanes;
 				(i ? d_coef_b : d_coef_n) = tab;
 				if (i == 0 && have_n) continue;   // the first step's table of the same planes
@@ -916,7 +935,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 					                   tab + (size_t)c * WHF * sc.P);
 			}
 			HIPCHK(hipGetLastError());
-			ctx->coef_n_ok = nimg_ok;
+			ctx->coef_n_ok = true;
 			ctx->coef_n_whf = WHF;
 			ctx->coef_n_shape[0] = sc.sPx;
 			ctx->coef_n_shape[1] = sc.sPt;
@@ -1454,6 +1473,7 @@ struct bm4dof_batch {
 	std::vector<TuneChoice> choice;       // per sequence
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
+	uint64_t moves_at_capture = 0;   // DevBuf::moves when the graph was captured
 	float last_ms = 0.f;
 	std::string report;
 };
@@ -1706,6 +1726,7 @@ int bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_pa
 			}
 			HIPCHK(hipStreamEndCapture(ctx->stream, &b->graph));
 			HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+			b->moves_at_capture = DevBuf::moves.load();
 		}
 		apply_choice(ctx, saved);
 	});
@@ -1729,7 +1750,9 @@ int bm4dof_batch_run(bm4dof_batch *b)
 		HIPCHK(hipEventCreate(&e0));
 		HIPCHK(hipEventCreate(&e1));
 		HIPCHK(hipEventRecord(e0, ctx->stream));
-		if (b->exec) HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
+		// the graph holds the context buffers' addresses of its capture: when
+		// work on the context has moved any of them since, enqueue instead
+		if (b->exec && b->moves_at_capture == DevBuf::moves.load()) HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
 		else batch_enqueue(b);
 		HIPCHK(hipEventRecord(e1, ctx->stream));
 		HIPCHK(hipEventSynchronize(e1));

@@ -218,7 +218,9 @@ int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
  * class and the fastest set is used for that class.  Buffers are device
  * pointers and must stay valid for the batch's lifetime; a captured batch
  * holds its scratch memory in the context, so runs on that context of other
- * work in between are allowed but not concurrently.
+ * work in between are allowed but not concurrently.  When such work has
+ * grown (moved) the context's buffers since the capture, the batch enqueues
+ * its launches instead of replaying the graph.
  */
 typedef struct bm4dof_seq {
 	const float *noisy;          /* w*h*f, one channel */

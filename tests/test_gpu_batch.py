@@ -91,6 +91,36 @@ def test_batch_equals_single_sequences(ctx, graph):
     assert not refs[4][1].any()
 
 
+def test_graph_batch_after_buffers_grow(ctx):
+    """A captured batch holds the context buffers' addresses; a run of a
+    larger video on the same context between replays makes the context
+    grow (move) them.  The next runs must still give the single-sequence
+    outputs (the batch then enqueues instead of replaying)."""
+    seqs = _make(seed0=500)[:3]
+    refs = [_single(ctx, d) for d in seqs]
+    batch = ctx.batch([_seq(d) for d in seqs], [d["p1"] for d in seqs], [d["p2"] for d in seqs],
+                      nparts=2, graph=True)
+    try:
+        clean = synth.video(160, 120, 10, seed=77)
+        big = {"shape": (10, 1, 120, 160), "noisy": torch.from_numpy(synth.add_noise(clean, 20.0, 78)).cuda(),
+               "ff": None, "bf": None,
+               "p1": A.cli_params(1, 20.0, (10, 1, 120, 160), px=4, pt=4, wx=9, np_=16),
+               "p2": A.cli_params(2, 20.0, (10, 1, 120, 160), px=4, pt=4, wx=9, np_=16)}
+        for step in range(2):
+            if step:
+                _single(ctx, big)
+            for d in seqs:
+                d["basic"].zero_()
+                d["final"].zero_()
+            torch.cuda.synchronize()
+            batch.run()
+            for i, (d, (rb, rf)) in enumerate(zip(seqs, refs)):
+                assert np.array_equal(d["basic"].cpu().numpy(), rb), (step, i)
+                assert np.array_equal(d["final"].cpu().numpy(), rf), (step, i)
+    finally:
+        batch.close()
+
+
 def test_autotuner_choices_give_equal_output(ctx):
     d = _make(seed0=500)[3]
     outs = []
