@@ -824,12 +824,10 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	const int maxCand = sc.sWx * sc.sWx * sWt;
 	uint32_t *skey = (uint32_t *)smem;          // maxCand distances
 	int *stie = (int *)(skey + maxCand);         // maxCand (radix fallback)
-	__shared__ uint32_t smin[64];
 	__shared__ uint32_t cand_key[BMR_CAP];
 	__shared__ int cand_rank[BMR_CAP];
 	__shared__ uint32_t out_key[64];
 	__shared__ int out_rank[64];
-	__shared__ uint32_t s_T0;
 
 	const int g = blockIdx.x;
 	if (g >= G) return;
@@ -921,20 +919,21 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	int kk = 1;
 	while (kk * 2 <= nSimP) kk *= 2;
 
-	// bound: the kk-th smallest lane minimum (lanes without candidates hold ~0)
-	smin[lane] = lmin;
-	if (lane == 0) s_T0 = 0xffffffffu;
-	__syncthreads();
+	// bound: the kk-th smallest lane minimum (lanes without candidates hold
+	// ~0, which then means no bound), by a bitonic sort across the wave
+	uint32_t T0;
 	{
-		int rk = 0;
-		for (int j = 0; j < 64; j++) {
-			const uint32_t v = smin[j];
-			rk += (v < lmin) || (v == lmin && j < lane);
-		}
-		if (rk == kk - 1 && lmin != 0xffffffffu) s_T0 = lmin;
+		uint32_t v = lmin;
+#pragma unroll
+		for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+			for (int j = k >> 1; j > 0; j >>= 1) {
+				const uint32_t o = (uint32_t)__shfl_xor((int)v, j, 64);
+				const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+				v = keep_min ? min(v, o) : max(v, o);
+			}
+		T0 = (uint32_t)__shfl((int)v, kk - 1, 64);
 	}
-	__syncthreads();
-	const uint32_t T0 = s_T0;
 	int cnt = 0;
 	bool fits = T0 != 0xffffffffu;
 	if (fits) {
