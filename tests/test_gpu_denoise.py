@@ -317,6 +317,24 @@ def test_color_register_paths_parity(ctx, channels, px, pt, flow, flat):
     assert np.array_equal(b2, gb, equal_nan=True) and np.array_equal(f2, gf, equal_nan=True)
 
 
+@pytest.mark.parametrize("cs1,cs2", [(1, 0), (0, 1)])
+def test_color_steps_in_different_colour_spaces(ctx, cs1, cs2):
+    """Steps in different colour spaces (one YUV, one RGB) filter different
+    noisy planes, so step 2 computes its own noisy coefficient table instead
+    of reading step 1's: bitwise equal to the CPU oracle, which transforms
+    around each step by its own parameter (nldct.cpp:363, 401, 422-423)."""
+    clean, noisy = _video(48, 40, 7, seed=300 + cs1, channels=3)
+    ff, bf = _smooth_flow(7, 40, 48, 8), _smooth_flow(7, 40, 48, 9)
+    a1, o1 = _params(noisy.shape, 1, px=4, pt=4, wx=7, np_=16)
+    a2, o2 = _params(noisy.shape, 2, px=4, pt=4, wx=7, np_=16)
+    a1.colorSpace = o1.colorSpace = cs1
+    a2.colorSpace = o2.colorSpace = cs2
+    ob, of_, _ = O.run(o1, o2, noisy, ff, bf, nparts=2)
+    gb, gf, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+    _same(gb, ob, "basic")
+    _same(gf, of_, "final")
+
+
 @pytest.mark.parametrize("channels,flow", [(3, True), (4, False)])
 def test_color_block_matching_kernels_agree(ctx, channels, flow):
     """Step-2 matching on colour video sums the distances of all channels
