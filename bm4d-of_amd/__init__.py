@@ -78,7 +78,8 @@ EXPORTS = [
     "bm4dof_tvl1_sequence", "bm4dof_ctx_set_option", "bm4dof_mlp_set", "bm4dof_mlp_get",
     "bm4dof_mlp_forward", "bm4dof_mlp_grad", "bm4dof_mlp_train", "bm4dof_learned_pairs",
     "bm4dof_run_learned", "bm4dof_run_learned_dev", "bm4dof_batch_create", "bm4dof_batch_run",
-    "bm4dof_batch_last_ms", "bm4dof_batch_report", "bm4dof_batch_destroy", "bm4dof_autotune",
+    "bm4dof_batch_last_ms", "bm4dof_batch_report", "bm4dof_batch_stats", "bm4dof_batch_destroy",
+    "bm4dof_autotune",
 ]
 
 BATCH_GRAPH = 1
@@ -163,6 +164,7 @@ def lib():
     L.bm4dof_batch_last_ms.restype = C.c_float
     L.bm4dof_batch_report.argtypes = [vp]
     L.bm4dof_batch_report.restype = C.c_char_p
+    L.bm4dof_batch_stats.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
     L.bm4dof_batch_destroy.argtypes = [vp]
     L.bm4dof_batch_destroy.restype = None
     L.bm4dof_autotune.argtypes = [vp, P, P, vp, C.c_uint, C.c_uint, C.c_uint, vp, vp, vp, vp, C.c_int,
@@ -462,6 +464,14 @@ class Batch:
 
     def report(self):
         return self.ctx._L.bm4dof_batch_report(self.h).decode()
+
+    def stats(self):
+        """How the last run ran: path (0 enqueued, 1 graph replayed, 2 graph
+        captured again), reruns (sequences run again for the mask scan),
+        captures (so far)."""
+        v = (C.c_int64 * 3)()
+        n = self.ctx._L.bm4dof_batch_stats(self.h, v, 3)
+        return dict(zip(("path", "reruns", "captures"), (int(v[i]) for i in range(n))))
 
     def close(self):
         if self.h:

@@ -95,6 +95,7 @@ bool read_png(const std::vector<unsigned char> &f, std::vector<float> &out, int 
 		if (o + 12 + len > f.size()) return fail("png: truncated chunk");
 		const unsigned char *d = &f[o + 8];
 		if (type == "IHDR") {
+			if (len != 13) return fail("png: bad IHDR chunk");
 			w = (int)be32(d);
 			h = (int)be32(d + 4);
 			depth = d[8];
@@ -117,6 +118,8 @@ bool read_png(const std::vector<unsigned char> &f, std::vector<float> &out, int 
 	default: return fail("png: palette images are not supported");
 	}
 	const int bps = depth / 8, bpp = bps * c;
+	// the sample limit of the reference's image library: fewer than 2^32
+	if ((uint64_t)w * (uint64_t)h * (uint64_t)c >= (1ull << 32)) return fail("png: image too large");
 	kind = bps == 1 ? SampleKind::U8 : SampleKind::U16;
 	const size_t stride = (size_t)w * bpp;
 	std::vector<unsigned char> raw((stride + 1) * h);

@@ -29,19 +29,13 @@ struct ArgError : std::runtime_error {
 			throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));               \
 	} while (0)
 
-// Grow-only device buffer.  `moves` counts, over the process, the times a
-// buffer's memory was freed or replaced: launches captured into a graph
-// hold raw addresses and are replayed only while it is unchanged.
+// Grow-only device buffer.
 struct DevBuf {
 	void *p = nullptr;
 	size_t n = 0;
-	static inline std::atomic<uint64_t> moves{0};
 	void *get(size_t bytes) {
 		if (bytes <= n && p) return p;
-		if (p) {
-			moves++;
-			HIPCHK(hipFree(p));
-		}
+		if (p) HIPCHK(hipFree(p));
 		p = nullptr;
 		n = 0;
 		if (bytes == 0) bytes = 256;
@@ -55,10 +49,7 @@ struct DevBuf {
 	}
 	template <class T> T *as(size_t count) { return (T *)get(count * sizeof(T)); }
 	void release() {
-		if (p) {
-			moves++;
-			(void)hipFree(p);
-		}
+		if (p) (void)hipFree(p);
 		p = nullptr;
 		n = 0;
 	}
@@ -84,8 +75,11 @@ struct bm4dof_ctx {
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
 	size_t est_budget = (size_t)24 << 30;
-	// bytes of per-anchor coefficient tables a step may hold (0: none)
+	// bytes of per-anchor coefficient tables a step may hold (0: none); at
+	// context creation the smaller of 96 GiB and 2/5 of the free device memory
 	size_t coef_budget = (size_t)96 << 30;
+	// keep the tables allocated between calls (else released at each call's end)
+	bool keep_tables = false;
 	// the noisy coefficient table of the last step that filled it (patch
 	// shape, channels, size, basis), and whether the next step of the same
 	// run may read it: set around the second step when both steps see the
@@ -115,6 +109,20 @@ struct bm4dof_ctx {
 	uint64_t tv_pixel_iters = 0;
 	// TV-L1 buffers
 	bm4dof::DevBuf tv_frames, tv_pairs, tv_part, tv_mm, tv_state, tv_work, tv_pyr, tv_taps;
+
+	// Every device buffer of the context: released by bm4dof_ctx_destroy;
+	// their addresses are what a captured batch graph depends on.
+	std::vector<bm4dof::DevBuf *> buffers()
+	{
+		return {&gpos, &gtile, &tiles, &idx, &nsim, &base, &keys, &vals, &skeys, &svals, &sort_tmp, &lstart,
+		        &lend, &rank_of, &etile, &entry_w, &est, &early, &gweight, &counters, &stale, &elist, &num,
+		        &wsum, &basis, &binv, &win, &noisy_w, &basic_w, &out_w, &io_noisy, &io_fflow, &io_bflow,
+		        &io_basic, &io_final, &planes, &planes_b, &coef_n, &coef_b, &gacc, &bsplit, &ties, &bm_image,
+		        &bm_wins, &bm_wmeta, &erec, &gflag, &fmask, &lp_idx, &stat64, &lastf, &fidx, &geo, &scan_tmp,
+		        &scan_tmp2, &mlp_theta, &mlp_m, &mlp_v, &mlp_wq, &mlp_inf,
+		        &mlp_ws, &mlp_grad, &mlp_xb, &mlp_tb, &mlp_sse, &tv_frames, &tv_pairs, &tv_part, &tv_mm,
+		        &tv_state, &tv_work, &tv_pyr, &tv_taps};
+	}
 };
 
 namespace bm4dof {

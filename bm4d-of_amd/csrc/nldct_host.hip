@@ -1466,32 +1466,108 @@ TuneChoice autotune(bm4dof_ctx *ctx, const bm4dof_params &p1, const bm4dof_param
 // captured into one hipGraph that every bm4dof_batch_run replays.
 struct bm4dof_batch {
 	bm4dof_ctx *ctx = nullptr;
+	int flags = 0, nparts = 2;
 	std::vector<bm4dof_seq> seqs;
 	std::vector<StepPlan> plans;          // 2 per sequence
 	std::vector<DevBuf> geo;              // 2 per sequence
 	std::vector<StepDev> dev;             // 2 per sequence
 	std::vector<TuneChoice> choice;       // per sequence
+	// per sequence and step: the count of groups whose first neighbour is not
+	// their own patch (first_moved_kernel, written inside the launch set)
+	DevBuf moved;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
-	uint64_t moves_at_capture = 0;   // DevBuf::moves when the graph was captured
+	std::vector<void *> captured_bufs;    // the context's buffer addresses the graph holds
+	int last_path = 0, last_reruns = 0, captures = 0;
 	float last_ms = 0.f;
 	std::string report;
 };
 
 namespace {
 
+std::vector<void *> buffer_addresses(bm4dof_ctx *ctx)
+{
+	std::vector<void *> a;
+	for (DevBuf *d : ctx->buffers()) a.push_back(d->p);
+	return a;
+}
+
+// Both steps of every sequence, in order.  Between a step's block matching
+// and its filter, first_moved_kernel counts the groups whose first neighbour
+// is another patch than their own (equal distances in heap order): the
+// reference then clears another position of its processing mask
+// (nldct.cpp:2325-2329) and skips groups, which only the eager path replays
+// (run_step), so bm4dof_batch_run runs such sequences again eagerly.
 void batch_enqueue(bm4dof_batch *b)
 {
 	bm4dof_ctx *ctx = b->ctx;
+	uint32_t *d_moved = b->moved.as<uint32_t>(2 * b->seqs.size());
+	uint32_t *d_idx = ctx->idx.as<uint32_t>(1);
 	for (size_t i = 0; i < b->seqs.size(); i++) {
 		const bm4dof_seq &q = b->seqs[i];
 		apply_choice(ctx, b->choice[i]);
 		const float *ff = (q.fflow && q.bflow) ? q.fflow : nullptr;
 		const float *bf = (q.fflow && q.bflow) ? q.bflow : nullptr;
-		enqueue_step(ctx, b->plans[2 * i], b->dev[2 * i], q.noisy, nullptr, ff, bf, q.basic, nullptr, nullptr);
-		enqueue_step(ctx, b->plans[2 * i + 1], b->dev[2 * i + 1], q.noisy, q.basic, ff, bf, q.final, nullptr,
-		             nullptr);
+		for (int s = 0; s < 2; s++) {
+			const StepPlan &pl = b->plans[2 * i + s];
+			const StepDev &dv = b->dev[2 * i + s];
+			const float *basic_in = s ? q.basic : nullptr;
+			float *out = s ? q.final : q.basic;
+			enqueue_step(ctx, pl, dv, q.noisy, basic_in, ff, bf, out, nullptr, nullptr, 1);
+			fill_u32(ctx, d_moved + 2 * i + s, 0u, 1);
+			if (pl.np > 1 && pl.G > 0) {
+				hipLaunchKernelGGL(first_moved_kernel, dim3(nblk(pl.G, 256)), dim3(256), 0, ctx->stream, d_idx,
+				                   dv.gpos, pl.np, pl.G, d_moved + 2 * i + s);
+				HIPCHK(hipGetLastError());
+			}
+			enqueue_step(ctx, pl, dv, q.noisy, basic_in, ff, bf, out, nullptr, nullptr, 2);
+		}
 	}
+}
+
+// (Re)captures the batch's launch set into a graph: every buffer the
+// launches use is reserved first, so nothing allocates inside the capture.
+void batch_capture(bm4dof_batch *b)
+{
+	bm4dof_ctx *ctx = b->ctx;
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+	for (const StepPlan &pl : b->plans) reserve_step(ctx, pl);
+	(void)b->moved.as<uint32_t>(2 * b->seqs.size());
+	HIPCHK(hipStreamSynchronize(ctx->stream));
+	const TuneChoice saved = current_choice(ctx);
+	HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+	try {
+		batch_enqueue(b);
+	} catch (...) {
+		hipGraph_t g = nullptr;
+		(void)hipStreamEndCapture(ctx->stream, &g);
+		if (g) (void)hipGraphDestroy(g);
+		apply_choice(ctx, saved);
+		throw;
+	}
+	apply_choice(ctx, saved);
+	HIPCHK(hipStreamEndCapture(ctx->stream, &b->graph));
+	HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+	b->captured_bufs = buffer_addresses(ctx);
+	b->captures++;
+}
+
+} // namespace
+
+namespace {
+
+// End of a denoiser call: the coefficient tables (several GB) go back to the
+// device unless the caller asked to keep them for the next call.
+void end_run(bm4dof_ctx *ctx)
+{
+	if (!ctx->keep_tables) {
+		ctx->coef_n.release();
+		ctx->coef_b.release();
+	}
+	ctx->coef_n_ok = false;
 }
 
 } // namespace
@@ -1555,6 +1631,10 @@ int bm4dof_ctx_create(bm4dof_ctx **out, int device)
 		HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+		// coefficient tables: at most 2/5 of the device memory free now
+		size_t free_b = 0, total_b = 0;
+		HIPCHK(hipMemGetInfo(&free_b, &total_b));
+		ctx->coef_budget = std::min(ctx->coef_budget, free_b / 5 * 2);
 	});
 	if (rc) {
 		g_global_err = ctx->err;
@@ -1574,20 +1654,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	if (ctx->side) (void)hipStreamSynchronize(ctx->side);
 	for (auto e : ctx->ev) (void)hipEventDestroy(e);
 	ctx->plan_cache.reset();
-	DevBuf *bufs[] = {&ctx->gpos, &ctx->gtile, &ctx->tiles, &ctx->idx, &ctx->nsim, &ctx->base,
-	                  &ctx->keys, &ctx->vals, &ctx->skeys, &ctx->svals, &ctx->sort_tmp, &ctx->lstart,
-	                  &ctx->lend, &ctx->rank_of, &ctx->etile, &ctx->entry_w, &ctx->est, &ctx->early,
-	                  &ctx->gweight, &ctx->counters, &ctx->stale, &ctx->elist, &ctx->num, &ctx->wsum,
-	                  &ctx->basis, &ctx->binv, &ctx->win, &ctx->noisy_w, &ctx->basic_w, &ctx->out_w,
-	                  &ctx->io_noisy, &ctx->io_fflow, &ctx->io_bflow, &ctx->io_basic, &ctx->io_final,
-	                  &ctx->planes, &ctx->planes_b, &ctx->coef_n, &ctx->coef_b, &ctx->gacc, &ctx->bsplit, &ctx->ties, &ctx->bm_image, &ctx->bm_wins, &ctx->bm_wmeta, &ctx->erec, &ctx->gflag, &ctx->fmask,
-	                  &ctx->lp_idx, &ctx->stat64, &ctx->lastf, &ctx->fidx, &ctx->geo, &ctx->scan_tmp,
-	                  &ctx->scan_tmp2, &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wq, &ctx->mlp_inf, &ctx->mlp_ws,
-	                  &ctx->mlp_grad, &ctx->mlp_xb, &ctx->mlp_tb, &ctx->mlp_sse};
-	for (DevBuf *b : bufs) b->release();
-	DevBuf *tvb[] = {&ctx->tv_frames, &ctx->tv_pairs, &ctx->tv_part, &ctx->tv_mm, &ctx->tv_state,
-	                 &ctx->tv_work, &ctx->tv_pyr, &ctx->tv_taps};
-	for (DevBuf *b : tvb) b->release();
+	for (DevBuf *b : ctx->buffers()) b->release();
 	if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
 	if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
 	if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1616,6 +1683,8 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		} else if (k == "coef_budget") {
 			if (value < 0) throw ArgError("coef_budget must be >= 0");
 			ctx->coef_budget = (size_t)value;
+		} else if (k == "keep_tables") {
+			ctx->keep_tables = value != 0;
 		} else if (k == "agg_cap") {
 			if (value != 512 && value != 768 && value != 1024 && value != 1536)
 				throw ArgError("agg_cap must be 512, 768, 1024 or 1536");
@@ -1635,6 +1704,7 @@ int bm4dof_run_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params
 	return guarded(ctx, [&] {
 		run_dev(ctx, p1, p2, noisy, (int)w, (int)h, (int)f, (int)c, (ff && bf) ? ff : nullptr,
 		        (ff && bf) ? bf : nullptr, basic, final, nparts, groups_ratio);
+		end_run(ctx);
 	});
 }
 
@@ -1664,6 +1734,8 @@ int bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_pa
 	*out = nullptr;
 	bm4dof_batch *b = new bm4dof_batch;
 	b->ctx = ctx;
+	b->flags = flags;
+	b->nparts = nparts;
 	const int rc = guarded(ctx, [&] {
 		HIPCHK(hipSetDevice(ctx->device));
 		if (nparts < 1 || nparts > 255) throw ArgError("nparts must be in 1..255");
@@ -1712,22 +1784,7 @@ int bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_pa
 					b->choice[i] = it->second;
 			}
 		}
-		// every buffer the launches use exists now: nothing allocates below
-		HIPCHK(hipStreamSynchronize(ctx->stream));
-		if (flags & BM4DOF_BATCH_GRAPH) {
-			HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-			try {
-				batch_enqueue(b);
-			} catch (...) {
-				hipGraph_t g = nullptr;
-				(void)hipStreamEndCapture(ctx->stream, &g);
-				if (g) (void)hipGraphDestroy(g);
-				throw;
-			}
-			HIPCHK(hipStreamEndCapture(ctx->stream, &b->graph));
-			HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-			b->moves_at_capture = DevBuf::moves.load();
-		}
+		if (flags & BM4DOF_BATCH_GRAPH) batch_capture(b);
 		apply_choice(ctx, saved);
 	});
 	if (rc) {
@@ -1746,21 +1803,60 @@ int bm4dof_batch_run(bm4dof_batch *b)
 	return guarded(ctx, [&] {
 		HIPCHK(hipSetDevice(ctx->device));
 		const TuneChoice saved = current_choice(ctx);
+		// the graph holds the addresses of the context's buffers at its
+		// capture: when other work on the context has moved one since, capture
+		// again
+		int path = 0;
+		if (b->flags & BM4DOF_BATCH_GRAPH) {
+			path = 1;
+			if (!b->exec || buffer_addresses(ctx) != b->captured_bufs) {
+				batch_capture(b);
+				path = 2;
+			}
+		}
 		hipEvent_t e0, e1;
 		HIPCHK(hipEventCreate(&e0));
 		HIPCHK(hipEventCreate(&e1));
 		HIPCHK(hipEventRecord(e0, ctx->stream));
-		// the graph holds the context buffers' addresses of its capture: when
-		// work on the context has moved any of them since, enqueue instead
-		if (b->exec && b->moves_at_capture == DevBuf::moves.load()) HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
+		if (path) HIPCHK(hipGraphLaunch(b->exec, ctx->stream));
 		else batch_enqueue(b);
+		// sequences where equal distances put another patch first in some
+		// group: the reference's processing-mask scan skips groups there,
+		// which the eager path replays; run them again that way
+		const size_t n = b->seqs.size();
+		std::vector<uint32_t> mv(2 * n);
+		HIPCHK(hipMemcpyAsync(mv.data(), b->moved.as<uint32_t>(2 * n), 8 * n, hipMemcpyDeviceToHost,
+		                      ctx->stream));
+		HIPCHK(hipStreamSynchronize(ctx->stream));
+		int reruns = 0;
+		for (size_t i = 0; i < n; i++) {
+			if (!mv[2 * i] && !mv[2 * i + 1]) continue;
+			const bm4dof_seq &q = b->seqs[i];
+			const bool flow = q.fflow && q.bflow;
+			apply_choice(ctx, b->choice[i]);
+			run_dev(ctx, &b->plans[2 * i].p, &b->plans[2 * i + 1].p, q.noisy, (int)q.w, (int)q.h, (int)q.f, 1,
+			        flow ? q.fflow : nullptr, flow ? q.bflow : nullptr, q.basic, q.final, b->nparts, nullptr);
+			reruns++;
+		}
 		HIPCHK(hipEventRecord(e1, ctx->stream));
 		HIPCHK(hipEventSynchronize(e1));
 		HIPCHK(hipEventElapsedTime(&b->last_ms, e0, e1));
 		(void)hipEventDestroy(e0);
 		(void)hipEventDestroy(e1);
+		b->last_path = path;
+		b->last_reruns = reruns;
 		apply_choice(ctx, saved);
 	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_batch_stats(const bm4dof_batch *b, int64_t *out, int max_slots)
+{
+	if (!b || !out) return 0;
+	const int64_t v[3] = {b->last_path, b->last_reruns, b->captures};
+	int k = 0;
+	for (; k < 3 && k < max_slots; k++) out[k] = v[k];
+	return k;
 }
 
 __attribute__((visibility("default")))
@@ -1777,6 +1873,7 @@ void bm4dof_batch_destroy(bm4dof_batch *b)
 	if (b->exec) (void)hipGraphExecDestroy(b->exec);
 	if (b->graph) (void)hipGraphDestroy(b->graph);
 	for (auto &g : b->geo) g.release();
+	b->moved.release();
 	delete b;
 }
 
@@ -1790,6 +1887,7 @@ int bm4dof_run_learned_dev(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4do
 	return guarded(ctx, [&] {
 		run_learned_dev(ctx, p1, p2, noisy, (int)w, (int)h, (int)f, (int)c, (ff && bf) ? ff : nullptr,
 		                (ff && bf) ? bf : nullptr, out, nparts, gvar_thr, flag_frac);
+		end_run(ctx);
 	});
 }
 
@@ -1818,6 +1916,7 @@ int bm4dof_run_learned(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_pa
 		                flag_frac);
 		HIPCHK(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, ctx->stream));
 		HIPCHK(hipStreamSynchronize(ctx->stream));
+		end_run(ctx);
 	});
 }
 
@@ -1867,6 +1966,7 @@ int bm4dof_run(bm4dof_ctx *ctx, const bm4dof_params *p1, const bm4dof_params *p2
 		HIPCHK(hipMemcpyAsync(basic, db, n * 4, hipMemcpyDeviceToHost, ctx->stream));
 		if (step != 1 && final) HIPCHK(hipMemcpyAsync(final, dfin, n * 4, hipMemcpyDeviceToHost, ctx->stream));
 		HIPCHK(hipStreamSynchronize(ctx->stream));
+		end_run(ctx);
 	});
 }
 

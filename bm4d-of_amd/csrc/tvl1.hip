@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <array>
 #include <stdexcept>
 
 namespace bm4dof {
@@ -444,14 +445,13 @@ TVPlan tv_plan(int P, int nxx, int nyy, int nscales, float zfactor)
 }
 
 // Gaussian smoothing of P images (in -> out, via tmp).
-void tv_gauss(bm4dof_ctx *ctx, const float *in, float *tmp, float *out, int nx, int ny, int P,
+void tv_gauss(bm4dof_ctx *ctx, double *dB, const float *in, float *tmp, float *out, int nx, int ny, int P,
               double sigma)
 {
 	std::vector<double> B;
 	const int size = gauss_taps(sigma, B);
 	if (size > nx) throw ArgError("GaussianSmooth: sigma too large");
 	if (size >= ny) throw ArgError("GaussianSmooth: sigma too large for the image height");
-	double *dB = ctx->tv_taps.as<double>(64);
 	if (size > 64) throw ArgError("GaussianSmooth: sigma too large");
 	HIPCHK(hipMemcpyAsync(dB, B.data(), size * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
 	const int n = nx * ny;
@@ -464,17 +464,80 @@ void tv_gauss(bm4dof_ctx *ctx, const float *in, float *tmp, float *out, int nx, 
 	HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
+// Device memory of one multiscale solve of P pairs: pair indices, the four
+// pyramids (I0, I1, u1, u2), 14 working planes at the finest size, min/max
+// partials, stopping-test partial sums, per-pair state and Gaussian taps.
+struct TVScratch {
+	int *pairs = nullptr;
+	float *pyr = nullptr, *work = nullptr, *mm = nullptr;
+	double *part = nullptr, *taps = nullptr;
+	int *state = nullptr;
+};
+constexpr int TV_MMB = 64;   // blocks of the min/max reduction per pair
+
+// Byte offsets of the parts of a TVScratch carved from one region, and the
+// region's size (the last entry), each part 256-byte aligned.
+std::array<size_t, 8> tv_scratch_layout(const TVPlan &pl, size_t size0)
+{
+	auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+	const size_t P = (size_t)pl.P;
+	std::array<size_t, 8> o{};
+	o[0] = 0;                                                          // pairs
+	o[1] = o[0] + al(2 * P * sizeof(int));                             // pyramids
+	o[2] = o[1] + al(pl.pyr * 4 * sizeof(float));                      // working planes
+	o[3] = o[2] + al(P * size0 * 14 * sizeof(float));                  // min/max
+	o[4] = o[3] + al(P * TV_MMB * 4 * sizeof(float));                  // partial sums
+	o[5] = o[4] + al(P * (size_t)tv_grid((int64_t)size0) * sizeof(double));
+	o[6] = o[5] + al((P + 1) * sizeof(int));                           // taps
+	o[7] = o[6] + al(64 * sizeof(double));                             // total
+	return o;
+}
+
+TVScratch tv_scratch_carve(char *base, const std::array<size_t, 8> &o)
+{
+	TVScratch t;
+	t.pairs = (int *)(base + o[0]);
+	t.pyr = (float *)(base + o[1]);
+	t.work = (float *)(base + o[2]);
+	t.mm = (float *)(base + o[3]);
+	t.part = (double *)(base + o[4]);
+	t.state = (int *)(base + o[5]);
+	t.taps = (double *)(base + o[6]);
+	return t;
+}
+
+TVScratch tv_scratch_ctx(bm4dof_ctx *ctx, const TVPlan &pl, size_t size0)
+{
+	const size_t P = (size_t)pl.P;
+	TVScratch t;
+	t.pairs = ctx->tv_pairs.as<int>(2 * P);
+	t.pyr = ctx->tv_pyr.as<float>(pl.pyr * 4);
+	t.work = ctx->tv_work.as<float>(P * size0 * 14);
+	t.mm = ctx->tv_mm.as<float>(P * TV_MMB * 4);
+	t.part = ctx->tv_part.as<double>(P * (size_t)tv_grid((int64_t)size0));
+	t.state = ctx->tv_state.as<int>(P + 1);
+	t.taps = ctx->tv_taps.as<double>(64);
+	return t;
+}
+
+void tv_check_params(const bm4dof_tvl1_params &prm)
+{
+	if (prm.nscales < 1 || prm.warps < 0 || prm.fscale < 0 || !(prm.zfactor > 0 && prm.zfactor < 1))
+		throw ArgError("invalid TV-L1 parameters");
+}
+
 // Multiscale TV-L1 of P pairs.  frames: device, nfr frames of nxx*nyy;
 // pair pr uses frames i0[pr] -> i1[pr].  Result in u1/u2 (P*nxx*nyy each,
-// device).  iters (host, may be NULL): P*nscales*warps ints.
+// device).  iters (host, may be NULL): P*nscales*warps ints.  Device
+// scratch from `fixed` when given (nothing is allocated then), else from the
+// context's buffers.
 void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *frames, int nxx,
                    int nyy, const std::vector<int> &i0, const std::vector<int> &i1, float *u1out,
-                   float *u2out, int *iters)
+                   float *u2out, int *iters, const TVScratch *fixed = nullptr)
 {
 	const int P = (int)i0.size();
 	if (P == 0) return;
-	if (prm.nscales < 1 || prm.warps < 0 || prm.fscale < 0 || !(prm.zfactor > 0 && prm.zfactor < 1))
-		throw ArgError("invalid TV-L1 parameters");
+	tv_check_params(prm);
 	const int nscales = prm.nscales;
 	TVPlan pl = tv_plan(P, nxx, nyy, nscales, prm.zfactor);
 	ctx->tv_pixel_iters = 0;
@@ -484,23 +547,23 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	const size_t size0 = (size_t)nxx * nyy;
 	hipStream_t st = ctx->stream;
 
-	int *d_pairs = ctx->tv_pairs.as<int>(2 * P);
+	const TVScratch sc = fixed ? *fixed : tv_scratch_ctx(ctx, pl, size0);
+	int *d_pairs = sc.pairs;
 	HIPCHK(hipMemcpyAsync(d_pairs, i0.data(), P * sizeof(int), hipMemcpyHostToDevice, st));
 	HIPCHK(hipMemcpyAsync(d_pairs + P, i1.data(), P * sizeof(int), hipMemcpyHostToDevice, st));
 	// pyramids: I0s, I1s, u1s, u2s
-	float *pyr = ctx->tv_pyr.as<float>(pl.pyr * 4);
+	float *pyr = sc.pyr;
 	float *I0s = pyr, *I1s = pyr + pl.pyr, *U1 = pyr + 2 * pl.pyr, *U2 = pyr + 3 * pl.pyr;
 	// working arrays at the finest size: tmp, tmp2, I1x, I1y, I1wx, I1wy, grad, rho_c, p11..p22
 	const size_t W = (size_t)P * size0;
-	float *work = ctx->tv_work.as<float>(W * 14);
+	float *work = sc.work;
 	float *tmp = work, *tmp2 = work + W, *I1x = work + 2 * W, *I1y = work + 3 * W,
 	      *I1wx = work + 4 * W, *I1wy = work + 5 * W, *grad = work + 6 * W, *rho_c = work + 7 * W,
 	      *p11 = work + 8 * W, *p12 = work + 9 * W, *p21 = work + 10 * W, *p22 = work + 11 * W;
-	const int mmb = 64;
-	float *d_mm = ctx->tv_mm.as<float>((size_t)P * mmb * 4);
-	const int nb_max = tv_grid((int64_t)size0);
-	double *d_part = ctx->tv_part.as<double>((size_t)P * nb_max);
-	int *d_state = ctx->tv_state.as<int>(P + 1);
+	const int mmb = TV_MMB;
+	float *d_mm = sc.mm;
+	double *d_part = sc.part;
+	int *d_state = sc.state;
 	int *d_nact = d_state + P;
 
 	// normalisation + presmoothing (tvl1flow_lib.c:381-386)
@@ -509,8 +572,8 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	hipLaunchKernelGGL(tv_normalize_kernel, dim3(tv_grid(size0), P), dim3(TV_THREADS), 0, st, frames,
 	                   (int)size0, d_pairs, d_pairs + P, d_mm, mmb, tmp2, tmp2 + W);
 	HIPCHK(hipGetLastError());
-	tv_gauss(ctx, tmp2, tmp, I0s + pl.off[0], nxx, nyy, P, TV_PRESMOOTHING_SIGMA);
-	tv_gauss(ctx, tmp2 + W, tmp, I1s + pl.off[0], nxx, nyy, P, TV_PRESMOOTHING_SIGMA);
+	tv_gauss(ctx, sc.taps, tmp2, tmp, I0s + pl.off[0], nxx, nyy, P, TV_PRESMOOTHING_SIGMA);
+	tv_gauss(ctx, sc.taps, tmp2 + W, tmp, I1s + pl.off[0], nxx, nyy, P, TV_PRESMOOTHING_SIGMA);
 	// pyramid (tvl1flow_lib.c:389-403; zoom_out, zoom.c:43-79)
 	const float zsig = TV_ZOOM_SIGMA_ZERO * sqrt(1.0 / (prm.zfactor * prm.zfactor) - 1.0);
 	for (int s = 1; s < nscales; s++) {
@@ -518,7 +581,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		for (int im = 0; im < 2; im++) {
 			float *src = (im ? I1s : I0s) + pl.off[s - 1];
 			float *dst = (im ? I1s : I0s) + pl.off[s];
-			tv_gauss(ctx, src, tmp, tmp2, nx, ny, P, zsig);
+			tv_gauss(ctx, sc.taps, src, tmp, tmp2, nx, ny, P, zsig);
 			hipLaunchKernelGGL(tv_zoom_out_kernel, dim3(tv_grid(pl.nx[s] * pl.ny[s]), P), dim3(TV_THREADS), 0,
 			                   st, tmp2, nx, ny, n, dst, pl.nx[s], pl.ny[s], pl.nx[s] * pl.ny[s], prm.zfactor);
 			HIPCHK(hipGetLastError());

@@ -123,10 +123,16 @@ const char *bm4dof_global_error(void);
  *                  hold in per-anchor coefficient tables (the forward
  *                  transform of every patch position, computed once per
  *                  image; step 1 one table, step 2 two, each C*W*H*F*64
- *                  floats; default 96 GiB).  Steps whose tables do not fit
- *                  the budget or the free device memory transform the
- *                  patches group by group; the output is the same either
- *                  way.  0 disables the tables.
+ *                  floats; default: the smaller of 96 GiB and 2/5 of the
+ *                  device memory free when the context was created).  Steps
+ *                  whose tables do not fit the budget or the free device
+ *                  memory transform the patches group by group; the output
+ *                  is the same either way.  0 disables the tables.
+ *   "keep_tables"  0 (default): the tables are freed at the end of every
+ *                  bm4dof_run / _run_dev / _run_learned(_dev) call; 1: they
+ *                  stay allocated for the next call on the context (saves
+ *                  their allocation when videos of one shape are denoised
+ *                  repeatedly).
  * Returns BM4DOF_EINVAL for an unknown key.
  */
 int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value);
@@ -208,19 +214,21 @@ int bm4dof_last_stats(const bm4dof_ctx *ctx, int64_t *stats, int max_slots);
  * noise level as one launch set: planned and uploaded at creation, then
  * either enqueued on every run or captured once into a hipGraph
  * (BM4DOF_BATCH_GRAPH) that every run replays.  Each sequence's output is
- * bitwise the output of bm4dof_run_dev on it alone (same nparts), except
- * where a run of identical patches puts a patch other than the reference
- * one first in a neighbour list: bm4dof_run_dev then replays the
- * reference's processing-mask scan (a host step between block matching and
- * filtering), which a batch does not.  With
- * BM4DOF_BATCH_AUTOTUNE the execution options that cannot change results
- * ("fast_bm", "bm_frames", "agg_cap") are timed on the first sequence of every shape
- * class and the fastest set is used for that class.  Buffers are device
- * pointers and must stay valid for the batch's lifetime; a captured batch
- * holds its scratch memory in the context, so runs on that context of other
- * work in between are allowed but not concurrently.  When such work has
- * grown (moved) the context's buffers since the capture, the batch enqueues
- * its launches instead of replaying the graph.
+ * bitwise the output of bm4dof_run_dev on it alone (same nparts).  Where
+ * equal distances put a patch other than the reference one first in a
+ * neighbour list, the reference's processing-mask scan skips groups
+ * (src/core/nldct.cpp:2325-2329); the launch set counts such groups on the
+ * device for every sequence, and bm4dof_batch_run then runs the sequences
+ * that have any once more through the path of bm4dof_run_dev, which replays
+ * that scan.  With BM4DOF_BATCH_AUTOTUNE the execution options that cannot
+ * change results ("fast_bm", "bm_frames", "agg_cap") are timed on the first
+ * sequence of every shape class and the fastest set is used for that class.
+ * Buffers are device pointers and must stay valid for the batch's lifetime;
+ * a captured batch holds its scratch memory in the context, so runs on that
+ * context of other work in between are allowed but not concurrently.  The
+ * graph holds the context buffers' addresses of its capture: when such work
+ * has moved one of them, the next bm4dof_batch_run captures the launch set
+ * again before replaying it.
  */
 typedef struct bm4dof_seq {
 	const float *noisy;          /* w*h*f, one channel */
@@ -236,6 +244,11 @@ int   bm4dof_batch_create(bm4dof_ctx *ctx, const bm4dof_seq *seqs, const bm4dof_
 int   bm4dof_batch_run(bm4dof_batch *b);
 float bm4dof_batch_last_ms(const bm4dof_batch *b);
 const char *bm4dof_batch_report(const bm4dof_batch *b);
+/* Statistics of the last bm4dof_batch_run: out[0] = how the launch set ran
+ * (0 enqueued, 1 graph replayed, 2 graph captured again and replayed),
+ * out[1] = sequences run again for the processing-mask scan, out[2] =
+ * captures so far.  Returns the number of slots filled. */
+int   bm4dof_batch_stats(const bm4dof_batch *b, int64_t *out, int max_slots);
 void  bm4dof_batch_destroy(bm4dof_batch *b);
 /* Times both steps on one sequence (device buffers, one channel) under every
  * candidate option set, keeps the fastest in ctx, and writes a report. */

@@ -91,32 +91,83 @@ def test_batch_equals_single_sequences(ctx, graph):
     assert not refs[4][1].any()
 
 
-def test_graph_batch_after_buffers_grow(ctx):
+def test_graph_batch_after_buffers_grow():
     """A captured batch holds the context buffers' addresses; a run of a
     larger video on the same context between replays makes the context
-    grow (move) them.  The next runs must still give the single-sequence
-    outputs (the batch then enqueues instead of replaying)."""
-    seqs = _make(seed0=500)[:3]
-    refs = [_single(ctx, d) for d in seqs]
-    batch = ctx.batch([_seq(d) for d in seqs], [d["p1"] for d in seqs], [d["p2"] for d in seqs],
-                      nparts=2, graph=True)
+    grow (move) them.  The next run captures the launch set again and still
+    gives the single-sequence outputs.  A fresh context, so that the
+    buffers really grow."""
+    ctx = A.Context(0)
     try:
+        seqs = _make(seed0=500)[:3]
+        refs = [_single(ctx, d) for d in seqs]
+        batch = ctx.batch([_seq(d) for d in seqs], [d["p1"] for d in seqs], [d["p2"] for d in seqs],
+                          nparts=2, graph=True)
         clean = synth.video(160, 120, 10, seed=77)
         big = {"shape": (10, 1, 120, 160), "noisy": torch.from_numpy(synth.add_noise(clean, 20.0, 78)).cuda(),
                "ff": None, "bf": None,
                "p1": A.cli_params(1, 20.0, (10, 1, 120, 160), px=4, pt=4, wx=9, np_=16),
                "p2": A.cli_params(2, 20.0, (10, 1, 120, 160), px=4, pt=4, wx=9, np_=16)}
-        for step in range(2):
-            if step:
-                _single(ctx, big)
+        try:
+            for step in range(3):
+                if step == 1:
+                    _single(ctx, big)
+                for d in seqs:
+                    d["basic"].zero_()
+                    d["final"].zero_()
+                torch.cuda.synchronize()
+                batch.run()
+                st = batch.stats()
+                # replayed as captured; captured again after the growth; then replayed
+                assert st["path"] == (2 if step == 1 else 1), (step, st)
+                assert st["captures"] == (1 if step == 0 else 2), (step, st)
+                for i, (d, (rb, rf)) in enumerate(zip(seqs, refs)):
+                    assert np.array_equal(d["basic"].cpu().numpy(), rb), (step, i)
+                    assert np.array_equal(d["final"].cpu().numpy(), rf), (step, i)
+        finally:
+            batch.close()
+    finally:
+        ctx.close()
+
+
+def test_batch_with_equal_distances_runs_the_mask_scan(ctx):
+    """Integer-valued sequences with constant regions: equal distances in
+    heap order put another patch first in some groups, and the reference's
+    processing-mask scan skips positions (nldct.cpp:2325-2329).  The launch
+    set flags those sequences and the run finishes them through the eager
+    path: bitwise equal to single runs and to the CPU oracle."""
+    import oracle as O
+    seqs = []
+    for i, (w, h, f) in enumerate([(56, 40, 7), (64, 48, 6)]):
+        clean = synth.video(w, h, f, seed=300 + i)
+        noisy = np.clip(np.round(synth.add_noise(clean, 20.0, 301 + i)), 0, 255).astype(np.float32)
+        if i == 0:
+            noisy[:, :, :20, :24] = 50.0
+        d = {"shape": noisy.shape, "np": noisy,
+             "noisy": torch.from_numpy(noisy).cuda(),
+             "basic": torch.zeros(noisy.shape, dtype=torch.float32, device="cuda"),
+             "final": torch.zeros(noisy.shape, dtype=torch.float32, device="cuda"), "ff": None, "bf": None,
+             "p1": A.cli_params(1, 20.0, noisy.shape, px=4, pt=4, wx=9, np_=16),
+             "p2": A.cli_params(2, 20.0, noisy.shape, px=4, pt=4, wx=9, np_=16)}
+        seqs.append(d)
+    refs = [_single(ctx, d) for d in seqs]
+    o1 = O.cli_params(1, 20.0, 7, px=4, pt=4, wx=9, np_=16)
+    o2 = O.cli_params(2, 20.0, 7, px=4, pt=4, wx=9, np_=16)
+    ob, of_, _ = O.run(o1, o2, seqs[0]["np"], nparts=2)
+    assert np.array_equal(refs[0][0], ob) and np.array_equal(refs[0][1], of_, equal_nan=True)
+    batch = ctx.batch([_seq(d) for d in seqs], [d["p1"] for d in seqs], [d["p2"] for d in seqs],
+                      nparts=2, graph=True)
+    try:
+        for replay in range(2):
             for d in seqs:
                 d["basic"].zero_()
                 d["final"].zero_()
             torch.cuda.synchronize()
             batch.run()
+            assert batch.stats()["reruns"] >= 1
             for i, (d, (rb, rf)) in enumerate(zip(seqs, refs)):
-                assert np.array_equal(d["basic"].cpu().numpy(), rb), (step, i)
-                assert np.array_equal(d["final"].cpu().numpy(), rf), (step, i)
+                assert np.array_equal(d["basic"].cpu().numpy(), rb), (replay, i)
+                assert np.array_equal(d["final"].cpu().numpy(), rf, equal_nan=True), (replay, i)
     finally:
         batch.close()
 

@@ -232,6 +232,28 @@ def test_gray_read_truncates_integer_samples(tmp_path):
         A.read_gray(q)
 
 
+def test_png_header_checks(tmp_path):
+    """A short IHDR chunk at the end of the file and sizes of 2^32 samples
+    or more are errors, not reads past the buffer or huge allocations."""
+    import struct
+    import zlib
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d))
+    sig = b"\x89PNG\r\n\x1a\n"
+    short = sig + chunk(b"IDAT", b"x" * 16) + chunk(b"IHDR", b"\x00\x00\x00\x10\x00")
+    p = tmp_path / "short.png"
+    p.write_bytes(short)
+    with pytest.raises(OSError, match="IHDR"):
+        A.read_image(p)
+    big = sig + chunk(b"IHDR", struct.pack(">IIBBBBB", 70000, 70000, 8, 6, 0, 0, 0)) + \
+        chunk(b"IDAT", zlib.compress(b"\x00" * 16)) + chunk(b"IEND", b"")
+    p = tmp_path / "big.png"
+    p.write_bytes(big)
+    with pytest.raises(OSError, match="too large"):
+        A.read_image(p)
+
+
 def test_read_errors(tmp_path):
     with pytest.raises(OSError):
         A.read_image(tmp_path / "missing.png")
