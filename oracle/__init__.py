@@ -80,6 +80,7 @@ def lib():
         L.or_tiles.argtypes = [C.c_uint, C.c_uint, C.c_int, C.c_int, _i32p]
         L.or_partial_sort.argtypes = [_f32p, _u32p, C.c_long, C.c_long, C.c_int]
         L.or_bases.argtypes = [P, _f32p, _f32p, _f32p]
+        L.or_set_predictive.argtypes = [C.c_int]
         L.or_filter_group.argtypes = [P, C.c_int, C.c_int, _f32p, C.c_void_p, C.POINTER(C.c_float)]
         L.or_run.argtypes = [P, P, _f32p, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
                              C.c_void_p, C.c_void_p, _f32p, C.c_void_p, C.c_int, C.c_int,
@@ -216,6 +217,13 @@ def bm(p, image, fflow=None, bflow=None, nparts=2, tie_mode=0):
     L.or_bm(C.byref(p), image, W, H, F, Cc, fa, ba, nparts, tie_mode, idx.ctypes.data,
             nsim.ctypes.data, ref.ctypes.data, tile.ctypes.data)
     return idx.reshape(n, npm), nsim, ref, tile
+
+
+def set_predictive(on):
+    """The reference's VBM3D_SEARCH predictive search in block matching
+    (nldct.cpp:904-1130, 1368-1589) on (1) or off (0, the reference's
+    default build)."""
+    lib().or_set_predictive(1 if on else 0)
 
 
 def bases(p):

@@ -530,6 +530,152 @@ static inline int imax(int a, int b) { return a > b ? a : b; }
 /* step: 1 -> distance on channel 0 of `im`; 2 -> all channels of `im`
  * (the basic image).  Writes index[0..nSimP) (tile-local reference indices,
  * t*whc + y*w + x) and returns nSimP. */
+/* The reference's predictive search, compiled in with VBM3D_SEARCH
+ * (src/core/nldct.h:20): estimateSimilarPatchesStep1 src/core/nldct.cpp:904-1130,
+ * estimateSimilarPatchesStep2 src/core/nldct.cpp:1368-1589.  The reference
+ * frame is searched in the full window; every other frame, in the union of
+ * nSimilarPatchesPred windows of sizeSearchWindowPred around the
+ * flow-propagated positions of the previous frame's best matches (insertion
+ * order: window by window, each in raster order, positions already inserted
+ * skipped, as the unordered_set test of :1044 does).  Distances on channel 0
+ * (step 2: the channel-0 sum repeated once per channel, since the step-2
+ * loop reads imBasic(x, y, t) without its channel, :1532-1535), minus
+ * dsub^2*255 at the reference's (x, y) in every frame (:1049, :1539); the
+ * nSimilarPatchesPred best of each frame by std::partial_sort are kept, and
+ * the kept ones of all frames sorted again for the group (:1076, :1102).
+ * Enabled by or_set_predictive(1). */
+static int g_predictive = 0;
+OR_EXPORT void or_set_predictive(int on) { g_predictive = on; }
+
+/* The reference keeps nSimilarPatchesPred candidates of every frame's
+ * search region (std::partial_sort to that middle, :1076); a region smaller
+ * than that is undefined behaviour there and an error here. */
+static int pred_check(const bm4dof_params *p, unsigned w, unsigned h, int nparts)
+{
+	if (!g_predictive || p->sizePatch == 0 || p->nSimilarPatches <= 1) return 0;
+	/* the smallest tile bounds the search regions */
+	int *tl = (int *)malloc((size_t)8 * (nparts > 0 ? nparts : 1) * sizeof(int));
+	if (or_tiles(w, h, (int)p->boundary, nparts, tl)) { free(tl); return -1; }
+	for (int n = 0; n < nparts; n++) {
+		const unsigned tw = (unsigned)(tl[8 * n + 2] - tl[8 * n]), th = (unsigned)(tl[8 * n + 3] - tl[8 * n + 1]);
+		if (tw < w) w = tw;
+		if (th < h) h = th;
+	}
+	free(tl);
+	const unsigned cx = w >= p->sizePatch ? w - p->sizePatch + 1 : 0;
+	const unsigned cy = h >= p->sizePatch ? h - p->sizePatch + 1 : 0;
+	const unsigned ws[2] = {p->sizeSearchWindow, p->sizeSearchWindowPred};
+	for (int k = 0; k < 2; k++) {
+		const unsigned a = ws[k] < cx ? ws[k] : cx, b = ws[k] < cy ? ws[k] : cy;
+		if (p->nSimilarPatchesPred < 1 || (size_t)a * b < p->nSimilarPatchesPred)
+			return fail("predictive search: a search region holds fewer than nSimilarPatchesPred patches");
+	}
+	return 0;
+}
+
+static unsigned bm_search_pred(const vid_t *im, const vid_t *fflow, const vid_t *bflow,
+                               unsigned pidx, const bm4dof_params *prm, int step, unsigned *index)
+{
+	const int sPx = (int)prm->sizePatch, sPt = (int)prm->sizePatchTime;
+	if (prm->nSimilarPatches <= 1) { index[0] = pidx; return 1; }
+	const int npf = (int)prm->nSimilarPatchesPred;
+	const float dsub = prm->dsub * prm->dsub * 255;
+	const float tau = prm->tau * prm->tau * sPx * sPx * sPt;
+	const int use_flow = fflow && fflow->w > 0;
+	const int sWt_f = (int)prm->sizeSearchTimeRangeFwd, sWt_b = (int)prm->sizeSearchTimeRangeBwd;
+	const int W = (int)im->w, H = (int)im->h, F = (int)im->f;
+	const int px = (int)(pidx % im->w), py = (int)((pidx % im->wh) / im->w), pt = (int)(pidx / im->whc);
+	const int shift_t = imin(0, pt - sWt_b) + imax(0, pt + sWt_f - F + sPt);
+	const int rt0 = imax(0, pt - sWt_b - shift_t), rt1 = imin(F - sPt, pt + sWt_f - shift_t);
+	const int sWt = rt1 - rt0 + 1;
+	int sched[64];
+	int ns = 0;
+	sched[ns++] = pt;
+	for (int qt = pt + 1; qt <= rt1; ++qt) sched[ns++] = qt;
+	for (int qt = pt - 1; qt >= rt0; --qt) sched[ns++] = qt;
+	const int wfull = (int)prm->sizeSearchWindow, wpred = (int)prm->sizeSearchWindowPred;
+	const int cap = imax(wfull * wfull, npf * wpred * wpred) + 1;
+	dpair *cand = (dpair *)malloc((size_t)cap * sizeof(dpair));
+	dpair *kept = (dpair *)malloc((size_t)(npf * sWt + 1) * sizeof(dpair));
+	int *tx = (int *)calloc((size_t)npf * sWt, sizeof(int)), *ty = (int *)calloc((size_t)npf * sWt, sizeof(int));
+	int *rx0 = (int *)malloc(npf * sizeof(int)), *rx1 = (int *)malloc(npf * sizeof(int));
+	int *ry0 = (int *)malloc(npf * sizeof(int)), *ry1 = (int *)malloc(npf * sizeof(int));
+	const int reps = step == 1 ? 1 : (int)im->c;
+	int nk = 0;
+	for (int ii = 0; ii < sWt; ++ii) {
+		const int qt = sched[ii], dt = qt - rt0;
+		const int dir = imax(-1, imin(1, qt - pt));
+		int nwin, sw;
+		int cx0[64], cy0[64];
+		if (dir != 0) {
+			sw = wpred;
+			nwin = npf;
+			for (int i = 0; i < npf; i++) {
+				const int cx = tx[i * sWt + dt - dir], cy = ty[i * sWt + dt - dir], ct = qt - dir;
+				float fx = 0.f, fy = 0.f;
+				if (use_flow) {
+					const vid_t *fl = dir > 0 ? fflow : bflow;
+					fx = fl->d[vidx(fl, cx, cy, ct, 0)];
+					fy = fl->d[vidx(fl, cx, cy, ct, 1)];
+				}
+				cx0[i] = (int)fmax_ref(0.f, fmin_ref((float)W - 1.f, roundf((float)cx + fx)));
+				cy0[i] = (int)fmax_ref(0.f, fmin_ref((float)H - 1.f, roundf((float)cy + fy)));
+			}
+		} else {
+			sw = wfull;
+			nwin = 1;
+			cx0[0] = px;
+			cy0[0] = py;
+		}
+		int nc = 0;
+		for (int i = 0; i < nwin; i++) {
+			int shx = imin(0, cx0[i] - (sw - 1) / 2), shy = imin(0, cy0[i] - (sw - 1) / 2);
+			shx += imax(0, cx0[i] + (sw - 1) / 2 - W + sPx);
+			shy += imax(0, cy0[i] + (sw - 1) / 2 - H + sPx);
+			rx0[i] = imax(0, cx0[i] - (sw - 1) / 2 - shx);
+			ry0[i] = imax(0, cy0[i] - (sw - 1) / 2 - shy);
+			rx1[i] = imin(W - sPx, cx0[i] + (sw - 1) / 2 - shx);
+			ry1[i] = imin(H - sPx, cy0[i] + (sw - 1) / 2 - shy);
+			for (int qy = ry0[i]; qy <= ry1[i]; qy++)
+				for (int qx = rx0[i]; qx <= rx1[i]; qx++) {
+					int seen = 0;
+					for (int j = 0; j < i && !seen; j++)
+						seen = qx >= rx0[j] && qx <= rx1[j] && qy >= ry0[j] && qy <= ry1[j];
+					if (seen) continue;
+					float d = 0.f, dif;
+					for (int r = 0; r < reps; r++)
+						for (int ht = 0; ht < sPt; ht++)
+							for (int hy = 0; hy < sPx; hy++)
+								for (int hx = 0; hx < sPx; hx++) {
+									dif = im->d[vidx(im, px + hx, py + hy, pt + ht, 0)] -
+									      im->d[vidx(im, qx + hx, qy + hy, qt + ht, 0)];
+									d += dif * dif;
+								}
+					d -= (qy == py && qx == px) ? dsub : 0;
+					cand[nc].d = d;
+					cand[nc].i = (uint32_t)vidx(im, qx, qy, qt, 0);
+					cand[nc].rank = nc;
+					nc++;
+				}
+		}
+		partial_sort_libstdcxx(cand, npf, nc);
+		for (int i = 0; i < npf; i++) {
+			kept[nk] = cand[i];
+			kept[nk].rank = nk;
+			nk++;
+			tx[i * sWt + dt] = (int)(cand[i].i % im->w);
+			ty[i * sWt + dt] = (int)((cand[i].i % im->wh) / im->w);
+		}
+	}
+	unsigned nSimP = umin(prm->nSimilarPatches, (unsigned)nk);
+	partial_sort_libstdcxx(kept, nSimP, nk);
+	unsigned n = 1;
+	while (2 * n <= nSimP && kept[n].d < tau) n *= 2;
+	for (unsigned k = 0; k < n; k++) index[k] = kept[k].i;
+	free(cand); free(kept); free(tx); free(ty); free(rx0); free(rx1); free(ry0); free(ry1);
+	return n;
+}
+
 static unsigned bm_search(const vid_t *im, const vid_t *fflow, const vid_t *bflow,
                           unsigned pidx, const bm4dof_params *prm, int step,
                           int tie_mode, dpair *dist, unsigned *index)
@@ -545,6 +691,7 @@ static unsigned bm_search(const vid_t *im, const vid_t *fflow, const vid_t *bflo
 		const unsigned px = (unsigned)(pidx % im->w);
 		const unsigned py = (unsigned)((pidx % im->wh) / im->w);
 		const unsigned pt = (unsigned)(pidx / im->whc);
+		if (g_predictive) return bm_search_pred(im, fflow, bflow, pidx, prm, step, index);
 
 		int shift_t = imin(0, (int)pt - sWt_b) + imax(0, (int)pt + sWt_f - F + sPt);
 		int rt0 = imax(0, (int)pt - sWt_b - shift_t);
@@ -1023,6 +1170,7 @@ OR_EXPORT int or_run(const bm4dof_params *p1, const bm4dof_params *p2,
 	const int step = p1->sizePatch ? (p2->sizePatch ? 0 : 1) : (p2->sizePatch ? 2 : -1);
 	if (step == -1) return fail("VideoNLB::runNlBayes: Both patch sizes are zero.");
 	if (nparts < 1) return fail("nparts must be >= 1");
+	if ((step != 2 && pred_check(p1, w, h, nparts)) || (step != 1 && pred_check(p2, w, h, nparts))) return -1;
 	if ((step != 2 && p1->doPasteBoost) || (step != 1 && p2->doPasteBoost))
 		return fail("doPasteBoost is not supported (the reference never enables it)");
 	if ((step != 2 && (p1->sizeSearchTimeRangeFwd + p1->sizeSearchTimeRangeBwd + 1 > 64 ||
@@ -1089,6 +1237,7 @@ OR_EXPORT long or_bm(const bm4dof_params *prm, const float *image, unsigned w, u
                      int nparts, int tie_mode, uint32_t *out_index, uint32_t *out_nsim,
                      uint32_t *out_ref, uint32_t *out_tile)
 {
+	if (pred_check(prm, w, h, nparts)) return -1;
 	int *tl = (int *)malloc((size_t)8 * nparts * sizeof(int));
 	if (or_tiles(w, h, (int)prm->boundary, nparts, tl)) { free(tl); return -1; }
 	vid_t full, ff = {0}, fb = {0};

@@ -164,8 +164,10 @@ class Mini:
     weight, groups skipped by the mask clear, and the ambiguity counters.
     """
 
-    def __init__(self, p1, p2, bior_bases=None, mistake=None):
+    def __init__(self, p1, p2, bior_bases=None, mistake=None, predictive=False):
         self.p = {1: p1, 2: p2}
+        # the reference's VBM3D_SEARCH build (nldct.h:20)
+        self.predictive = predictive
         # a planted misreading, for the tests that show the comparison with
         # the oracle detects it (None in every other use)
         self.mistake = mistake
@@ -219,6 +221,8 @@ class Mini:
         px, py, pt = ref
         if p.nSimilarPatches <= 1:
             return [(px, py, pt)]
+        if self.predictive:
+            return self.match_pred(img, ff, bf, p, ref, len(channels))
         wf, wb = p.sizeSearchTimeRangeFwd, p.sizeSearchTimeRangeBwd
         sW = p.sizeSearchWindow
         # non-centred temporal range (nldct.cpp:1178-1186)
@@ -283,6 +287,83 @@ class Mini:
         while 2 * n <= nsim and d32[sel[n]] < tau:
             n *= 2
         return [cand[i] for i in sel[:n]]
+
+    # ---- predictive search of a VBM3D_SEARCH build, nldct.cpp:904-1130, 1368-1589
+    def match_pred(self, img, ff, bf, p, ref, reps):
+        """Per frame, in the schedule pt, pt+1.., pt-1..: the full window in
+        the reference frame, else the union (in insertion order, as an
+        std::unordered_set insertion test lets through) of npf windows of
+        sizeSearchWindowPred around the flow-advanced positions of the
+        previous frame's npf best; distances on channel 0, the sum repeated
+        `reps` times (step 2 reads imBasic(x, y, t) inside its channel loop),
+        minus dsub^2 * 255 at the reference's (x, y) in any frame; the npf
+        best of each frame kept (std::partial_sort), then all kept sorted
+        again and cut at a power of two below tau."""
+        F, Cc, h, w = img.shape
+        sPx, sPt = p.sizePatch, p.sizePatchTime
+        px, py, pt = ref
+        npf = p.nSimilarPatchesPred
+        f32 = np.float32
+        dsub = f32(f32(p.dsub) * f32(p.dsub)) * f32(255)
+        wf, wb = p.sizeSearchTimeRangeFwd, p.sizeSearchTimeRangeBwd
+        sh = min(0, pt - wb) + max(0, pt + wf - F + sPt)
+        t0, t1 = max(0, pt - wb - sh), min(F - sPt, pt + wf - sh)
+        order = [pt] + list(range(pt + 1, t1 + 1)) + list(range(pt - 1, t0 - 1, -1))
+        # the planes a distance runs over, in order: channel 0, once per
+        # repetition (a planted misreading reads every channel instead)
+        chans = list(range(Cc)) if self.mistake == "pred_all_channels" and reps > 1 else [0] * reps
+        planes = [img[:, c].astype(np.float32) for c in chans]
+        refp = np.concatenate([q[pt:pt + sPt, py:py + sPx, px:px + sPx].ravel() for q in planes])
+        traj = {}
+        kept_d, kept_c = [], []
+        for qt in order:
+            if qt == pt:
+                centres, sw = [(px, py)], p.sizeSearchWindow
+            else:
+                prev = qt - 1 if qt > pt else qt + 1
+                fl = ff if qt > pt else bf
+                centres, sw = [], p.sizeSearchWindowPred
+                for (cx, cy) in traj[prev]:
+                    ux = fl[prev, 0, cy, cx] if fl is not None else 0.0
+                    uy = fl[prev, 1, cy, cx] if fl is not None else 0.0
+                    rx = np.float32(np.float32(cx) + np.float32(ux))
+                    ry = np.float32(np.float32(cy) + np.float32(uy))
+                    rx = np.sign(rx) * np.floor(np.abs(rx) + 0.5)
+                    ry = np.sign(ry) * np.floor(np.abs(ry) + 0.5)
+                    centres.append((int(min(max(rx, 0), w - 1)), int(min(max(ry, 0), h - 1))))
+            seen, cand, pats = set(), [], []
+            r = (sw - 1) // 2
+            for (cx, cy) in centres:
+                shx = min(0, cx - r) + max(0, cx + r - w + sPx)
+                shy = min(0, cy - r) + max(0, cy + r - h + sPx)
+                for qy in range(max(0, cy - r - shy), min(h - sPx, cy + r - shy) + 1):
+                    for qx in range(max(0, cx - r - shx), min(w - sPx, cx + r - shx) + 1):
+                        if (qx, qy) in seen:
+                            continue
+                        seen.add((qx, qy))
+                        cand.append((qx, qy, qt))
+                        pats.append(np.concatenate([q[qt:qt + sPt, qy:qy + sPx, qx:qx + sPx].ravel()
+                                                    for q in planes]))
+            dif = refp[None, :] - np.stack(pats)          # float32
+            sq = dif * dif
+            d = np.zeros(len(cand), np.float32)
+            for j in range(sq.shape[1]):
+                d = d + sq[:, j]
+            bias = np.array([qx == px and qy == py and (qt == pt or self.mistake != "pred_bias_reference_frame")
+                             for (qx, qy, _) in cand])
+            d = np.where(bias, d - dsub, d).astype(np.float32)
+            sel = partial_sort_order(d, npf)
+            traj[qt] = [(cand[i][0], cand[i][1]) for i in sel]
+            kept_d += [d[i] for i in sel]
+            kept_c += [cand[i] for i in sel]
+        kd = np.array(kept_d, np.float32)
+        nsim = min(p.nSimilarPatches, len(kd))
+        sel = partial_sort_order(kd, nsim)
+        tau = np.float32(p.tau) * np.float32(p.tau) * sPx * sPx * sPt
+        n = 1
+        while 2 * n <= nsim and kd[sel[n]] < tau:
+            n *= 2
+        return [kept_c[i] for i in sel[:n]]
 
     # ---- filters: computeBayesEstimateStep{1,2}, nldct.cpp:1799-2242
     def filter1(self, groups, p, B, Bi):

@@ -284,3 +284,81 @@ def test_bior_step1_image():
     assert p1.transform == 1 and p2.transform == 0
     B, Bi, _ = O.bases(p1)
     _compare_run(p1, p2, v, nparts=2, bior=(B, Bi))
+
+
+# ------------------------------------------- predictive search (VBM3D_SEARCH)
+
+def _pred_params(step, F, **kw):
+    p = _params(step, F, wx=7, **kw)
+    p.sizeSearchWindowPred = 3
+    p.nSimilarPatchesPred = 3
+    p.dsub = 3.0 if step == 1 else 2.0
+    return p
+
+
+@pytest.fixture
+def predictive():
+    O.set_predictive(True)
+    yield
+    O.set_predictive(False)
+
+
+@pytest.mark.parametrize("nparts", [1, 4])
+def test_predictive_search_neighbour_lists(predictive, nparts):
+    """The reference's VBM3D_SEARCH block matching (nldct.cpp:904-1130,
+    1368-1589): full window in the reference frame, unions of small windows
+    around the flow-advanced best matches of the previous frame elsewhere,
+    the distance bias at the reference's position, per-frame and final
+    std::partial_sort; step 2 sums channel 0 once per channel."""
+    v = _int_video(30, 26, 9, c=3, seed=21, hi=127)
+    F, _, H, W = v.shape
+    ff, bf = _flows(F, H, W, seed=22, push=2.0)
+    for step in (1, 2):
+        p = _pred_params(step, F)
+        idx, nsim, ref, tile = O.bm(p, v, ff, bf, nparts=nparts, tie_mode=0)
+        m = M.Mini(p, p, predictive=True)
+        groups = m.bm_all(step, v, ff, bf, nparts=nparts)
+        assert len(groups) == len(ref) > 50
+        bad = [g for g, (ti, r, lst) in enumerate(groups)
+               if ti != tile[g] or r != ref[g] or list(idx[g, :nsim[g]]) != lst]
+        assert not bad, (step, len(bad))
+    # the lists differ from the exhaustive search
+    O.set_predictive(False)
+    idx2, nsim2, _, _ = O.bm(_pred_params(1, F), v, ff, bf, nparts=nparts)
+    O.set_predictive(True)
+    idx1, nsim1, _, _ = O.bm(_pred_params(1, F), v, ff, bf, nparts=nparts)
+    assert not (np.array_equal(idx1, idx2) and np.array_equal(nsim1, nsim2))
+
+
+@pytest.mark.parametrize("mistake", ["pred_bias_reference_frame", "pred_all_channels"])
+def test_predictive_pins_are_sensitive(predictive, mistake):
+    v = _int_video(30, 26, 9, c=3, seed=21, hi=127)
+    F, _, H, W = v.shape
+    ff, bf = _flows(F, H, W, seed=22, push=2.0)
+    step = 2 if mistake == "pred_all_channels" else 1
+    p = _pred_params(step, F)
+    idx, nsim, ref, tile = O.bm(p, v, ff, bf, nparts=1, tie_mode=0)
+    groups = M.Mini(p, p, predictive=True, mistake=mistake).bm_all(step, v, ff, bf, nparts=1)
+    assert any(list(idx[g, :nsim[g]]) != lst for g, (_, _, lst) in enumerate(groups))
+
+
+def test_predictive_two_steps(predictive):
+    v = _int_video(30, 26, 9, seed=23)
+    F, _, H, W = v.shape
+    ff, bf = _flows(F, H, W, seed=24, push=2.0)
+    p1, p2 = _pred_params(1, F), _pred_params(2, F)
+    ob, of_, orat = O.run(p1, p2, v, ff, bf, nparts=2)
+    none = O.cli_params(2, SIG, F, px=0)
+    mb, _, r1 = M.Mini(p1, none, predictive=True).run(v, ff, bf, nparts=2)
+    _, mf, r2 = M.Mini(O.cli_params(1, SIG, F, px=0), p2, predictive=True).run(v, ff, bf, basic=ob,
+                                                                                nparts=2)
+    assert _maxdiff(mb, ob) < 1e-3 and _maxdiff(mf, of_) < 1e-3
+    assert abs(r1[0] - orat[0]) < 1e-6 * orat[0] and abs(r2[1] - orat[1]) < 1e-6 * orat[1]
+
+
+def test_predictive_rejects_regions_smaller_than_the_kept_count(predictive):
+    v = _int_video(30, 26, 9, seed=23)
+    p1 = _pred_params(1, 9)
+    p1.nSimilarPatchesPred = 10   # more than a 3x3 predictive window holds
+    with pytest.raises(RuntimeError, match="nSimilarPatchesPred"):
+        O.run(p1, O.cli_params(2, SIG, 9, px=0), v, nparts=2)
