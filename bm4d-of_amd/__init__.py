@@ -79,7 +79,7 @@ EXPORTS = [
     "bm4dof_mlp_forward", "bm4dof_mlp_grad", "bm4dof_mlp_train", "bm4dof_learned_pairs",
     "bm4dof_run_learned", "bm4dof_run_learned_dev", "bm4dof_batch_create", "bm4dof_batch_run",
     "bm4dof_batch_last_ms", "bm4dof_batch_report", "bm4dof_batch_stats", "bm4dof_batch_destroy",
-    "bm4dof_autotune",
+    "bm4dof_autotune", "bm4dof_tvl1_noalloc",
 ]
 
 BATCH_GRAPH = 1
@@ -182,6 +182,7 @@ def lib():
                                                   C.c_float, C.c_int, C.c_float, C.c_bool]
     L.bm4dof_tvl1.argtypes = [vp, C.POINTER(TVL1Params), vp, vp, vp, vp, C.c_int, C.c_int,
                               C.c_int, vp]
+    L.bm4dof_tvl1_noalloc.argtypes = [vp, C.POINTER(TVL1Params), vp, vp, vp, vp, C.c_int, C.c_int, vp]
     L.bm4dof_tvl1_sequence.argtypes = [vp, C.POINTER(TVL1Params), vp, C.c_int, C.c_int,
                                        C.c_int, vp, vp, C.c_int]
     _LIB = L

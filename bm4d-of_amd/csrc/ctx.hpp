@@ -131,6 +131,10 @@ namespace bm4dof {
 // error (bm4dof_global_error).
 void set_error(bm4dof_ctx *ctx, const std::string &s);
 
+// Records, for `device` (current), the address of the fixed scratch of
+// bm4dof_tvl1_noalloc (tvl1.hip).  Called once per context creation.
+void tv_scratch_init(int device);
+
 // Runs fn, mapping exceptions to the C-ABI's error codes.
 template <class Fn> int guarded(bm4dof_ctx *ctx, Fn fn)
 {

@@ -1631,6 +1631,7 @@ int bm4dof_ctx_create(bm4dof_ctx **out, int device)
 		HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+		tv_scratch_init(device);
 		// coefficient tables: at most 2/5 of the device memory free now
 		size_t free_b = 0, total_b = 0;
 		HIPCHK(hipMemGetInfo(&free_b, &total_b));

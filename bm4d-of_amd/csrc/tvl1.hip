@@ -19,6 +19,7 @@
 #include <vector>
 #include <algorithm>
 #include <array>
+#include <mutex>
 #include <stdexcept>
 
 namespace bm4dof {
@@ -393,6 +394,30 @@ __global__ void tv_copy_frame_kernel(float *__restrict__ flow, int size, int fro
 
 using namespace bm4dof;
 
+// Scratch of bm4dof_tvl1_noalloc: one fixed region per GPU, part of the
+// loaded code object, held until the process ends.  Host code never uses
+// this name as an address: tv_scratch_init records, per device, the address
+// hipGetSymbolAddress gives with that device current, and the entry point
+// passes that address to the kernels.
+static __device__ __align__(256) unsigned char scratch[BM4DOF_TVL1_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+// Called by bm4dof_ctx_create with `device` current.
+void bm4dof::tv_scratch_init(int device)
+{
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (device < 0 || device >= kMaxDevices) throw ArgError("device index above the no-alloc scratch table");
+	if (g_scratch_ptr[device]) return;
+	void *ptr = nullptr;
+	HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+	g_scratch_ptr[device] = ptr;
+}
+
 namespace {
 
 inline int tv_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TV_THREADS - 1) / TV_THREADS, 4096)); }
@@ -686,6 +711,39 @@ int bm4dof_tvl1(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm, const float *I0,
 		HIPCHK(hipMemcpyAsync(u1, du1, n * 4, kb, ctx->stream));
 		HIPCHK(hipMemcpyAsync(u2, du2, n * 4, kb, ctx->stream));
 		HIPCHK(hipStreamSynchronize(ctx->stream));
+	});
+}
+
+__attribute__((visibility("default")))
+int bm4dof_tvl1_noalloc(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm, const float *I0, const float *I1,
+                        float *u1, float *u2, int nx, int ny, int *iters)
+{
+	if (!ctx || !prm || !I0 || !I1 || !u1 || !u2) { set_error(ctx, "NULL argument"); return BM4DOF_EINVAL; }
+	return guarded(ctx, [&] {
+		HIPCHK(hipSetDevice(ctx->device));
+		if (nx < 3 || ny < 3) throw ArgError("image smaller than 3x3");
+		tv_check_params(*prm);
+		void *base = nullptr;
+		{
+			std::lock_guard<std::mutex> lk(g_scratch_mu);
+			if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_ptr[ctx->device];
+		}
+		if (!base) throw ArgError("bm4dof_tvl1_noalloc: no scratch address recorded for this device");
+		const size_t n = (size_t)nx * ny;
+		const TVPlan pl = tv_plan(1, nx, ny, prm->nscales, prm->zfactor);
+		const std::array<size_t, 8> lay = tv_scratch_layout(pl, n);
+		const size_t need = lay[7] + (2 * n * sizeof(float) + 255) / 256 * 256;
+		if (need > (size_t)BM4DOF_TVL1_NOALLOC_BYTES)
+			throw ArgError("bm4dof_tvl1_noalloc: a " + std::to_string(nx) + "x" + std::to_string(ny) +
+			               " pair with these parameters needs " + std::to_string(need) +
+			               " bytes of scratch; the fixed scratch holds " +
+			               std::to_string((size_t)BM4DOF_TVL1_NOALLOC_BYTES));
+		char *b = (char *)base;
+		const TVScratch t = tv_scratch_carve(b, lay);
+		float *fr = (float *)(b + lay[7]);
+		HIPCHK(hipMemcpyAsync(fr, I0, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+		HIPCHK(hipMemcpyAsync(fr + n, I1, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+		tv_flow_pairs(ctx, *prm, fr, nx, ny, {0}, {1}, u1, u2, iters, &t);
 	});
 }
 

@@ -357,6 +357,27 @@ int bm4dof_tvl1(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm,
                 int nx, int ny, int on_device, int *iters);
 
 /*
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops with a fixed memory
+ * budget).  The flow of one frame pair on DEVICE buffers (I0, I1, u1, u2:
+ * nx*ny floats each), the same computation as bm4dof_tvl1 with on_device = 1
+ * and bitwise the same result, with every scratch buffer carved from one
+ * fixed device region of BM4DOF_TVL1_NOALLOC_BYTES per GPU (part of the
+ * library's code object, held until the process ends) instead of the
+ * context's buffers; no device memory is allocated or freed.  The scratch
+ * needed is about 86 bytes per pixel at zfactor 0.5 (a 960x540 pair fits;
+ * a larger zfactor needs more): a pair that does not fit is rejected with
+ * BM4DOF_EINVAL and a message giving both sizes.  Every call on a GPU shares
+ * that GPU's region, which keeps the last call's contents; calls that
+ * overlap in time on one GPU (two threads, two contexts) race and must be
+ * serialised by the caller.
+ */
+#define BM4DOF_TVL1_NOALLOC_BYTES (48u << 20)
+int bm4dof_tvl1_noalloc(bm4dof_ctx *ctx, const bm4dof_tvl1_params *prm,
+                        const float *I0, const float *I1, float *u1, float *u2,
+                        int nx, int ny, int *iters);
+
+/*
  * Forward and backward flow of a whole sequence, replacing the per-pair
  * process loop of scripts/tvl1flow-seq.sh:27-61 in-process:
  *   fwd(i) = flow(I_i -> I_{i+1}),  bwd(i) = flow(I_i -> I_{i-1}),
