@@ -59,26 +59,37 @@ def _psnr(a, b):
 
 
 def _stage_work(stats, anchors, pixels, P=64, C=1):
-    """Algorithmic work of one launch of each stage's main kernel, as
-    (bound, amount, basis).  Group filters: the dense patch transforms as the
-    reference defines them, one multiply + one add per basis term, step 1
-    forward + inverse 4 N P^2, step 2 two forward + one inverse 6 N P^2
-    (N = group rows).  The filters take the forward transforms from
-    per-anchor tables that coef_kernel fills once per image (2 A P^2 FLOP,
-    A = patch anchors, stages s1_coef / s2_coef), so a filter launch
-    executes less than its algorithmic figure.  Ordered aggregation: bytes,
-    each group row's estimate and weight read and the numerator and weight
-    sums written once, N (4 P C + 4) + 4 pixels (C + 1)."""
-    tr = "dense fp32 patch transforms, one multiply + one add per basis term"
-    fw = "; forward transforms read from the per-anchor tables of s1_coef / s2_coef"
-    return {"s1_coef": ("mfma", 2.0 * anchors * P * P * C, tr + "; 2 A P^2, A = patch anchors"),
-            "s2_coef": ("mfma", 2.0 * anchors * P * P * C, tr + "; 2 A P^2, A = patch anchors"),
-            "s1_filter": ("mfma", 4.0 * stats["rows1"] * P * P * C, tr + "; 4 N P^2, N = group rows" + fw),
-            "s2_filter": ("mfma", 6.0 * stats["rows2"] * P * P * C, tr + "; 6 N P^2, N = group rows" + fw),
-            "s1_aggregate": ("hbm", stats["rows1"] * (4.0 * P * C + 4) + 4.0 * pixels * (C + 1),
+    """Work one launch of each stage's main kernel executes, as (bound,
+    amount, basis).  coef_kernel: the forward transform of every patch
+    anchor, 2 A P^2 FLOP (A = anchors).  Group filters (fp32 VALU, no matrix
+    cores): the forward transforms come from those tables, so a filter
+    launch executes the inverse transform, 2 N P^2, the Haar transforms and
+    the shrinkage: step 1 + 8 P (N - 1) + 2 P N, step 2 + 12 P (N - 1) + 5 P N
+    (N = group rows, the counts of the reference's definitions).  Ordered
+    aggregation: bytes, each group row's estimate and weight read and the
+    numerator and weight sums written once, N (4 P C + 4) + 4 pixels (C + 1)."""
+    n1, n2, g1, g2 = stats["rows1"], stats["rows2"], stats["groups1"], stats["groups2"]
+    coef = ("valu", 2.0 * anchors * P * P * C, "forward transform of every patch anchor, 2 A P^2")
+    f1 = 2.0 * n1 * P * P * C + 8.0 * P * C * (n1 - g1) + 2.0 * P * C * n1
+    f2 = 2.0 * n2 * P * P * C + 12.0 * P * C * (n2 - g2) + 5.0 * P * C * n2
+    return {"s1_coef": coef, "s2_coef": coef,
+            "s1_filter": ("valu", f1, "executed: inverse transform 2 N P^2 + Haar 8 P (N - 1) + threshold 2 P N"),
+            "s2_filter": ("valu", f2, "executed: inverse transform 2 N P^2 + Haar 12 P (N - 1) + Wiener 5 P N"),
+            "s1_aggregate": ("hbm", n1 * (4.0 * P * C + 4) + 4.0 * pixels * (C + 1),
                              "estimate rows + weights read, sums written once"),
-            "s2_aggregate": ("hbm", stats["rows2"] * (4.0 * P * C + 4) + 4.0 * pixels * (C + 1),
+            "s2_aggregate": ("hbm", n2 * (4.0 * P * C + 4) + 4.0 * pixels * (C + 1),
                              "estimate rows + weights read, sums written once")}
+
+
+def _step_flop(stats, cand, P=64):
+    """The denoiser's algorithmic work per video by the reference's
+    definitions (block matching 3 P Cand per group, dense transforms
+    4 N P^2 / 6 N P^2, Haar, shrinkage, aggregation 5 N P)."""
+    n1, n2, g1, g2 = stats["rows1"], stats["rows2"], stats["groups1"], stats["groups2"]
+    bm = 3.0 * P * cand * (g1 + g2)
+    tr = 4.0 * n1 * P * P + 6.0 * n2 * P * P
+    rest = 8.0 * P * (n1 - g1) + 2.0 * P * n1 + 12.0 * P * (n2 - g2) + 5.0 * P * n2 + 5.0 * P * (n1 + n2)
+    return bm + tr + rest
 
 
 def _roofline(stage, stage_ms, launches, steps, work, kern):
@@ -89,7 +100,7 @@ def _roofline(stage, stage_ms, launches, steps, work, kern):
                 "avg_launch_ms": round(avg_ms, 3)}
     bound, amount, basis = work[stage]
     per_launch = amount / (launches[stage] / steps)
-    if bound == "mfma":
+    if bound == "valu":
         ach = per_launch / (avg_ms * 1e-3) / 1e12
         peak, unit, key = PEAK_FP32_TFLOPS, "TFLOP/s", "flop_per_launch"
     else:
@@ -126,13 +137,17 @@ def _cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(noisy, clean, ff, bf, crop, threads, gpu_run):
+def cpu_baseline(noisy, clean, ff, bf, crop, threads, gpu_run, reps=3, tv_pairs=0):
     """Time the oracle (C restatement of nldct.cpp, scalar per thread) on a
-    crop of the workload; returns the bench's cpu_baseline object and the
-    comparison of the GPU output on the same crop and tiling (gpu_run) with
-    the oracle's: max-abs difference and PSNR difference of the final
-    estimates."""
+    crop of the workload, the median of `reps` runs; returns the bench's
+    cpu_baseline object and the comparison of the GPU output on the same crop
+    and tiling (gpu_run) with the oracle's: max-abs difference and PSNR
+    difference of the final estimates.  With tv_pairs > 0 the oracle's TV-L1
+    (single-threaded, as the reference's with one thread) is also timed on
+    that many full-size pairs, one per thread, and the 38 solves of the
+    sequence extrapolated from it."""
     import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, REPO)
     import oracle as O
     cw, ch = crop
@@ -144,9 +159,13 @@ def cpu_baseline(noisy, clean, ff, bf, crop, threads, gpu_run):
     o1 = O.cli_params(1, SIGMA, F, px=4, pt=4, wx=11, np_=32)
     o2 = O.cli_params(2, SIGMA, F, px=4, pt=4, wx=11, np_=32)
     O._omp_threads(threads)
-    t0 = time.perf_counter()
-    _, cpu_final, _ = O.run(o1, o2, n, f1, b1, nparts=2 * threads)
-    dt = time.perf_counter() - t0
+    times = []
+    for r in range(reps):
+        t0 = time.perf_counter()
+        _, cpu_final, _ = O.run(o1, o2, n, f1, b1, nparts=2 * threads)
+        times.append(time.perf_counter() - t0)
+        print(f"[bench] CPU oracle run {r + 1}/{reps}: {times[-1]:.1f} s", file=sys.stderr, flush=True)
+    dt = float(np.median(times))
     gpu_final = gpu_run(n, f1, b1, 2 * threads)
     c = clean[sl]
     parity = {"sample": f"{cw}x{ch}x{F} crop, nParts={2 * threads}",
@@ -154,11 +173,31 @@ def cpu_baseline(noisy, clean, ff, bf, crop, threads, gpu_run):
               "identical": bool(np.array_equal(gpu_final, cpu_final)),
               "dpsnr_db": round(_psnr(gpu_final, c) - _psnr(cpu_final, c), 6),
               "psnr_cpu_db": round(_psnr(cpu_final, c), 3)}
-    return {"value": round(cw * ch * F / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{cw}x{ch}x{F} centre crop of the same video and flows, both steps, "
-                      f"{threads} OpenMP threads of {os.cpu_count()} ({_cpu_model()}), "
-                      f"nParts={2 * threads}, {dt:.1f} s"}, parity
+    out = {"value": round(cw * ch * F / dt / 1e6, 5), "unit": "Mpix/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{cw}x{ch}x{F} centre crop of the same video and flows, both steps, "
+                     f"{threads} OpenMP threads of {os.cpu_count()} ({_cpu_model()}), "
+                     f"nParts={2 * threads}, median of {reps}: {dt:.1f} s "
+                     f"(runs {', '.join(f'{t:.1f}' for t in times)} s); the C restatement of the "
+                     "reference with a sequential-K sgemm (the reference itself needs cblas.h and "
+                     "fftw3.h, absent here)"}
+    if tv_pairs > 0:
+        pairs = [(i, i + 1) for i in range(tv_pairs)]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(min(threads, tv_pairs)) as ex:
+            list(ex.map(lambda ij: O.tvl1(noisy[ij[0], 0], noisy[ij[1], 0], lam=0.15), pairs))
+        tv_s = time.perf_counter() - t0
+        per_pair = tv_s * min(threads, tv_pairs) / tv_pairs
+        seq_s = per_pair * 2 * (F - 1) / threads
+        out["tvl1"] = {"pairs_timed": tv_pairs, "wall_s": round(tv_s, 2),
+                       "s_per_pair_one_thread": round(per_pair, 2),
+                       "sequence_38_pairs_s": round(seq_s, 2),
+                       "sample": f"{tv_pairs} full-size 960x540 pairs, one per thread "
+                                 f"(5 scales, 10 warps, lambda 0.15); the 38 solves of the sequence "
+                                 f"at {threads} pairs at a time"}
+        full_denoise_s = dt * (W * H) / (cw * ch)
+        out["e2e_value"] = round(W * H * F / (full_denoise_s + seq_s) / 1e6, 5)
+    return out, parity
 
 
 def batch32(A, S, ctx, torch, nparts):
@@ -288,8 +327,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned single pass")
-    ap.add_argument("--batch32", action="store_true",
-                    help="also time the batched workload: 32 uneven gray sequences in one hipGraph")
+    ap.add_argument("--no-batch32", action="store_true",
+                    help="skip the batched workload (32 uneven gray sequences in one hipGraph)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="CPU baseline runs (median)")
+    ap.add_argument("--cpu-tvl1-pairs", type=int, default=16,
+                    help="full-size pairs of the CPU TV-L1 sample (0: none)")
     ap.add_argument("--learned-eval", type=int, default=0, metavar="N",
                     help="also evaluate the shipped network on N held-out 960x540x20 sequences")
     ap.add_argument("--no-color", action="store_true",
@@ -332,6 +374,9 @@ def main():
     d_final = torch.empty_like(d_noisy)
     torch.cuda.synchronize()
     ctx = A.Context(0)
+    # a caller that denoises videos of one shape repeatedly keeps the
+    # coefficient tables between calls (the default frees them at each end)
+    ctx.set_option("keep_tables", 1)
 
     # flows of the sequence (a first call sizes the buffers; the second is timed)
     ctx.tvl1_sequence_dev(d_noisy.data_ptr(), W, H, F, d_ff.data_ptr(), d_bf.data_ptr())
@@ -382,6 +427,12 @@ def main():
     ranked = sorted((k for k in stage_ms if k in kern), key=lambda k: -stage_ms[k])
     roofline = _roofline(ranked[0], stage_ms, launches, args.steps, work, kern)
     roofline_next = _roofline(ranked[1], stage_ms, launches, args.steps, work, kern)
+    step_flop = _step_flop(stats, 11 * 11 * 9)
+    step_tf = step_flop / (ms_per_step * 1e-3) / 1e12
+    step_roofline = {"bound": "valu", "flop_per_step": step_flop, "achieved": round(step_tf, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(step_tf / PEAK_FP32_TFLOPS, 4),
+                     "basis": "the reference's algorithmic work per video: block matching 3 P Cand per group, "
+                              "dense transforms 4 N P^2 and 6 N P^2, Haar, shrinkage, aggregation 5 N P"}
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Mpix/s", "n_gpus": 1,
@@ -393,6 +444,7 @@ def main():
                    "nparts": args.nparts, "path": "plain"},
         "roofline": roofline,
         "roofline_second": roofline_next,
+        "step_roofline": step_roofline,
         "stages_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
         "tvl1_sequence_s": round(tvl1_s, 3),
         # TV-L1 iterations: 64 B per pixel and iteration (u1, u2, p11..p22 read
@@ -437,7 +489,7 @@ def main():
             "weights": "bm4d-of_amd/weights/gain_mlp_66_128_128_64.f32",
         }
         print(f"[bench] learned single pass: {lv:.2f} Mpix/s", file=sys.stderr, flush=True)
-    if args.batch32:
+    if not args.no_batch32:
         out["batch32"] = batch32(A, S, ctx, torch, args.nparts)
     if args.learned_eval:
         L = _load("bm4dof_amd.learned", os.path.join("bm4d-of_amd", "learned.py"))
@@ -459,7 +511,8 @@ def main():
             return ctx.run(q1, q2, n, f1, b1, nparts=nparts)[1]
 
         out["cpu_baseline"], out["parity_vs_cpu"] = cpu_baseline(
-            noisy, clean, d_ff.cpu().numpy(), d_bf.cpu().numpy(), (cw, ch), thr, gpu_run)
+            noisy, clean, d_ff.cpu().numpy(), d_bf.cpu().numpy(), (cw, ch), thr, gpu_run,
+            reps=args.cpu_reps, tv_pairs=args.cpu_tvl1_pairs)
     else:
         out["cpu_baseline"] = None
     ctx.close()

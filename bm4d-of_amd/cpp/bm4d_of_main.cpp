@@ -8,6 +8,11 @@
 //   * -t1/-t2 change tau only with -print-prms: on the denoising path the
 //     reference applies them inside a block compiled only with VBM3D_SEARCH,
 //     which it does not define (main_vnlb.cpp:233-240, nldct.h:20);
+//   * -vbm3d-search (no flag of the reference: its compile-time switch made a
+//     run-time one) selects the predictive patch search of a VBM3D_SEARCH
+//     build; then -wxf1/-wxf2/-npf1/-npf2/-dsub1/-dsub2 apply and -t1/-t2
+//     apply on the denoising path too, as in such a build
+//     (main_vnlb.cpp:71-78, 165-172, 227-236);
 //   * optical flow is used only when last - first > step (unsigned
 //     arithmetic, main_vnlb.cpp:136);
 //   * -b1 sets both beta and betaMean; -bm1 then overrides betaMean.
@@ -26,6 +31,9 @@ struct Overrides {
 	int wx1, wx2, px1, px2, pt1, pt2, np1, np2;
 	float tau1, tau2, beta1, beta2, bm1, bm2;
 	bool no_paste1, no_paste2, no_step1, no_step2, agg_win1, agg_win2, order_inv;
+	int wxf1, wxf2, npf1, npf2;
+	float dsub1, dsub2;
+	bool predictive;
 };
 
 // Command-line values over the defaults, in the reference's order
@@ -44,8 +52,16 @@ void apply(VideoNLB::nlbParams &p1, VideoNLB::nlbParams &p2, const VideoSize &sz
 	if (o.bm2 >= 0) p2.betaMean = o.bm2;
 	if (o.pt1 >= 0) p1.sizePatchTime = (unsigned)o.pt1;
 	if (o.pt2 >= 0) p2.sizePatchTime = (unsigned)o.pt2;
-	if (with_tau && o.tau1 >= 0) p1.tau = o.tau1;
-	if (with_tau && o.tau2 >= 0) p2.tau = o.tau2;
+	if (o.predictive) {
+		if (o.wxf1 >= 0) p1.sizeSearchWindowPred = (unsigned)o.wxf1;
+		if (o.wxf2 >= 0) p2.sizeSearchWindowPred = (unsigned)o.wxf2;
+		if (o.npf1 >= 0) p1.nSimilarPatchesPred = (unsigned)o.npf1;
+		if (o.npf2 >= 0) p2.nSimilarPatchesPred = (unsigned)o.npf2;
+		if (o.dsub1 >= 0) p1.dsub = o.dsub1;
+		if (o.dsub2 >= 0) p2.dsub = o.dsub2;
+	}
+	if ((with_tau || o.predictive) && o.tau1 >= 0) p1.tau = o.tau1;
+	if ((with_tau || o.predictive) && o.tau2 >= 0) p2.tau = o.tau2;
 	if (o.agg_win1) p1.agg_window = true;
 	if (o.agg_win2) p2.agg_window = true;
 	if (o.no_paste1) p1.doPasteBoost = false;
@@ -92,6 +108,15 @@ int run(int argc, char **argv)
 	o.np2 = opt.get("-np2", -1, "< number of similar patches, step 2");
 	o.tau1 = opt.get("-t1", -1.f, "< step 1 distance threshold");
 	o.tau2 = opt.get("-t2", -1.f, "< step 2 distance threshold");
+	o.predictive = opt.get("-vbm3d-search", false, "< predictive patch search (as built with VBM3D_SEARCH)");
+	o.wxf1 = opt.get("-wxf1", -1, "< search window for predictive search, step 1");
+	o.wxf2 = opt.get("-wxf2", -1, "< search window for predictive search, step 2");
+	o.npf1 = opt.get("-npf1", -1, "< number of similar patches per frame, step 1");
+	o.npf2 = opt.get("-npf2", -1, "< number of similar patches per frame, step 2");
+	o.dsub1 = opt.get("-dsub1", -1.f, "< distance bias, step 1");
+	o.dsub2 = opt.get("-dsub2", -1.f, "< distance bias, step 2");
+	VideoNLB::setPredictiveSearch(o.predictive || VideoNLB::predictiveSearch());
+	o.predictive = VideoNLB::predictiveSearch();
 	o.beta1 = opt.get("-b1", -1.f, "< noise correction factor beta, step 1");
 	o.beta2 = opt.get("-b2", -1.f, "< noise correction factor beta, step 2");
 	o.bm1 = opt.get("-bm1", -1.f, "< noise correction factor beta for mean, step 1");

@@ -71,6 +71,17 @@ VideoNLB::nlbParams from_c(const bm4dof_params &c)
 	return p;
 }
 
+int g_predictive = -1;   // -1: not set, read BM4DOF_VBM3D_SEARCH
+
+bool predictive_on()
+{
+	if (g_predictive < 0) {
+		const char *e = std::getenv("BM4DOF_VBM3D_SEARCH");
+		g_predictive = e && std::atoi(e) ? 1 : 0;
+	}
+	return g_predictive != 0;
+}
+
 // One context for the process, created on first use on BM4DOF_DEVICE (0).
 bm4dof_ctx *context()
 {
@@ -82,6 +93,7 @@ bm4dof_ctx *context()
 		if (bm4dof_ctx_create(&ctx, d ? std::atoi(d) : 0) != BM4DOF_OK)
 			throw std::runtime_error(bm4dof_global_error());
 	}
+	bm4dof_ctx_set_option(ctx, "vbm3d_search", predictive_on() ? 1 : 0);
 	return ctx;
 }
 
@@ -171,6 +183,9 @@ void setNSimilarPatches(nlbParams &prms, unsigned nSimilarPatches)
 	prms = from_c(c);
 }
 
+void setPredictiveSearch(bool on) { g_predictive = on ? 1 : 0; }
+bool predictiveSearch() { return predictive_on(); }
+
 void printNlbParameters(const nlbParams &p)
 {
 	// the reference prints the backward range on both sides (nldct.cpp:223)
@@ -183,6 +198,12 @@ void printNlbParameters(const nlbParams &p)
 	std::printf("\t\tSpatial search window       = %dx%d\n", wx, wx);
 	std::printf("\t\tTemporal search window      = [-%d,%d]\n", wtb, wtb);
 	std::printf("\t\tDistance threshold (tau)    = %g\n", p.tau);
+	if (predictive_on()) {   // lines of a VBM3D_SEARCH build (nldct.cpp:233-239)
+		const int wxpred = (int)p.sizeSearchWindowPred;
+		std::printf("\t\tPred. spatial search window = %dx%d\n", wxpred, wxpred);
+		std::printf("\t\tPred. distance bias (dsub)  = %g\n", p.dsub);
+		std::printf("\t\tSimilar patches per frame   = %d\n", (int)p.nSimilarPatchesPred);
+	}
 	std::printf("\tGroup filtering:\n");
 	std::printf("\t\tTransform                   = %s\n", p.transform == dct ? "dct" : "bior1.5");
 	std::printf("\t\tBeta                        = %g\n", p.beta);

@@ -125,4 +125,11 @@ std::vector<float> runNlBayes(const Video<float> &i_imNoisy, const Video<float> 
                               const Video<float> &i_bflow, Video<float> &o_imBasic, Video<float> &o_imFinal,
                               const nlbParams p_params1, const nlbParams p_params2);
 
+// The reference selects its predictive patch search at compile time
+// (VBM3D_SEARCH, src/core/nldct.h:20; off by default): here a run-time
+// switch, for the block matching of runNlBayes and for printNlbParameters'
+// extra lines.  Also set by BM4DOF_VBM3D_SEARCH=1 in the environment.
+void setPredictiveSearch(bool on);
+bool predictiveSearch();
+
 } // namespace VideoNLB

@@ -28,6 +28,10 @@ struct StepConst {
 	int order_inv;        // orderInvariance
 	int use_flow;         // flows present
 	int nW, wTmp, hTmp;   // tile grid: columns, interior width / height
+	// predictive search (a VBM3D_SEARCH build): window of the frames after
+	// the first, matches kept per frame, distance bias dsub^2*255
+	int pred_w, pred_np;
+	float pred_dsub;
 };
 
 // Pixel block of the ordered aggregation: RX x RY x RT pixels, frame blocks

@@ -81,6 +81,9 @@ __global__ void early_fix_kernel(const uint8_t *, const uint32_t *, const float 
                                  int, int, float *, float *);
 __global__ void filtered_index_kernel(const uint8_t *, int, uint32_t *);
 __global__ void first_moved_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
+__global__ void bm_pred_kernel(const float *, const float *, const float *, const uint32_t *, const uint32_t *,
+                               const TileBox *, StepConst, int, int, int, float, int, int, uint32_t *,
+                               uint32_t *, int);
 __global__ void drop_groups_kernel(const uint32_t *, int, uint32_t *);
 __global__ void first_index_kernel(const uint32_t *, int, int, uint32_t *);
 #define AGG_ARGS const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, \
@@ -471,6 +474,9 @@ StepConst make_const(const bm4dof_params &p, int W, int H, int F, int C, bool us
 	sc.nW = t.nW;
 	sc.wTmp = t.wTmp;
 	sc.hTmp = t.hTmp;
+	sc.pred_w = (int)p.sizeSearchWindowPred;
+	sc.pred_np = (int)p.nSimilarPatchesPred;
+	sc.pred_dsub = p.dsub * p.dsub * 255;   // nldct.cpp:920, 1387
 	return sc;
 }
 
@@ -519,6 +525,18 @@ void run_bm(bm4dof_ctx *ctx, const StepConst &sc, int step, const float *img, co
 	const size_t lds = (size_t)maxCand * 12 + (size_t)sc.C * sc.P * 4;
 	if (lds > 60 * 1024) throw ArgError("search region too large for the block-matching kernel", BM4DOF_EUNSUPP);
 	if (G == 0) return;
+	if (ctx->vbm3d_search && sc.npmax > 1) {
+		// predictive search: libstdc++'s selection throughout, no tie pass
+		const int cap = std::max(sc.sWx * sc.sWx, sc.pred_np * sc.pred_w * sc.pred_w);
+		const int kcap = sc.pred_np * sWt;
+		const size_t lds_p = ((size_t)2 * cap + (size_t)4 * kcap + (size_t)sc.P) * 4;
+		if (lds_p > 60 * 1024) throw ArgError("predictive search regions too large", BM4DOF_EUNSUPP);
+		hipLaunchKernelGGL(bm_pred_kernel, dim3(G), dim3(64), lds_p, ctx->stream, img, ff, bf, d_gpos, d_gtile,
+		                   d_tiles, sc, step == 1 ? 1 : sc.C, sc.pred_w, sc.pred_np, sc.pred_dsub, cap, kcap,
+		                   d_idx, d_nsim, G);
+		HIPCHK(hipGetLastError());
+		return;
+	}
 	// groups whose list depends on the order of equal distances: counted in
 	// ties[0], listed in ties[1..G], then matched again in the reference's
 	// own order (bm_tie_kernel)
@@ -693,6 +711,24 @@ struct LearnedArgs {
 };
 
 // Host-side plan of one step: parameters, geometry and derived sizes.
+// The reference keeps nSimilarPatchesPred candidates of every frame's search
+// region in its predictive search (std::partial_sort to that middle,
+// nldct.cpp:1076); a smaller region is undefined behaviour there and an error
+// here.  Regions are clamped to the tiles: the smallest tile bounds them.
+void check_predictive(const bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, const StepGeom &geo)
+{
+	if (!ctx->vbm3d_search || p.nSimilarPatches <= 1) return;
+	if (p.nSimilarPatchesPred < 1 || p.nSimilarPatchesPred > 64 || p.sizeSearchWindowPred < 1)
+		throw ArgError("predictive search: nSimilarPatchesPred must be in 1..64 and sizeSearchWindowPred positive",
+		               BM4DOF_EUNSUPP);
+	int mw = W, mh = H;
+	for (const TileBox &c : geo.til.crop) { mw = std::min(mw, c.ex - c.ox); mh = std::min(mh, c.ey - c.oy); }
+	const long cx = std::max(0, mw - (int)p.sizePatch + 1), cy = std::max(0, mh - (int)p.sizePatch + 1);
+	for (unsigned ws : {p.sizeSearchWindow, p.sizeSearchWindowPred})
+		if (std::min<long>(ws, cx) * std::min<long>(ws, cy) < (long)p.nSimilarPatchesPred)
+			throw ArgError("predictive search: a search region holds fewer than nSimilarPatchesPred patches");
+}
+
 struct StepPlan {
 	bm4dof_params p;
 	int W = 0, H = 0, F = 0, C = 0, nparts = 0;
@@ -716,6 +752,7 @@ StepPlan plan_step(bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H, int F,
 	check_step_params(p, (unsigned)C);
 	build_bases(p, pl.hb, pl.hbi, pl.hw);
 	pl.geo = make_geometry(p, W, H, F, nparts, C, !full_grid);
+	check_predictive(ctx, p, W, H, pl.geo);
 	pl.sc = make_const(p, W, H, F, C, use_flow, pl.geo.til);
 	pl.G = (int)pl.geo.gpos.size();
 	pl.np = pl.sc.npmax;
@@ -1061,7 +1098,7 @@ std::string plan_key(const bm4dof_ctx *ctx, const bm4dof_params &p, int W, int H
                      int nparts, bool learned, bool full_grid)
 {
 	const long long f[] = {W, H, F, C, use_flow, nparts, learned, ctx->fast_filter, (long long)ctx->est_budget,
-	                       full_grid};
+	                       full_grid, ctx->vbm3d_search};
 	std::string k(reinterpret_cast<const char *>(&p), sizeof p);
 	k.append(reinterpret_cast<const char *>(f), sizeof f);
 	return k;
@@ -1686,6 +1723,8 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 			ctx->coef_budget = (size_t)value;
 		} else if (k == "keep_tables") {
 			ctx->keep_tables = value != 0;
+		} else if (k == "vbm3d_search") {
+			ctx->vbm3d_search = value != 0;
 		} else if (k == "agg_cap") {
 			if (value != 512 && value != 768 && value != 1024 && value != 1536)
 				throw ArgError("agg_cap must be 512, 768, 1024 or 1536");
@@ -1994,6 +2033,7 @@ int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image, unsig
 		const int W = (int)w, H = (int)h, F = (int)f, C = (int)c;
 		const bool flow = ff && bf;
 		StepGeom geo = make_geometry(*p, W, H, F, nparts, C, false);
+		check_predictive(ctx, *p, W, H, geo);
 		StepConst sc = make_const(*p, W, H, F, C, flow, geo.til);
 		const int G = (int)geo.gpos.size();
 		const size_t n = (size_t)W * H * F * C, nf = (size_t)W * H * F * 2;

@@ -474,6 +474,176 @@ void bm_tie_kernel(const float *__restrict__ img, const float *__restrict__ fflo
 }
 
 // ---------------------------------------------------------------------------
+// Predictive block matching of a VBM3D_SEARCH build of the reference
+// (estimateSimilarPatchesStep1 nldct.cpp:904-1130, Step2 :1368-1589), one
+// wave per reference patch.  Frames in the schedule pt, pt+1.., pt-1..: the
+// reference frame is searched in the full window; every other frame in the
+// union of npf windows of wpred around the positions its previous frame's
+// npf best matches reach along the flow (read at the match, rounded half
+// away from zero, clamped to the tile).  Union order: window by window, each
+// in raster order, a position already inside an earlier window skipped (what
+// the reference's unordered_set insertion test lets through, :1044).
+// Distances run on channel 0 (step 2 repeats the channel-0 sum once per
+// channel: its loop reads imBasic(x, y, t) inside the channel loop, :1532),
+// minus dsub at the reference's (x, y) in every frame (:1049).  The npf best
+// of each frame (std::partial_sort, :1076) are kept, the kept ones of all
+// frames sorted again (:1102), and the power-of-two tau cut applied
+// (:1117-1121).  Both selections are libstdc++'s heap select + sort_heap
+// (ls_partial_sort), run by lane 0.
+//
+// Dynamic LDS: cap keys + cap values (candidates of one frame), kcap keys +
+// kcap values (kept), kcap x and y (the matches of every frame, the starts of
+// the next frame's windows), P floats (reference patch); kcap = npf x frames.
+// ---------------------------------------------------------------------------
+#define PRED_MAX_WIN 64
+__global__ __launch_bounds__(64)
+void bm_pred_kernel(const float *__restrict__ img, const float *__restrict__ fflow,
+                    const float *__restrict__ bflow, const uint32_t *__restrict__ gpos,
+                    const uint32_t *__restrict__ gtile, const TileBox *__restrict__ tiles, StepConst sc,
+                    int reps, int wpred, int npf, float dsub, int cap, int kcap,
+                    uint32_t *__restrict__ out_idx, uint32_t *__restrict__ out_nsim, int G)
+{
+	extern __shared__ unsigned char smem[];
+	uint32_t *ckey = (uint32_t *)smem;
+	int *cval = (int *)(ckey + cap);
+	uint32_t *kkey = (uint32_t *)(cval + cap);
+	int *kval = (int *)(kkey + kcap);
+	int *t_cx = kval + kcap, *t_cy = t_cx + kcap;   // [frame - rt0][i]
+	float *sref = (float *)(t_cy + kcap);
+	__shared__ int w_rx0[PRED_MAX_WIN], w_rx1[PRED_MAX_WIN], w_ry0[PRED_MAX_WIN], w_ry1[PRED_MAX_WIN];
+	__shared__ int s_n;
+
+	const int g = blockIdx.x;
+	if (g >= G) return;
+	const int lane = threadIdx.x;
+	const uint32_t pos = gpos[g];
+	int gx, gy, gt;
+	pos_decode(pos, sc.W, sc.WH, gx, gy, gt);
+	const TileBox tb = tiles[gtile[g]];
+	const int Wt = tb.ex - tb.ox, Ht = tb.ey - tb.oy;
+	const int px = gx - tb.ox, py = gy - tb.oy, pt = gt;
+	const int sPx = sc.sPx, sPt = sc.sPt, P = sc.P;
+	uint32_t *oidx = out_idx + (size_t)g * sc.npmax;
+	if (sc.npmax <= 1) {
+		if (lane == 0) { oidx[0] = pos; out_nsim[g] = 1; }
+		return;
+	}
+	const int F = sc.F;
+	const int shift_t = min(0, pt - sc.wtb) + max(0, pt + sc.wtf - F + sPt);
+	const int rt0 = max(0, pt - sc.wtb - shift_t), rt1 = min(F - sPt, pt + sc.wtf - shift_t);
+	const int sWt = rt1 - rt0 + 1;
+	for (int k = lane; k < P; k += 64) {
+		const int ht = k / (sPx * sPx), r = k - ht * sPx * sPx;
+		const int hy = r / sPx, hx = r - hy * sPx;
+		sref[k] = img[video_addr(gx + hx, gy + hy, gt + ht, 0, sc)];
+	}
+	int nk = 0;
+	for (int ii = 0; ii < sWt; ii++) {
+		const int qt = ii == 0 ? pt : (ii <= rt1 - pt ? pt + ii : pt - (ii - (rt1 - pt)));
+		const int dir = qt > pt ? 1 : (qt < pt ? -1 : 0);
+		const int dt = qt - rt0;
+		const int sw = dir ? wpred : sc.sWx, half = (sw - 1) / 2;
+		const int nwin = dir ? npf : 1;
+		// window centres, then the windows shifted into the tile
+		// (nldct.cpp:1019-1036, 1238-1259)
+		for (int i = lane; i < nwin; i += 64) {
+			int cx = px, cy = py;
+			if (dir) {
+				const int ct = qt - dir;
+				cx = t_cx[(dt - dir) * npf + i];
+				cy = t_cy[(dt - dir) * npf + i];
+				float fx = 0.f, fy = 0.f;
+				if (sc.use_flow) {
+					const float *fl = dir > 0 ? fflow : bflow;
+					const size_t base = (size_t)ct * 2 * sc.WH + (size_t)(cy + tb.oy) * sc.W + (cx + tb.ox);
+					fx = fl[base];
+					fy = fl[base + sc.WH];
+				}
+				cx = (int)fmax_ref(0.f, fmin_ref((float)Wt - 1.f, roundf((float)cx + fx)));
+				cy = (int)fmax_ref(0.f, fmin_ref((float)Ht - 1.f, roundf((float)cy + fy)));
+			}
+			int shx = min(0, cx - half), shy = min(0, cy - half);
+			shx += max(0, cx + half - Wt + sPx);
+			shy += max(0, cy + half - Ht + sPx);
+			w_rx0[i] = max(0, cx - half - shx);
+			w_ry0[i] = max(0, cy - half - shy);
+			w_rx1[i] = min(Wt - sPx, cx + half - shx);
+			w_ry1[i] = min(Ht - sPx, cy + half - shy);
+		}
+		__syncthreads();
+		// the union, in insertion order
+		int nc = 0;
+		for (int i = 0; i < nwin; i++) {
+			const int nx = w_rx1[i] - w_rx0[i] + 1, ny = w_ry1[i] - w_ry0[i] + 1;
+			const int cells = max(0, nx) * max(0, ny);
+			for (int c0 = 0; c0 < cells; c0 += 64) {
+				const int c = c0 + lane;
+				bool keep = false;
+				int qx = 0, qy = 0;
+				if (c < cells) {
+					qy = w_ry0[i] + c / nx;
+					qx = w_rx0[i] + c % nx;
+					keep = true;
+					for (int j = 0; j < i && keep; j++)
+						keep = !(qx >= w_rx0[j] && qx <= w_rx1[j] && qy >= w_ry0[j] && qy <= w_ry1[j]);
+				}
+				const unsigned long long b = __ballot(keep);
+				if (keep) {
+					const int slot = nc + (int)__popcll(b & ((1ull << lane) - 1ull));
+					if (slot < cap) cval[slot] = qx | (qy << 16);
+				}
+				nc += (int)__popcll(b);
+			}
+		}
+		nc = min(nc, cap);
+		__syncthreads();
+		// distances in the reference's order, then the bias
+		for (int s = lane; s < nc; s += 64) {
+			const int qx = cval[s] & 0xffff, qy = cval[s] >> 16;
+			const int ax = qx + tb.ox, ay = qy + tb.oy;
+			float d = 0.f;
+			for (int r = 0; r < reps; r++) {
+				int k = 0;
+				for (int ht = 0; ht < sPt; ht++)
+					for (int hy = 0; hy < sPx; hy++) {
+						const float *row = img + video_addr(ax, ay + hy, qt + ht, 0, sc);
+						for (int hx = 0; hx < sPx; hx++, k++) {
+							const float dif = sref[k] - row[hx];
+							d = d + dif * dif;
+						}
+					}
+			}
+			if (qy == py && qx == px) d = d - dsub;
+			ckey[s] = __float_as_uint(d);
+		}
+		__syncthreads();
+		if (lane == 0) {
+			ls_partial_sort(ckey, cval, npf, nc);
+			for (int i = 0; i < npf; i++) {
+				const int qx = cval[i] & 0xffff, qy = cval[i] >> 16;
+				t_cx[dt * npf + i] = qx;
+				t_cy[dt * npf + i] = qy;
+				kkey[nk + i] = ckey[i];
+				kval[nk + i] = (int)((uint32_t)qt * (uint32_t)sc.WH + (uint32_t)(qy + tb.oy) * (uint32_t)sc.W +
+				                     (uint32_t)(qx + tb.ox));
+			}
+		}
+		nk += npf;
+		__syncthreads();
+	}
+	if (lane == 0) {
+		const int nSimP = min(sc.npmax, nk);
+		ls_partial_sort(kkey, kval, nSimP, nk);
+		int n = 1;
+		while (2 * n <= nSimP && __uint_as_float(kkey[n]) < sc.tau2) n *= 2;
+		s_n = n;
+		out_nsim[g] = (uint32_t)n;
+	}
+	__syncthreads();
+	for (int k = lane; k < s_n; k += 64) oidx[k] = (uint32_t)kval[k];
+}
+
+// ---------------------------------------------------------------------------
 // Block matching, staged form (one channel, patches of PX x PX x PT).
 //
 // bm_window_kernel: one thread per group computes the flow trajectory and

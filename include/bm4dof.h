@@ -66,9 +66,9 @@ typedef struct bm4dof_params {
 	int32_t  verbose;
 	int32_t  colorSpace;            /* BM4DOF_RGB / BM4DOF_YUV          */
 	int32_t  transform;             /* BM4DOF_DCT / BM4DOF_BIOR1_5      */
-	uint32_t sizeSearchWindowPred;  /* VBM3D predictive search (unused) */
-	uint32_t nSimilarPatchesPred;   /* VBM3D predictive search (unused) */
-	float    dsub;                  /* VBM3D predictive search (unused) */
+	uint32_t sizeSearchWindowPred;  /* predictive search ("vbm3d_search")  */
+	uint32_t nSimilarPatchesPred;   /* predictive search ("vbm3d_search")  */
+	float    dsub;                  /* predictive search ("vbm3d_search")  */
 	int32_t  agg_window;
 } bm4dof_params;
 
@@ -101,8 +101,8 @@ const char *bm4dof_global_error(void);
 
 /*
  * Execution options of a context (no counterpart in the reference, whose
- * execution is fixed).  They select between kernels that compute the same
- * result; outputs do not depend on them.  Keys:
+ * execution is fixed).  All but "vbm3d_search" select between kernels that
+ * compute the same result; outputs do not depend on them.  Keys:
  *   "fast_filter"  1 (default): register-resident group filter where the
  *                  group shape allows it (64-sample patches, fewer than 64
  *                  similar patches, no orderInvariance; colour one launch
@@ -128,6 +128,12 @@ const char *bm4dof_global_error(void);
  *                  whose tables do not fit the budget or the free device
  *                  memory transform the patches group by group; the output
  *                  is the same either way.  0 disables the tables.
+ *   "vbm3d_search" 0 (default): block matching as the reference builds it;
+ *                  1: the predictive search of a build with VBM3D_SEARCH
+ *                  defined (src/core/nldct.h:20; nldct.cpp:904-1130,
+ *                  1368-1589), driven by the parameters' sizeSearchWindowPred,
+ *                  nSimilarPatchesPred and dsub.  This one changes results:
+ *                  it selects the search a differently built reference runs.
  *   "keep_tables"  0 (default): the tables are freed at the end of every
  *                  bm4dof_run / _run_dev / _run_learned(_dev) call; 1: they
  *                  stay allocated for the next call on the context (saves

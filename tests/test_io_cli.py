@@ -294,6 +294,20 @@ def test_bm4d_of_flow_and_patch_checks():
     assert rc == 1 and err.startswith("Given patch sizes for both steps are 0. Nothing to do.")
 
 
+def test_bm4d_of_predictive_search_flags():
+    """-vbm3d-search: the flags and printed lines of a VBM3D_SEARCH build
+    (main_vnlb.cpp:71-78, 165-172; nldct.cpp:233-239); without it the
+    predictive flags change nothing and print nothing."""
+    args = ("-i", "x", "-print-prms", "1", "-sigma", "20", "-wxf1", "3", "-npf2", "6", "-dsub1", "2.5")
+    rc, out, _ = _tool("bm4d-of", *args, "-vbm3d-search", "1")
+    assert rc == 0
+    b1, b2 = out.split("\x1b[37;01mParameters for step ")[1:]
+    assert "Pred. spatial search window = 3x3" in b1 and "Pred. distance bias (dsub)  = 2.5" in b1
+    assert "Similar patches per frame   = 6" in b2 and "Pred. distance bias (dsub)  = 3\n" in b2
+    rc, out, _ = _tool("bm4d-of", *args)
+    assert rc == 0 and "Pred." not in out
+
+
 def test_bm4d_of_print_parameters():
     """-print-prms: the defaults for a 0-frame video of the given channels,
     then the overrides, printed in the reference's format."""
