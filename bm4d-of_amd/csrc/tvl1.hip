@@ -242,119 +242,255 @@ __global__ void tv_warp_kernel(const float *__restrict__ I0, const float *__rest
 	}
 }
 
-// divergence with backward differences at one pixel (mask.c:43-92).
-__device__ __forceinline__ float div_at(const float *v1, const float *v2, int i, int j, int nx, int ny)
+// ------------------------------------------------- primal-dual iterations
+// K iterations of the TV-L1 primal-dual scheme (tvl1flow_lib.c:165-246) per
+// launch, on LDS tiles: a block loads its TX x TY tile with a K-pixel halo
+// (clipped to the image) and iterates there; after iteration j the pixels at
+// least j pixels inside a cut halo edge are exact (an iteration reaches one
+// pixel in each direction: the divergence backwards, the gradient forwards),
+// so after K iterations the tile itself is, and only it is written back.
+// Per iteration: the TH step, divergence of p and u update for every pixel
+// (phase A), then the forward gradient of the new u and the p update
+// (phase B); each pixel's arithmetic is the reference's, operation for
+// operation, as in the one-iteration form.  The squared update of every
+// tile pixel and iteration is summed in double (pixels in thread order,
+// lanes by xor tree, waves in order) into part[pair][j][tile].
+//
+// The state (u1, u2, p11, p12, p21, p22) is double-buffered: a pair reads
+// buffer cur[pair] and writes the other.  ctl[pair] is the number of
+// iterations to run this round (0: idle).  tv_round_kernel then applies the
+// stopping test to each iteration of the round in order; if it fires before
+// the round's last iteration, the pair runs that many iterations again from
+// the same source buffer (an exact re-run), so the state and the iteration
+// count are those of the iteration the test fired on.
+// ---------------------------------------------------------------------------
+struct TVPlanes {
+	float *u1, *u2, *p11, *p12, *p21, *p22;
+};
+
+#define TV_TX 64
+#define TV_TY 32
+#define TV_K 4
+#define TV_NT 1024
+enum { TV_RUN = 0, TV_REDO = 1 };
+
+// divergence of (v1, v2) with backward differences at one pixel of an LDS
+// tile (mask.c:43-92); el / eu index the left / upper neighbours.
+__device__ __forceinline__ float div_lds(const float *v1, const float *v2, int e, int el, int eu, bool top,
+                                         bool bot, bool left, bool right)
 {
-	const int p = i * nx + j;
-	const bool top = i == 0, bot = i == ny - 1, left = j == 0, right = j == nx - 1;
-	if (!top && !bot && !left && !right) return (v1[p] - v1[p - 1]) + (v2[p] - v2[p - nx]);
-	if (top && !left && !right) return v1[p] - v1[p - 1] + v2[p];
-	if (bot && !left && !right) return v1[p] - v1[p - 1] - v2[p - nx];
-	if (left && !top && !bot) return v1[p] + v2[p] - v2[p - nx];
-	if (right && !top && !bot) return -v1[p - 1] + v2[p] - v2[p - nx];
-	if (top && left) return v1[p] + v2[p];
-	if (top && right) return -v1[p - 1] + v2[p];
-	if (bot && left) return v1[p] - v2[p - nx];
-	return -v1[p - 1] - v2[p - nx];
+	if (!top && !bot && !left && !right) return (v1[e] - v1[el]) + (v2[e] - v2[eu]);
+	if (top && !left && !right) return v1[e] - v1[el] + v2[e];
+	if (bot && !left && !right) return v1[e] - v1[el] - v2[eu];
+	if (left && !top && !bot) return v1[e] + v2[e] - v2[eu];
+	if (right && !top && !bot) return -v1[el] + v2[e] - v2[eu];
+	if (top && left) return v1[e] + v2[e];
+	if (top && right) return -v1[el] + v2[e];
+	if (bot && left) return v1[e] - v2[eu];
+	return -v1[el] - v2[eu];
 }
 
-// One primal update: TH step, divergence, u update and the squared update
-// (tvl1flow_lib.c:165-225).  Per-block partial sums in double.
-__global__ void tv_u_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1wy,
-                            const float *__restrict__ grad, const float *__restrict__ rho_c,
-                            float *__restrict__ u1, float *__restrict__ u2,
-                            const float *__restrict__ p11, const float *__restrict__ p12,
-                            const float *__restrict__ p21, const float *__restrict__ p22,
-                            int nx, int ny, int stride, float l_t, float theta,
-                            const int *__restrict__ state, double *__restrict__ part)
+template <int TX, int TY, int K, int NT>
+__global__ __launch_bounds__(NT)
+void tv_iter_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1wy,
+                    const float *__restrict__ grad, const float *__restrict__ rho_c, TVPlanes A, TVPlanes B,
+                    int nx, int ny, int stride, int tiles_x, float l_t, float theta, float taut,
+                    const int *__restrict__ ctl, const int *__restrict__ cur, double *__restrict__ part)
 {
+	constexpr int EN = (TX + 2 * K) * (TY + 2 * K), NPT = (EN + NT - 1) / NT, NW = NT / 64;
+	__shared__ float U1[EN], U2[EN], Q11[EN], Q12[EN], Q21[EN], Q22[EN];
+	__shared__ double red[NW][K];
 	const int pr = blockIdx.y;
-	__shared__ double s[TV_THREADS];
-	double e = 0.0;
-	if (state[pr] < 0) {
-		const size_t off = (size_t)pr * stride;
-		for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
-			const int i = p / nx, j = p - i * nx;
-			const float wx = I1wx[off + p], wy = I1wy[off + p], g = grad[off + p];
-			const float u1k = u1[off + p], u2k = u2[off + p];
-			const float rho = rho_c[off + p] + (wx * u1k + wy * u2k);
+	const int kk = ctl[pr];
+	if (kk <= 0) return;
+	const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+	const int tile = blockIdx.x, ntiles = gridDim.x;
+	const int tx0 = (tile % tiles_x) * TX, ty0 = (tile / tiles_x) * TY;
+	const int xs = max(0, tx0 - K), ys = max(0, ty0 - K);
+	const int EW = min(nx, tx0 + TX + K) - xs, EH = min(ny, ty0 + TY + K) - ys, en = EW * EH;
+	const int ix1 = min(nx, tx0 + TX), iy1 = min(ny, ty0 + TY);
+	const bool flip = cur[pr] != 0;
+	const TVPlanes S = flip ? B : A, D = flip ? A : B;
+	const size_t off = (size_t)pr * stride;
+	float cwx[NPT], cwy[NPT], cg[NPT], crc[NPT];
+	int cxy[NPT];
+#pragma unroll
+	for (int k = 0; k < NPT; k++) {
+		const int e = tid + k * NT;
+		cxy[k] = -1;
+		if (e < en) {
+			const int ey = e / EW, ex = e - ey * EW;
+			const size_t gi = off + (size_t)(ys + ey) * nx + (xs + ex);
+			cwx[k] = I1wx[gi]; cwy[k] = I1wy[gi]; cg[k] = grad[gi]; crc[k] = rho_c[gi];
+			U1[e] = S.u1[gi]; U2[e] = S.u2[gi];
+			Q11[e] = S.p11[gi]; Q12[e] = S.p12[gi]; Q21[e] = S.p21[gi]; Q22[e] = S.p22[gi];
+			cxy[k] = ex | (ey << 16);
+		}
+	}
+	double err[K];
+#pragma unroll
+	for (int j = 0; j < K; j++) err[j] = 0.0;
+	__syncthreads();
+	for (int j = 0; j < kk; j++) {
+		// phase A: TH, divergence, u (tvl1flow_lib.c:170-225)
+		double ej = 0.0;
+#pragma unroll
+		for (int k = 0; k < NPT; k++) {
+			if (cxy[k] < 0) continue;
+			const int ex = cxy[k] & 0xffff, ey = cxy[k] >> 16, e = ey * EW + ex;
+			const int gx = xs + ex, gy = ys + ey;
+			const float u1k = U1[e], u2k = U2[e];
+			const float wx = cwx[k], wy = cwy[k], g = cg[k];
+			const float rho = crc[k] + (wx * u1k + wy * u2k);
 			float d1, d2;
 			if (rho < -l_t * g) { d1 = l_t * wx; d2 = l_t * wy; }
 			else if (rho > l_t * g) { d1 = -l_t * wx; d2 = -l_t * wy; }
 			else if (g < TV_GRAD_IS_ZERO) d1 = d2 = 0;
 			else { const float fi = -rho / g; d1 = fi * wx; d2 = fi * wy; }
 			const float v1 = u1k + d1, v2 = u2k + d2;
-			const float dp1 = div_at(p11 + off, p12 + off, i, j, nx, ny);
-			const float dp2 = div_at(p21 + off, p22 + off, i, j, nx, ny);
+			const int el = ex > 0 ? e - 1 : e, eu = ey > 0 ? e - EW : e;
+			const bool top = gy == 0, bot = gy == ny - 1, left = gx == 0, right = gx == nx - 1;
+			const float dp1 = div_lds(Q11, Q12, e, el, eu, top, bot, left, right);
+			const float dp2 = div_lds(Q21, Q22, e, el, eu, top, bot, left, right);
 			const float n1 = v1 + theta * dp1;
 			const float n2 = v2 + theta * dp2;
-			u1[off + p] = n1;
-			u2[off + p] = n2;
-			e += (double)((n1 - u1k) * (n1 - u1k) + (n2 - u2k) * (n2 - u2k));
+			U1[e] = n1;
+			U2[e] = n2;
+			if (gx >= tx0 && gx < ix1 && gy >= ty0 && gy < iy1)
+				ej += (double)((n1 - u1k) * (n1 - u1k) + (n2 - u2k) * (n2 - u2k));
 		}
-	}
-	s[threadIdx.x] = e;
-	__syncthreads();
-	for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
-		if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+#pragma unroll
+		for (int q = 0; q < K; q++) if (q == j) err[q] = ej;
+		__syncthreads();
+		// phase B: forward gradient of the new u, p (tvl1flow_lib.c:228-245)
+#pragma unroll
+		for (int k = 0; k < NPT; k++) {
+			if (cxy[k] < 0) continue;
+			const int ex = cxy[k] & 0xffff, ey = cxy[k] >> 16, e = ey * EW + ex;
+			const int gx = xs + ex, gy = ys + ey;
+			const int er = ex + 1 < EW ? e + 1 : e, ed = ey + 1 < EH ? e + EW : e;
+			const float a1 = U1[e], a2 = U2[e];
+			const float u1x = gx < nx - 1 ? U1[er] - a1 : 0.f;
+			const float u1y = gy < ny - 1 ? U1[ed] - a1 : 0.f;
+			const float u2x = gx < nx - 1 ? U2[er] - a2 : 0.f;
+			const float u2y = gy < ny - 1 ? U2[ed] - a2 : 0.f;
+			const float g1 = (float)sqrt((double)u1x * u1x + (double)u1y * u1y);
+			const float g2 = (float)sqrt((double)u2x * u2x + (double)u2y * u2y);
+			const float ng1 = 1.0 + taut * g1;
+			const float ng2 = 1.0 + taut * g2;
+			Q11[e] = (Q11[e] + taut * u1x) / ng1;
+			Q12[e] = (Q12[e] + taut * u1y) / ng1;
+			Q21[e] = (Q21[e] + taut * u2x) / ng2;
+			Q22[e] = (Q22[e] + taut * u2y) / ng2;
+		}
 		__syncthreads();
 	}
-	if (threadIdx.x == 0) part[(size_t)pr * gridDim.x + blockIdx.x] = s[0];
+	// the tile itself back to the other buffer
+#pragma unroll
+	for (int k = 0; k < NPT; k++) {
+		if (cxy[k] < 0) continue;
+		const int ex = cxy[k] & 0xffff, ey = cxy[k] >> 16, e = ey * EW + ex;
+		const int gx = xs + ex, gy = ys + ey;
+		if (gx < tx0 || gx >= ix1 || gy < ty0 || gy >= iy1) continue;
+		const size_t gi = off + (size_t)gy * nx + gx;
+		D.u1[gi] = U1[e]; D.u2[gi] = U2[e];
+		D.p11[gi] = Q11[e]; D.p12[gi] = Q12[e]; D.p21[gi] = Q21[e]; D.p22[gi] = Q22[e];
+	}
+	// squared updates of the tile, per iteration
+#pragma unroll
+	for (int j = 0; j < K; j++) {
+		double v = err[j];
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+		if (lane == 0) red[wv][j] = v;
+	}
+	__syncthreads();
+	if (tid < kk) {
+		double v = 0.0;
+		for (int w = 0; w < NW; w++) v += red[w][tid];
+		part[((size_t)pr * K + tid) * ntiles + tile] = v;
+	}
 }
 
-// Stopping test of iteration k (tvl1flow_lib.c:165, 225): state becomes k
-// when error <= epsilon^2 or k reaches MAX_ITERATIONS.
-// The block partials are summed by TV_THREADS threads in a fixed order
-// (strided partial sums, then a tree), so the result does not depend on
-// timing.
-__global__ void tv_decide_kernel(const double *__restrict__ part, int nblocks, int size, float eps2,
-                                 int k, int *__restrict__ state, int *__restrict__ n_active)
+// The stopping test of every iteration of a round, in order
+// (tvl1flow_lib.c:165, 225): error = (sum over the tiles, in tile order) /
+// size, and the loop ends when error <= epsilon^2 or the count reaches
+// MAX_ITERATIONS.  One block per pair.  state[pair] receives the final count.
+__global__ void tv_round_kernel(const double *__restrict__ part, int K, int ntiles, int size, float eps2,
+                                int *__restrict__ ctl, int *__restrict__ cur, int *__restrict__ iters,
+                                int *__restrict__ mode, int *__restrict__ state, int *__restrict__ n_active)
 {
 	const int pr = blockIdx.x;
-	if (state[pr] >= 0) return;
+	const int kk = ctl[pr];
+	if (kk <= 0) return;
 	__shared__ double s[TV_THREADS];
-	double e = 0.0;
-	for (int b = threadIdx.x; b < nblocks; b += TV_THREADS) e += part[(size_t)pr * nblocks + b];
-	s[threadIdx.x] = e;
-	__syncthreads();
-	for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
-		if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+	__shared__ float errs[TV_K];
+	for (int j = 0; j < kk; j++) {
+		double e = 0.0;
+		for (int t = threadIdx.x; t < ntiles; t += TV_THREADS) e += part[((size_t)pr * K + j) * ntiles + t];
+		s[threadIdx.x] = e;
+		__syncthreads();
+		for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
+			if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) errs[j] = (float)s[0] / (float)size;
 		__syncthreads();
 	}
 	if (threadIdx.x != 0) return;
-	const float error = (float)s[0] / (float)size;
-	if (!(error > eps2) || k >= TV_MAX_ITERATIONS) {
-		state[pr] = k;
+	const int done = iters[pr];
+	if (mode[pr] == TV_REDO) {   // the exact re-run of the iterations up to the test
+		iters[pr] = done + kk;
+		cur[pr] ^= 1;
+		state[pr] = done + kk;
+		ctl[pr] = 0;
 		atomicSub(n_active, 1);
+		return;
+	}
+	int stop = -1;
+	for (int j = 0; j < kk && stop < 0; j++)
+		if (!(errs[j] > eps2) || done + j + 1 >= TV_MAX_ITERATIONS) stop = j;
+	if (stop < 0) {
+		iters[pr] = done + kk;
+		cur[pr] ^= 1;
+		ctl[pr] = min(K, TV_MAX_ITERATIONS - (done + kk));
+	} else if (stop == kk - 1) {
+		iters[pr] = done + kk;
+		cur[pr] ^= 1;
+		state[pr] = done + kk;
+		ctl[pr] = 0;
+		atomicSub(n_active, 1);
+	} else {
+		mode[pr] = TV_REDO;
+		ctl[pr] = stop + 1;
 	}
 }
 
-// Dual update with the forward-difference gradient of the new u
-// (tvl1flow_lib.c:228-245, mask.c:101-144); runs for pairs that were active
-// at the start of iteration k.
-__global__ void tv_p_kernel(const float *__restrict__ u1, const float *__restrict__ u2,
-                            float *__restrict__ p11, float *__restrict__ p12,
-                            float *__restrict__ p21, float *__restrict__ p22, int nx, int ny,
-                            int stride, float taut, int k, const int *__restrict__ state)
+// Per pair: run state of the solve of one warp.
+__global__ void tv_round_init_kernel(int P, int K, int *__restrict__ ctl, int *__restrict__ cur,
+                                     int *__restrict__ iters, int *__restrict__ mode, int *__restrict__ state,
+                                     int *__restrict__ n_active)
+{
+	const int pr = blockIdx.x * blockDim.x + threadIdx.x;
+	if (pr == 0) *n_active = P;
+	if (pr >= P) return;
+	ctl[pr] = min(K, TV_MAX_ITERATIONS);
+	cur[pr] = 0;
+	iters[pr] = 0;
+	mode[pr] = TV_RUN;
+	state[pr] = -1;
+}
+
+// Pairs whose last round wrote the second buffer: copy it to the first.
+__global__ void tv_settle_kernel(TVPlanes A, TVPlanes B, int n, int stride, const int *__restrict__ cur)
 {
 	const int pr = blockIdx.y;
-	if (!(state[pr] < 0 || state[pr] == k)) return;
+	if (!cur[pr]) return;
 	const size_t off = (size_t)pr * stride;
-	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
-		const int i = p / nx, j = p - i * nx;
-		const float a1 = u1[off + p], a2 = u2[off + p];
-		const float u1x = j < nx - 1 ? u1[off + p + 1] - a1 : 0.f;
-		const float u1y = i < ny - 1 ? u1[off + p + nx] - a1 : 0.f;
-		const float u2x = j < nx - 1 ? u2[off + p + 1] - a2 : 0.f;
-		const float u2y = i < ny - 1 ? u2[off + p + nx] - a2 : 0.f;
-		const float g1 = (float)sqrt((double)u1x * u1x + (double)u1y * u1y);
-		const float g2 = (float)sqrt((double)u2x * u2x + (double)u2y * u2y);
-		const float ng1 = 1.0 + taut * g1;
-		const float ng2 = 1.0 + taut * g2;
-		p11[off + p] = (p11[off + p] + taut * u1x) / ng1;
-		p12[off + p] = (p12[off + p] + taut * u1y) / ng1;
-		p21[off + p] = (p21[off + p] + taut * u2x) / ng2;
-		p22[off + p] = (p22[off + p] + taut * u2y) / ng2;
+	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+		A.u1[off + p] = B.u1[off + p]; A.u2[off + p] = B.u2[off + p];
+		A.p11[off + p] = B.p11[off + p]; A.p12[off + p] = B.p12[off + p];
+		A.p21[off + p] = B.p21[off + p]; A.p22[off + p] = B.p22[off + p];
 	}
 }
 
@@ -421,6 +557,7 @@ void bm4dof::tv_scratch_init(int device)
 namespace {
 
 inline int tv_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TV_THREADS - 1) / TV_THREADS, 4096)); }
+inline int nblk_tv(int64_t n) { return (int)((n + TV_THREADS - 1) / TV_THREADS); }
 
 // Normalised 1-D Gaussian taps (mask.c:227-249), computed on the host with
 // the same libm calls as the reference.
@@ -490,15 +627,21 @@ void tv_gauss(bm4dof_ctx *ctx, double *dB, const float *in, float *tmp, float *o
 }
 
 // Device memory of one multiscale solve of P pairs: pair indices, the four
-// pyramids (I0, I1, u1, u2), 14 working planes at the finest size, min/max
-// partials, stopping-test partial sums, per-pair state and Gaussian taps.
+// pyramids (I0, I1, u1, u2), 16 working planes at the finest size (12 of
+// them the scheme's arrays, the last four with the two Gaussian temporaries
+// the second state buffer of the iterations), min/max partials, the
+// iterations' per-tile partial sums, the per-pair round control (ctl, cur,
+// iters, mode, state, then the active count) and Gaussian taps.
 struct TVScratch {
 	int *pairs = nullptr;
 	float *pyr = nullptr, *work = nullptr, *mm = nullptr;
 	double *part = nullptr, *taps = nullptr;
 	int *state = nullptr;
 };
-constexpr int TV_MMB = 64;   // blocks of the min/max reduction per pair
+constexpr int TV_MMB = 64;      // blocks of the min/max reduction per pair
+constexpr int TV_WORK = 16;     // working planes at the finest size
+
+inline int tv_tiles(int nx, int ny) { return ((nx + TV_TX - 1) / TV_TX) * ((ny + TV_TY - 1) / TV_TY); }
 
 // Byte offsets of the parts of a TVScratch carved from one region, and the
 // region's size (the last entry), each part 256-byte aligned.
@@ -510,10 +653,10 @@ std::array<size_t, 8> tv_scratch_layout(const TVPlan &pl, size_t size0)
 	o[0] = 0;                                                          // pairs
 	o[1] = o[0] + al(2 * P * sizeof(int));                             // pyramids
 	o[2] = o[1] + al(pl.pyr * 4 * sizeof(float));                      // working planes
-	o[3] = o[2] + al(P * size0 * 14 * sizeof(float));                  // min/max
-	o[4] = o[3] + al(P * TV_MMB * 4 * sizeof(float));                  // partial sums
-	o[5] = o[4] + al(P * (size_t)tv_grid((int64_t)size0) * sizeof(double));
-	o[6] = o[5] + al((P + 1) * sizeof(int));                           // taps
+	o[3] = o[2] + al(P * size0 * TV_WORK * sizeof(float));             // min/max
+	o[4] = o[3] + al(P * TV_MMB * 4 * sizeof(float));                  // iteration partial sums
+	o[5] = o[4] + al(P * TV_K * (size_t)tv_tiles(pl.nx[0], pl.ny[0]) * sizeof(double));
+	o[6] = o[5] + al((5 * P + 1) * sizeof(int));                       // taps
 	o[7] = o[6] + al(64 * sizeof(double));                             // total
 	return o;
 }
@@ -537,10 +680,10 @@ TVScratch tv_scratch_ctx(bm4dof_ctx *ctx, const TVPlan &pl, size_t size0)
 	TVScratch t;
 	t.pairs = ctx->tv_pairs.as<int>(2 * P);
 	t.pyr = ctx->tv_pyr.as<float>(pl.pyr * 4);
-	t.work = ctx->tv_work.as<float>(P * size0 * 14);
+	t.work = ctx->tv_work.as<float>(P * size0 * TV_WORK);
 	t.mm = ctx->tv_mm.as<float>(P * TV_MMB * 4);
-	t.part = ctx->tv_part.as<double>(P * (size_t)tv_grid((int64_t)size0));
-	t.state = ctx->tv_state.as<int>(P + 1);
+	t.part = ctx->tv_part.as<double>(P * TV_K * (size_t)tv_tiles(pl.nx[0], pl.ny[0]));
+	t.state = ctx->tv_state.as<int>(5 * P + 1);
 	t.taps = ctx->tv_taps.as<double>(64);
 	return t;
 }
@@ -579,7 +722,8 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	// pyramids: I0s, I1s, u1s, u2s
 	float *pyr = sc.pyr;
 	float *I0s = pyr, *I1s = pyr + pl.pyr, *U1 = pyr + 2 * pl.pyr, *U2 = pyr + 3 * pl.pyr;
-	// working arrays at the finest size: tmp, tmp2, I1x, I1y, I1wx, I1wy, grad, rho_c, p11..p22
+	// working arrays at the finest size: tmp, tmp2, I1x, I1y, I1wx, I1wy, grad, rho_c, p11..p22,
+	// and four planes that with tmp and tmp2 hold the iterations' second state buffer
 	const size_t W = (size_t)P * size0;
 	float *work = sc.work;
 	float *tmp = work, *tmp2 = work + W, *I1x = work + 2 * W, *I1y = work + 3 * W,
@@ -588,7 +732,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	const int mmb = TV_MMB;
 	float *d_mm = sc.mm;
 	double *d_part = sc.part;
-	int *d_state = sc.state;
+	int *d_ctl = sc.state, *d_cur = d_ctl + P, *d_iters = d_cur + P, *d_mode = d_iters + P, *d_state = d_mode + P;
 	int *d_nact = d_state + P;
 
 	// normalisation + presmoothing (tvl1flow_lib.c:381-386)
@@ -624,6 +768,9 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	const float eps2 = prm.epsilon * prm.epsilon;
 	const float scale = (float)1.0 / prm.zfactor;
 	std::vector<int> h_state(P);
+	// rounds of TV_K iterations between looks at the active count
+	constexpr int kRoundsPerCheck = 3;
+	const int max_rounds = (TV_MAX_ITERATIONS + TV_K - 1) / TV_K + 1;
 
 	for (int s = nscales - 1; s >= prm.fscale; s--) {
 		// Dual_TVL1_optic_flow, tvl1flow_lib.c:93-275
@@ -631,6 +778,9 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		float *u1 = U1 + pl.off[s], *u2 = U2 + pl.off[s];
 		const float *I0 = I0s + pl.off[s], *I1 = I1s + pl.off[s];
 		const int g = tv_grid(n);
+		const TVPlanes SA = {u1, u2, p11, p12, p21, p22};
+		const TVPlanes SB = {tmp, tmp2, work + 12 * W, work + 13 * W, work + 14 * W, work + 15 * W};
+		const int tiles_x = (nx + TV_TX - 1) / TV_TX, ntiles = tv_tiles(nx, ny);
 		hipLaunchKernelGGL(tv_cgrad_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1, nx, ny, n, I1x, I1y);
 		const size_t nP = (size_t)P * n;
 		for (float *q : {p11, p12, p21, p22})
@@ -638,17 +788,16 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		for (int w = 0; w < prm.warps; w++) {
 			hipLaunchKernelGGL(tv_warp_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I0, I1, I1x, I1y, u1, u2,
 			                   nx, ny, n, I1wx, I1wy, grad, rho_c);
-			hipLaunchKernelGGL(tv_fill_int_kernel, dim3(1), dim3(TV_THREADS), 0, st, d_state, P, -1);
-			hipLaunchKernelGGL(tv_fill_int_kernel, dim3(1), dim3(1), 0, st, d_nact, 1, P);
+			hipLaunchKernelGGL(tv_round_init_kernel, dim3(nblk_tv(P)), dim3(TV_THREADS), 0, st, P, TV_K, d_ctl,
+			                   d_cur, d_iters, d_mode, d_state, d_nact);
 			HIPCHK(hipGetLastError());
-			for (int k = 1; k <= TV_MAX_ITERATIONS; k++) {
-				hipLaunchKernelGGL(tv_u_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1wx, I1wy, grad, rho_c,
-				                   u1, u2, p11, p12, p21, p22, nx, ny, n, l_t, prm.theta, d_state, d_part);
-				hipLaunchKernelGGL(tv_decide_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, g, n, eps2, k,
-				                   d_state, d_nact);
-				hipLaunchKernelGGL(tv_p_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, u1, u2, p11, p12, p21,
-				                   p22, nx, ny, n, taut, k, d_state);
-				if (k % 8 == 0 || k == TV_MAX_ITERATIONS) {
+			for (int r = 0; r < max_rounds; r++) {
+				hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0, st,
+				                   I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut,
+				                   d_ctl, d_cur, d_part);
+				hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n,
+				                   eps2, d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
+				if ((r + 1) % kRoundsPerCheck == 0 || r == max_rounds - 1) {
 					HIPCHK(hipGetLastError());
 					int nact = 0;
 					HIPCHK(hipMemcpyAsync(&nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -656,6 +805,8 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 					if (nact == 0) break;
 				}
 			}
+			hipLaunchKernelGGL(tv_settle_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, SA, SB, n, n, d_cur);
+			HIPCHK(hipGetLastError());
 			HIPCHK(hipMemcpyAsync(h_state.data(), d_state, P * sizeof(int), hipMemcpyDeviceToHost, st));
 			HIPCHK(hipStreamSynchronize(st));
 			for (int pr = 0; pr < P; pr++) ctx->tv_pixel_iters += (uint64_t)std::max(h_state[pr], 0) * (uint64_t)n;
