@@ -271,34 +271,132 @@ struct TVPlanes {
 #define TV_TX 64
 #define TV_TY 32
 #define TV_K 4
-#define TV_NT 1024
+#define TV_NT 512
 enum { TV_RUN = 0, TV_REDO = 1 };
 
-// divergence of (v1, v2) with backward differences at one pixel of an LDS
-// tile (mask.c:43-92); el / eu index the left / upper neighbours.
-__device__ __forceinline__ float div_lds(const float *v1, const float *v2, int e, int el, int eu, bool top,
-                                         bool bot, bool left, bool right)
+// Per-slot flags of a tile pixel (the LDS index is in the low 16 bits).
+enum : unsigned {
+	TVF_VALID = 1u << 16,  // the slot holds a pixel
+	TVF_TILE = 1u << 17,   // inside the tile itself (written back, counted in the error)
+	TVF_TOP = 1u << 18,    // image row 0
+	TVF_BOT = 1u << 19,    // image row ny - 1
+	TVF_LEFT = 1u << 20,   // image column 0
+	TVF_RIGHT = 1u << 21,  // image column nx - 1
+	TVF_BORDER = TVF_TOP | TVF_BOT | TVF_LEFT | TVF_RIGHT,
+};
+
+// divergence of (v1, v2) with backward differences at an image-border pixel
+// (mask.c:43-92); v1m / v2u are the left / upper neighbours.
+__device__ __forceinline__ float div_border(float v1, float v1m, float v2, float v2u, unsigned f)
 {
-	if (!top && !bot && !left && !right) return (v1[e] - v1[el]) + (v2[e] - v2[eu]);
-	if (top && !left && !right) return v1[e] - v1[el] + v2[e];
-	if (bot && !left && !right) return v1[e] - v1[el] - v2[eu];
-	if (left && !top && !bot) return v1[e] + v2[e] - v2[eu];
-	if (right && !top && !bot) return -v1[el] + v2[e] - v2[eu];
-	if (top && left) return v1[e] + v2[e];
-	if (top && right) return -v1[el] + v2[e];
-	if (bot && left) return v1[e] - v2[eu];
-	return -v1[el] - v2[eu];
+	const bool top = f & TVF_TOP, bot = f & TVF_BOT, left = f & TVF_LEFT, right = f & TVF_RIGHT;
+	if (top && !left && !right) return v1 - v1m + v2;
+	if (bot && !left && !right) return v1 - v1m - v2u;
+	if (left && !top && !bot) return v1 + v2 - v2u;
+	if (right && !top && !bot) return -v1m + v2 - v2u;
+	if (top && left) return v1 + v2;
+	if (top && right) return -v1m + v2;
+	if (bot && left) return v1 - v2u;
+	return -v1m - v2u;
+}
+
+// kk iterations on the LDS tile (planes at L + c * PS, rows of RW floats);
+// BORDER: the tile touches the image border, so the boundary cases of the
+// divergence and gradient apply to some of its pixels.
+template <int RW, int PS, int NPT, int K, bool BORDER>
+__device__ __forceinline__ void tv_iterate(float *L, unsigned (&sl)[NPT], const float (&cwx)[NPT],
+                                           const float (&cwy)[NPT], const float (&cg)[NPT],
+                                           const float (&crc)[NPT], int kk, float l_t, float theta,
+                                           float taut, double (&err)[K])
+{
+	constexpr int U1 = 0, U2 = PS, P11 = 2 * PS, P12 = 3 * PS, P21 = 4 * PS, P22 = 5 * PS;
+	for (int j = 0; j < kk; j++) {
+		// the flags are re-read every iteration (not held as masks across it)
+#pragma unroll
+		for (int k = 0; k < NPT; k++) asm volatile("" : "+v"(sl[k]));
+		// phase A: TH, divergence, u (tvl1flow_lib.c:170-225)
+		double ej = 0.0;
+#pragma unroll
+		for (int k = 0; k < NPT; k++) {
+			const unsigned f = sl[k];
+			if (!(f & TVF_VALID)) continue;
+			const int e = f & 0xffff;
+			const float u1k = L[U1 + e], u2k = L[U2 + e];
+			const float wx = cwx[k], wy = cwy[k], g = cg[k];
+			const float rho = crc[k] + (wx * u1k + wy * u2k);
+			float d1, d2;
+			if (rho < -l_t * g) { d1 = l_t * wx; d2 = l_t * wy; }
+			else if (rho > l_t * g) { d1 = -l_t * wx; d2 = -l_t * wy; }
+			else if (g < TV_GRAD_IS_ZERO) d1 = d2 = 0;
+			else { const float fi = -rho / g; d1 = fi * wx; d2 = fi * wy; }
+			const float v1 = u1k + d1, v2 = u2k + d2;
+			const float a11 = L[P11 + e], a11m = L[P11 + e - 1], a12 = L[P12 + e], a12u = L[P12 + e - RW];
+			const float a21 = L[P21 + e], a21m = L[P21 + e - 1], a22 = L[P22 + e], a22u = L[P22 + e - RW];
+			float dp1, dp2;
+			if (!BORDER || !(f & TVF_BORDER)) {
+				dp1 = (a11 - a11m) + (a12 - a12u);
+				dp2 = (a21 - a21m) + (a22 - a22u);
+			} else {
+				dp1 = div_border(a11, a11m, a12, a12u, f);
+				dp2 = div_border(a21, a21m, a22, a22u, f);
+			}
+			const float n1 = v1 + theta * dp1;
+			const float n2 = v2 + theta * dp2;
+			L[U1 + e] = n1;
+			L[U2 + e] = n2;
+			// squared update of the tile's own pixels (an and-mask, not a branch)
+			const double q = (double)((n1 - u1k) * (n1 - u1k) + (n2 - u2k) * (n2 - u2k));
+			const long long tm = (f & TVF_TILE) ? -1ll : 0ll;
+			ej += __longlong_as_double(__double_as_longlong(q) & tm);
+		}
+#pragma unroll
+		for (int q = 0; q < K; q++) if (q == j) err[q] = ej;
+		__syncthreads();
+		// phase B: forward gradient of the new u, p (tvl1flow_lib.c:228-245,
+		// mask.c:101-144)
+#pragma unroll
+		for (int k = 0; k < NPT; k++) {
+			const unsigned f = sl[k];
+			if (!(f & TVF_VALID)) continue;
+			const int e = f & 0xffff;
+			const float a1 = L[U1 + e], a2 = L[U2 + e];
+			const float r1 = L[U1 + e + 1], d1 = L[U1 + e + RW], r2 = L[U2 + e + 1], d2 = L[U2 + e + RW];
+			float u1x = r1 - a1, u1y = d1 - a1, u2x = r2 - a2, u2y = d2 - a2;
+			if (BORDER) {
+				if (f & TVF_RIGHT) u1x = u2x = 0.f;
+				if (f & TVF_BOT) u1y = u2y = 0.f;
+			}
+			const float g1 = (float)sqrt((double)u1x * u1x + (double)u1y * u1y);
+			const float g2 = (float)sqrt((double)u2x * u2x + (double)u2y * u2y);
+			// the reference adds 1.0 in double and stores to float; for a
+			// float t >= 0 that is the float sum 1 + t: the double sum is
+			// exact for 2^-29 <= t < 2^53, rounds to a value that rounds to 1
+			// below, and to t above, as the float sum does
+			const float ng1 = 1.0f + taut * g1;
+			const float ng2 = 1.0f + taut * g2;
+			L[P11 + e] = (L[P11 + e] + taut * u1x) / ng1;
+			L[P12 + e] = (L[P12 + e] + taut * u1y) / ng1;
+			L[P21 + e] = (L[P21 + e] + taut * u2x) / ng2;
+			L[P22 + e] = (L[P22 + e] + taut * u2y) / ng2;
+		}
+		__syncthreads();
+	}
 }
 
 template <int TX, int TY, int K, int NT>
-__global__ __launch_bounds__(NT)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 void tv_iter_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1wy,
                     const float *__restrict__ grad, const float *__restrict__ rho_c, TVPlanes A, TVPlanes B,
                     int nx, int ny, int stride, int tiles_x, float l_t, float theta, float taut,
                     const int *__restrict__ ctl, const int *__restrict__ cur, double *__restrict__ part)
 {
-	constexpr int EN = (TX + 2 * K) * (TY + 2 * K), NPT = (EN + NT - 1) / NT, NW = NT / 64;
-	__shared__ float U1[EN], U2[EN], Q11[EN], Q12[EN], Q21[EN], Q22[EN];
+	// LDS: six planes u1, u2, p11, p12, p21, p22 of the tile with its halo,
+	// rows of RW floats, one spare row before and after so that the
+	// neighbours of every slot are inside the plane
+	constexpr int RW = TX + 2 * K, RH = TY + 2 * K, EN = RW * RH, PS = EN + 2 * RW;
+	constexpr int NPT = (EN + NT - 1) / NT, NW = NT / 64;
+	constexpr int U1 = 0, U2 = PS, P11 = 2 * PS, P12 = 3 * PS, P21 = 4 * PS, P22 = 5 * PS;
+	__shared__ float L[6 * PS];
 	__shared__ double red[NW][K];
 	const int pr = blockIdx.y;
 	const int kk = ctl[pr];
@@ -308,93 +406,54 @@ void tv_iter_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1
 	const int tx0 = (tile % tiles_x) * TX, ty0 = (tile / tiles_x) * TY;
 	const int xs = max(0, tx0 - K), ys = max(0, ty0 - K);
 	const int EW = min(nx, tx0 + TX + K) - xs, EH = min(ny, ty0 + TY + K) - ys, en = EW * EH;
-	const int ix1 = min(nx, tx0 + TX), iy1 = min(ny, ty0 + TY);
 	const bool flip = cur[pr] != 0;
-	const TVPlanes S = flip ? B : A, D = flip ? A : B;
+	const TVPlanes S = flip ? B : A;
 	const size_t off = (size_t)pr * stride;
 	float cwx[NPT], cwy[NPT], cg[NPT], crc[NPT];
-	int cxy[NPT];
+	unsigned sl[NPT];
 #pragma unroll
 	for (int k = 0; k < NPT; k++) {
-		const int e = tid + k * NT;
-		cxy[k] = -1;
-		if (e < en) {
-			const int ey = e / EW, ex = e - ey * EW;
-			const size_t gi = off + (size_t)(ys + ey) * nx + (xs + ex);
+		const int q = tid + k * NT;
+		sl[k] = 0;
+		cwx[k] = cwy[k] = cg[k] = crc[k] = 0.f;
+		if (q < en) {
+			const int ey = q / EW, ex = q - ey * EW;
+			const int gx = xs + ex, gy = ys + ey;
+			const int e = RW + ey * RW + ex;
+			const size_t gi = off + (size_t)gy * nx + gx;
 			cwx[k] = I1wx[gi]; cwy[k] = I1wy[gi]; cg[k] = grad[gi]; crc[k] = rho_c[gi];
-			U1[e] = S.u1[gi]; U2[e] = S.u2[gi];
-			Q11[e] = S.p11[gi]; Q12[e] = S.p12[gi]; Q21[e] = S.p21[gi]; Q22[e] = S.p22[gi];
-			cxy[k] = ex | (ey << 16);
+			L[U1 + e] = S.u1[gi]; L[U2 + e] = S.u2[gi];
+			L[P11 + e] = S.p11[gi]; L[P12 + e] = S.p12[gi]; L[P21 + e] = S.p21[gi]; L[P22 + e] = S.p22[gi];
+			unsigned f = (unsigned)e | TVF_VALID;
+			if (gx >= tx0 && gx < tx0 + TX && gy >= ty0 && gy < ty0 + TY) f |= TVF_TILE;
+			if (gy == 0) f |= TVF_TOP;
+			if (gy == ny - 1) f |= TVF_BOT;
+			if (gx == 0) f |= TVF_LEFT;
+			if (gx == nx - 1) f |= TVF_RIGHT;
+			sl[k] = f;
 		}
 	}
 	double err[K];
 #pragma unroll
 	for (int j = 0; j < K; j++) err[j] = 0.0;
 	__syncthreads();
-	for (int j = 0; j < kk; j++) {
-		// phase A: TH, divergence, u (tvl1flow_lib.c:170-225)
-		double ej = 0.0;
-#pragma unroll
-		for (int k = 0; k < NPT; k++) {
-			if (cxy[k] < 0) continue;
-			const int ex = cxy[k] & 0xffff, ey = cxy[k] >> 16, e = ey * EW + ex;
-			const int gx = xs + ex, gy = ys + ey;
-			const float u1k = U1[e], u2k = U2[e];
-			const float wx = cwx[k], wy = cwy[k], g = cg[k];
-			const float rho = crc[k] + (wx * u1k + wy * u2k);
-			float d1, d2;
-			if (rho < -l_t * g) { d1 = l_t * wx; d2 = l_t * wy; }
-			else if (rho > l_t * g) { d1 = -l_t * wx; d2 = -l_t * wy; }
-			else if (g < TV_GRAD_IS_ZERO) d1 = d2 = 0;
-			else { const float fi = -rho / g; d1 = fi * wx; d2 = fi * wy; }
-			const float v1 = u1k + d1, v2 = u2k + d2;
-			const int el = ex > 0 ? e - 1 : e, eu = ey > 0 ? e - EW : e;
-			const bool top = gy == 0, bot = gy == ny - 1, left = gx == 0, right = gx == nx - 1;
-			const float dp1 = div_lds(Q11, Q12, e, el, eu, top, bot, left, right);
-			const float dp2 = div_lds(Q21, Q22, e, el, eu, top, bot, left, right);
-			const float n1 = v1 + theta * dp1;
-			const float n2 = v2 + theta * dp2;
-			U1[e] = n1;
-			U2[e] = n2;
-			if (gx >= tx0 && gx < ix1 && gy >= ty0 && gy < iy1)
-				ej += (double)((n1 - u1k) * (n1 - u1k) + (n2 - u2k) * (n2 - u2k));
-		}
-#pragma unroll
-		for (int q = 0; q < K; q++) if (q == j) err[q] = ej;
-		__syncthreads();
-		// phase B: forward gradient of the new u, p (tvl1flow_lib.c:228-245)
-#pragma unroll
-		for (int k = 0; k < NPT; k++) {
-			if (cxy[k] < 0) continue;
-			const int ex = cxy[k] & 0xffff, ey = cxy[k] >> 16, e = ey * EW + ex;
-			const int gx = xs + ex, gy = ys + ey;
-			const int er = ex + 1 < EW ? e + 1 : e, ed = ey + 1 < EH ? e + EW : e;
-			const float a1 = U1[e], a2 = U2[e];
-			const float u1x = gx < nx - 1 ? U1[er] - a1 : 0.f;
-			const float u1y = gy < ny - 1 ? U1[ed] - a1 : 0.f;
-			const float u2x = gx < nx - 1 ? U2[er] - a2 : 0.f;
-			const float u2y = gy < ny - 1 ? U2[ed] - a2 : 0.f;
-			const float g1 = (float)sqrt((double)u1x * u1x + (double)u1y * u1y);
-			const float g2 = (float)sqrt((double)u2x * u2x + (double)u2y * u2y);
-			const float ng1 = 1.0 + taut * g1;
-			const float ng2 = 1.0 + taut * g2;
-			Q11[e] = (Q11[e] + taut * u1x) / ng1;
-			Q12[e] = (Q12[e] + taut * u1y) / ng1;
-			Q21[e] = (Q21[e] + taut * u2x) / ng2;
-			Q22[e] = (Q22[e] + taut * u2y) / ng2;
-		}
-		__syncthreads();
-	}
+	// block-uniform: does the tile with its halo reach the image border
+	if (xs == 0 || ys == 0 || xs + EW == nx || ys + EH == ny)
+		tv_iterate<RW, PS, NPT, K, true>(L, sl, cwx, cwy, cg, crc, kk, l_t, theta, taut, err);
+	else
+		tv_iterate<RW, PS, NPT, K, false>(L, sl, cwx, cwy, cg, crc, kk, l_t, theta, taut, err);
 	// the tile itself back to the other buffer
+	const TVPlanes D = flip ? A : B;
 #pragma unroll
 	for (int k = 0; k < NPT; k++) {
-		if (cxy[k] < 0) continue;
-		const int ex = cxy[k] & 0xffff, ey = cxy[k] >> 16, e = ey * EW + ex;
-		const int gx = xs + ex, gy = ys + ey;
-		if (gx < tx0 || gx >= ix1 || gy < ty0 || gy >= iy1) continue;
-		const size_t gi = off + (size_t)gy * nx + gx;
-		D.u1[gi] = U1[e]; D.u2[gi] = U2[e];
-		D.p11[gi] = Q11[e]; D.p12[gi] = Q12[e]; D.p21[gi] = Q21[e]; D.p22[gi] = Q22[e];
+		const unsigned f = sl[k];
+		if (!(f & TVF_TILE)) continue;
+		const int q = tid + k * NT;
+		const int ey = q / EW, ex = q - ey * EW;
+		const int e = f & 0xffff;
+		const size_t gi = off + (size_t)(ys + ey) * nx + (xs + ex);
+		D.u1[gi] = L[U1 + e]; D.u2[gi] = L[U2 + e];
+		D.p11[gi] = L[P11 + e]; D.p12[gi] = L[P12 + e]; D.p21[gi] = L[P21 + e]; D.p22[gi] = L[P22 + e];
 	}
 	// squared updates of the tile, per iteration
 #pragma unroll
