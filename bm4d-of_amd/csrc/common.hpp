@@ -38,6 +38,18 @@ struct StepConst {
 // counted from tb0.
 struct AggRegion { int RX, RY, RT, tb0; };
 
+// Binned ordered aggregation: bins of 8 x 8 pixels x AGG_TB frames; a bin
+// gathers the rows of at most AGG_BPOS patch positions ((7 + sPx)^2 x
+// (AGG_TB + sPt - 1)).
+#define AGG_TB 4
+#define AGG_BPOS 1024
+
+// stable radix sort passes (sortscan.hip): threads per block, pairs per
+// thread and per block (tile)
+#define RS_THREADS 256
+#define RS_IPT 16
+#define RS_TILE (RS_THREADS * RS_IPT)
+
 // Decode a 3-index position t*WH + y*W + x.
 __host__ __device__ inline void pos_decode(uint32_t pos, int W, int WH, int &x, int &y, int &t)
 {

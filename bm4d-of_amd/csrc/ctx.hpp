@@ -70,7 +70,7 @@ struct bm4dof_ctx {
 	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
 	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
 	    io_final, planes, planes_b, coef_n, coef_b, gacc, bsplit, ties, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx, stat64, lastf, fidx, geo, scan_tmp,
-	    scan_tmp2;
+	    scan_tmp2, bin_cnt, bin_poff, bin_k0, bin_k1, bin_v1, bin_hist, bin_hoff, bin_off, bin_list;
 	// timings of the last run
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
@@ -97,6 +97,7 @@ struct bm4dof_ctx {
 	int bm_frames = 2;         // frames of search window staged per round of the staged matching
 	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
 	int agg_cap = 1024;        // rows per e-range of the chunked aggregation (one channel)
+	bool agg_binned = true;    // binned aggregation (row lists per 8 x 8 x 4 bin) where the shape allows it
 	bool vbm3d_search = false; // block matching of a VBM3D_SEARCH build (predictive search)
 	// learned gain network (mlp.hip)
 	bm4dof::DevBuf mlp_theta, mlp_m, mlp_v, mlp_wq, mlp_inf, mlp_ws, mlp_grad, mlp_xb, mlp_tb, mlp_sse;
@@ -120,7 +121,8 @@ struct bm4dof_ctx {
 		        &wsum, &basis, &binv, &win, &noisy_w, &basic_w, &out_w, &io_noisy, &io_fflow, &io_bflow,
 		        &io_basic, &io_final, &planes, &planes_b, &coef_n, &coef_b, &gacc, &bsplit, &ties, &bm_image,
 		        &bm_wins, &bm_wmeta, &erec, &gflag, &fmask, &lp_idx, &stat64, &lastf, &fidx, &geo, &scan_tmp,
-		        &scan_tmp2, &mlp_theta, &mlp_m, &mlp_v, &mlp_wq, &mlp_inf,
+		        &scan_tmp2, &bin_cnt, &bin_poff, &bin_k0, &bin_k1, &bin_v1, &bin_hist, &bin_hoff, &bin_off,
+		        &bin_list, &mlp_theta, &mlp_m, &mlp_v, &mlp_wq, &mlp_inf,
 		        &mlp_ws, &mlp_grad, &mlp_xb, &mlp_tb, &mlp_sse, &tv_frames, &tv_pairs, &tv_part, &tv_mm,
 		        &tv_state, &tv_work, &tv_pyr, &tv_taps};
 	}

@@ -94,6 +94,16 @@ template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
 	StepConst, int, int, float *, float *, uint32_t *
 __global__ void pack_rows_kernel(const uint32_t *, const uint32_t *, const float *, int64_t, uint4 *);
 template <int CC, int CAP, int WP> __global__ void agg_chunk_kernel(AGG2_ARGS);
+__global__ void bin_pair_count_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, uint32_t *);
+__global__ void bin_pair_emit_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, int, int,
+                                     const uint32_t *, uint32_t *, uint32_t *);
+__global__ void bin_bounds_kernel(const uint32_t *, const uint32_t *, int64_t, int, uint32_t *);
+__global__ void rs_hist_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
+__global__ void rs_scatter_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int, const uint32_t *,
+                                  uint32_t *, uint32_t *);
+#define AGGB_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, StepConst, \
+	int, int, int, float *, float *
+template <int CC, int NBR, int PQ, int SQ> __global__ void agg_bin_fold_kernel(AGGB_ARGS);
 __global__ void band_bounds_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
 __global__ void normalize_kernel(const float *, const float *, const float *, StepConst,
                                  float *);
@@ -631,6 +641,88 @@ void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n,
 	HIPCHK(hipGetLastError());
 }
 
+// Binned aggregation: shapes it takes (64-sample patches 4x4x4 or 8x8x1,
+// one block overlapping at most 2 x 2 tile interiors) and the list sizes
+// (at most 8 bins per row, pair counts in 32 bits).
+bool agg_binned_ok(const bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub)
+{
+	return ctx->agg_binned && ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8 &&
+	       ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1)) &&
+	       (sc.C == 1 || sc.C == 3 || sc.C == 4) && 8 * Mub < ((int64_t)1 << 32) - RS_TILE;
+}
+
+struct BinGrid { int nbx, nby, nbins; };
+BinGrid bin_grid(const StepConst &sc)
+{
+	BinGrid g;
+	g.nbx = (sc.W + 7) / 8;
+	g.nby = (sc.H + 7) / 8;
+	g.nbins = g.nbx * g.nby * ((sc.F + AGG_TB - 1) / AGG_TB);
+	return g;
+}
+
+// Device buffers of the bin lists for Mub rows: per-row pair counts and
+// offsets, pairs (two key / value buffers), digit counts per radix tile,
+// list starts per bin.  The sorted values end in bin_v0.
+struct BinBufs {
+	uint32_t *pcnt, *poff, *k0, *v0, *k1, *v1, *hist, *hoff, *boff;
+	int nblk;
+};
+BinBufs bin_bufs(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub)
+{
+	const BinGrid bg = bin_grid(sc);
+	const size_t npair = (size_t)8 * Mub;
+	BinBufs q;
+	q.nblk = (int)((npair + RS_TILE - 1) / RS_TILE);
+	q.pcnt = ctx->bin_cnt.as<uint32_t>((size_t)Mub + 1);
+	q.poff = ctx->bin_poff.as<uint32_t>((size_t)Mub + 1);
+	q.k0 = ctx->bin_k0.as<uint32_t>(npair);
+	q.v0 = ctx->bin_list.as<uint32_t>(npair);
+	q.k1 = ctx->bin_k1.as<uint32_t>(npair);
+	q.v1 = ctx->bin_v1.as<uint32_t>(npair);
+	q.hist = ctx->bin_hist.as<uint32_t>((size_t)256 * q.nblk);
+	q.hoff = ctx->bin_hoff.as<uint32_t>((size_t)256 * q.nblk + 1);
+	q.boff = ctx->bin_off.as<uint32_t>((size_t)bg.nbins + 1);
+	return q;
+}
+
+// Each bin's rows in increasing e (on the side stream, beside the filter):
+// (bin, e) pairs in increasing e, a stable radix sort by bin, list starts.
+void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint32_t *epos, const uint32_t *Mp)
+{
+	const BinGrid bg = bin_grid(sc);
+	const BinBufs q = bin_bufs(ctx, sc, Mub);
+	hipStream_t st = ctx->side;
+	hipLaunchKernelGGL(bin_pair_count_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, st, epos, Mp, Mub, sc, q.pcnt);
+	dscan(ctx, 0, q.pcnt, Mub, Mub + 1, q.poff, true);
+	const uint32_t *np = q.poff + Mub;   // pair count
+	int bits = 1;
+	while ((1ll << bits) < (int64_t)bg.nbins) bits++;
+	const int passes = (bits + 7) / 8;
+	// the passes alternate between the buffers and end in bin_k0 / bin_list
+	uint32_t *kin = q.k0, *vin = q.v0, *kout = q.k1, *vout = q.v1;
+	if (passes % 2) {
+		std::swap(kin, kout);
+		std::swap(vin, vout);
+	}
+	hipLaunchKernelGGL(bin_pair_emit_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, st, epos, Mp, Mub, sc, bg.nbx,
+	                   bg.nby, q.poff, kin, vin);
+	HIPCHK(hipGetLastError());
+	for (int ps = 0; ps < passes; ps++) {
+		hipLaunchKernelGGL(rs_hist_kernel, dim3(q.nblk), dim3(RS_THREADS), 0, st, kin, np, 8 * ps, q.nblk, q.hist);
+		dscan(ctx, 0, q.hist, (int64_t)256 * q.nblk, (int64_t)256 * q.nblk, q.hoff, true);
+		hipLaunchKernelGGL(rs_scatter_kernel, dim3(q.nblk), dim3(RS_THREADS), 0, st, kin, vin, np, 8 * ps, q.nblk,
+		                   q.hoff, kout, vout);
+		HIPCHK(hipGetLastError());
+		std::swap(kin, kout);
+		std::swap(vin, vout);
+	}
+	const int64_t nub = 8 * Mub;
+	hipLaunchKernelGGL(bin_bounds_kernel, dim3(nblk(nub + 1, 256)), dim3(256), 0, st, q.k0, np, nub, bg.nbins,
+	                   q.boff);
+	HIPCHK(hipGetLastError());
+}
+
 // Adds a chunk's rows, in row order, into the running float sums.  The row
 // count is on the device (*Mp); Mub bounds it.  Bands (row ranges processed
 // one after the other, banded kernel only) start at the first group of every
@@ -650,6 +742,29 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 		hipLaunchKernelGGL(pack_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, epos, etile, ew, Mub,
 		                   d_erec);
 		HIPCHK(hipGetLastError());
+		if (t0c == 0 && t1c == sc.F && agg_binned_ok(ctx, sc, Mub)) {
+			// the bin lists were built on the side stream (agg_bin_lists)
+			const BinGrid bg = bin_grid(sc);
+			const BinBufs q = bin_bufs(ctx, sc, Mub);
+			const uint32_t *boff = q.boff, *blist = q.v0;
+			auto fold = agg_bin_fold_kernel<1, 32, 16, 4>;
+			if (sc.sPx == 4) {
+				if (sc.C == 3) fold = agg_bin_fold_kernel<3, 16, 16, 4>;
+				else if (sc.C == 4) fold = agg_bin_fold_kernel<4, 16, 16, 4>;
+			} else {
+				fold = sc.C == 1 ? agg_bin_fold_kernel<1, 32, 16, 16>
+				                 : sc.C == 3 ? agg_bin_fold_kernel<3, 16, 16, 16> : agg_bin_fold_kernel<4, 16, 16, 16>;
+			}
+			const unsigned nb8 = (unsigned)(((size_t)bg.nbins + 7) / 8 * 8);
+			hipLaunchKernelGGL(fold, dim3(nb8), dim3(64), 0, ctx->stream, boff, blist, d_erec, est, win, sc, bg.nbx,
+			                   bg.nby, bg.nbins, num, wsum);
+			HIPCHK(hipGetLastError());
+			ctx->agg_blocks += (uint64_t)bg.nbins;
+			ctx->agg_region[0] = 8;
+			ctx->agg_region[1] = 8;
+			ctx->agg_region[2] = AGG_TB;
+			return;
+		}
 		const dim3 cgrid((unsigned)(((size_t)(t1c - t0c) * ((sc.W + 7) / 8) * ((sc.H + 7) / 8) + 7) / 8 * 8));
 		// rows held in LDS per e-range: 512 / 768 / 1024 / 1536 (one channel,
 		// patches of up to 64 samples); window table of 64 or 256 samples
@@ -863,6 +978,7 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 	(void)ctx->entry_w.as<float>(Mub);
 	(void)ctx->est.as<float>(Mub * pl.C * pl.sc.P);
 	(void)ctx->erec.as<uint4>(Mub);
+	if (agg_binned_ok(ctx, pl.sc, (int64_t)Mub)) (void)bin_bufs(ctx, pl.sc, (int64_t)Mub);
 	(void)ctx->lastf.as<uint32_t>(G);
 	(void)ctx->fidx.as<uint32_t>(G);
 	(void)ctx->bm_wins.as<int4>(G * sWt);
@@ -1002,15 +1118,19 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		                   d_base, np, g0, Gc, d_epos);
 		HIPCHK(hipGetLastError());
 		hipLaunchKernelGGL(add_rows_kernel, dim3(1), dim3(64), 0, ctx->side, d_M, d_rows);
-		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1, ctx->side);
-		hipLaunchKernelGGL(count_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_epos, d_M, Mub,
-		                   d_pcnt);
-		dscan(ctx, 0, d_pcnt, (int64_t)WHF, (int64_t)WHF + 1, d_kstart, true);
-		fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1, ctx->side);
-		hipLaunchKernelGGL(scatter_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_epos, d_M, Mub,
-		                   d_kstart, d_pcnt, d_svals);
-		hipLaunchKernelGGL(sort_runs_kernel, dim3(nblk((int64_t)WHF, 256)), dim3(256), 0, ctx->side, d_kstart,
-		                   (int64_t)WHF, d_svals);
+		if (agg_binned_ok(ctx, sc, Mub)) {
+			agg_bin_lists(ctx, sc, Mub, d_epos, d_M);
+		} else {
+			fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1, ctx->side);
+			hipLaunchKernelGGL(count_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_epos, d_M, Mub,
+			                   d_pcnt);
+			dscan(ctx, 0, d_pcnt, (int64_t)WHF, (int64_t)WHF + 1, d_kstart, true);
+			fill_u32(ctx, d_pcnt, 0u, (int64_t)WHF + 1, ctx->side);
+			hipLaunchKernelGGL(scatter_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_epos, d_M, Mub,
+			                   d_kstart, d_pcnt, d_svals);
+			hipLaunchKernelGGL(sort_runs_kernel, dim3(nblk((int64_t)WHF, 256)), dim3(256), 0, ctx->side, d_kstart,
+			                   (int64_t)WHF, d_svals);
+		}
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
 		mark("l_sort", "s1_sort", "s2_sort");
@@ -1711,6 +1831,7 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		const std::string k = key ? key : "";
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
 		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
+		else if (k == "agg_binned") ctx->agg_binned = value != 0;
 		else if (k == "fast_bm") {
 			if (value < 0 || value > 2) throw ArgError("fast_bm must be 0, 1 or 2");
 			ctx->fast_bm = (int)value;

@@ -3097,6 +3097,293 @@ __global__ void band_bounds_kernel(const uint32_t *__restrict__ base, const uint
 }
 
 // computeWeightedAggregation, nldct.cpp:2436-2448.
+// ---------------------------------------------------------------------------
+// Ordered aggregation by bins of 8 x 8 pixels x AGG_TB frames.
+//
+// bin_pair_count_kernel / bin_pair_emit_kernel (side stream, beside the
+// filter): every row e emits one (bin, e) pair per bin its patch overlaps,
+// in increasing e; a stable radix sort by bin (sortscan.hip) then leaves
+// each bin's rows in increasing e, and bin_bounds_kernel marks where each
+// bin's list starts.  agg_bin_fold_kernel: one wave per bin, lane = pixel
+// of the 8 x 8 block, AGG_TB running sums per lane (one per frame of the
+// bin).  The bin's list is read in chunks of NBR rows; each row's slices at
+// the bin's frames are staged once in LDS and added, frame by frame, into
+// the lanes whose pixel the row covers, in increasing e: for every pixel the
+// rows come in the order computeAggregationStep{1,2} adds them
+// (nldct.cpp:2284-2300, 2376-2393), one float addition at a time, so the
+// sums are those of the per-frame kernels above.
+// ---------------------------------------------------------------------------
+
+// bins of row e's patch (x blocks, y blocks, frame bins)
+__device__ __forceinline__ void bin_span(uint32_t q, const StepConst &sc, int &bx0, int &bx1, int &by0, int &by1,
+                                         int &bt0, int &bt1)
+{
+	int qx, qy, qt;
+	pos_decode(q, sc.W, sc.WH, qx, qy, qt);
+	bx0 = qx >> 3;
+	bx1 = (qx + sc.sPx - 1) >> 3;
+	by0 = qy >> 3;
+	by1 = (qy + sc.sPx - 1) >> 3;
+	bt0 = qt / AGG_TB;
+	bt1 = (qt + sc.sPt - 1) / AGG_TB;
+}
+
+__global__ void bin_pair_count_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
+                                      int64_t Mub, StepConst sc, uint32_t *__restrict__ cnt)
+{
+	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (e >= Mub) return;
+	uint32_t c = 0;
+	if (e < (int64_t)*Mp) {
+		int bx0, bx1, by0, by1, bt0, bt1;
+		bin_span(epos[e], sc, bx0, bx1, by0, by1, bt0, bt1);
+		c = (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1) * (bt1 - bt0 + 1));
+	}
+	cnt[e] = c;
+}
+
+__global__ void bin_pair_emit_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
+                                     int64_t Mub, StepConst sc, int nbx, int nby,
+                                     const uint32_t *__restrict__ poff, uint32_t *__restrict__ keys,
+                                     uint32_t *__restrict__ vals)
+{
+	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	if (e >= Mub || e >= (int64_t)*Mp) return;
+	int bx0, bx1, by0, by1, bt0, bt1;
+	bin_span(epos[e], sc, bx0, bx1, by0, by1, bt0, bt1);
+	uint32_t o = poff[e];
+	for (int bt = bt0; bt <= bt1; bt++)
+		for (int by = by0; by <= by1; by++)
+			for (int bx = bx0; bx <= bx1; bx++) {
+				keys[o] = (uint32_t)((bt * nby + by) * nbx + bx);
+				vals[o] = (uint32_t)e;
+				o++;
+			}
+}
+
+// boff[b] = number of sorted keys below b, b = 0 .. nbins
+__global__ void bin_bounds_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ np, int64_t nub,
+                                  int nbins, uint32_t *__restrict__ boff)
+{
+	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+	const uint32_t n = *np;
+	if (i > nub || i > (int64_t)n) return;
+	const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1u;
+	const uint32_t hi = i == (int64_t)n ? (uint32_t)nbins : keys[i];
+	for (uint32_t b = lo; b <= hi; b++) boff[b] = (uint32_t)i;
+}
+
+// CC channels; NBR rows per chunk; patches of 4 PQ samples, frame slices of
+// 4 SQ samples (PQ <= 16).
+template <int CC, int NBR, int PQ, int SQ>
+__global__ __launch_bounds__(64)
+void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__restrict__ blist,
+                         const uint4 *__restrict__ erec, const float *__restrict__ est,
+                         const float *__restrict__ win, StepConst sc, int nbx, int nby, int nbins,
+                         float *__restrict__ num, float *__restrict__ wsum)
+{
+	constexpr int TB = AGG_TB;
+	constexpr int P = 4 * PQ, SL = 4 * SQ, QR = CC * PQ;   // QR: float4 of one row
+	constexpr int SPL = (NBR * QR + 63) / 64;              // float4 staged per lane per chunk
+	__shared__ __align__(16) float4 stage4[NBR * QR];
+	const float *stage = reinterpret_cast<const float *>(stage4);
+	__shared__ float swin[P];
+	// per row of a chunk: slice offset of the block pixel (0, 0)
+	// (dy0 sPx + dx0), patch frame qt, weight bits, row index e
+	__shared__ int4 rrec[2][NBR];
+	const int lane = threadIdx.x;
+	// bins are dealt to the 8 XCDs in contiguous eighths (x fastest)
+	const uint32_t per_xcd = ((uint32_t)nbins + 7) / 8;
+	const uint32_t b = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+	if (b >= (uint32_t)nbins) return;
+	const uint32_t k0 = boff[b], k1 = boff[b + 1];
+	if (k0 >= k1) return;
+	const int bx = (int)(b % (uint32_t)nbx);
+	const uint32_t rb = b / (uint32_t)nbx;
+	const int x0 = bx * 8, y0 = (int)(rb % (uint32_t)nby) * 8, t0 = (int)(rb / (uint32_t)nby) * TB;
+	const int sPx = sc.sPx, sPt = sc.sPt;
+	const int lx = lane & 7, ly = lane >> 3;
+	const int x = x0 + lx, y = y0 + ly;
+	const bool valid = x < sc.W && y < sc.H;
+	// the lanes owned by each tile present in the block
+	const int owner = valid ? owner_tile(x, y, sc) : -1;
+	int own_t[4] = {-1, -1, -1, -1};
+	uint64_t own_m[4] = {0, 0, 0, 0};
+	{
+		uint64_t rest = __ballot(valid);
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			if (rest) {
+				const int lead = __ffsll((unsigned long long)rest) - 1;
+				const int o = __shfl(owner, lead, 64);
+				const uint64_t m = __ballot(owner == o);
+				own_t[i] = o;
+				own_m[i] = m;
+				rest &= ~m;
+			}
+		}
+	}
+	float aw[TB], an[TB][CC];
+#pragma unroll
+	for (int f = 0; f < TB; f++) {
+		aw[f] = 0.f;
+#pragma unroll
+		for (int c = 0; c < CC; c++) an[f][c] = 0.f;
+		if (valid && t0 + f < sc.F) {
+			aw[f] = wsum[(size_t)(t0 + f) * sc.WH + (size_t)y * sc.W + x];
+#pragma unroll
+			for (int c = 0; c < CC; c++)
+				an[f][c] = num[(size_t)(t0 + f) * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x];
+		}
+	}
+	for (int i = lane; i < P; i += 64) swin[i] = win[i];
+	const int pix_l = ly * sPx + lx;
+	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
+	const float4 *est4 = reinterpret_cast<const float4 *>(est);
+	const int nch = (int)((k1 - k0 + NBR - 1) / NBR);
+
+	// Software pipeline: the row records of chunk c+2 and the slices of
+	// chunk c+1 are in flight while chunk c is folded.
+	uint32_t eN = 0;
+	uint4 recN = make_uint4(0, 0, 0, 0);
+	auto meta_load = [&](int c) {
+		const uint32_t kc = k0 + (uint32_t)c * NBR;
+		if (c < nch && kc + lane < k1 && lane < NBR) {
+			eN = blist[kc + lane];
+			recN = erec[eN];
+		}
+	};
+	// the chunk's records into rrec[buf]; returns the lane's coverage mask
+	// (lanes = rows) and the rows present at each frame of the bin
+	auto meta_commit = [&](int c, uint64_t (&vm)[TB]) -> uint64_t {
+		const uint32_t kc = k0 + (uint32_t)c * NBR;
+		const bool rv = lane < NBR && kc + lane < k1;
+		uint64_t msp = 0;
+		int qt = 0;
+		if (rv) {
+			const uint32_t q = recN.x;
+			qt = (int)udiv_rcp(q, (uint32_t)sc.WH, rcpWH);
+			const uint32_t rr = q - (uint32_t)qt * (uint32_t)sc.WH;
+			const int qy = (int)udiv_rcp(rr, (uint32_t)sc.W, rcpW);
+			const int qx = (int)(rr - (uint32_t)qy * (uint32_t)sc.W);
+			const int dx0 = qx - x0, dy0 = qy - y0;
+			const uint32_t cols = span8(dx0, dx0 + sPx), rows = span8(dy0, dy0 + sPx);
+			uint64_t rsel = 0;
+#pragma unroll
+			for (int r = 0; r < 8; r++) if ((rows >> r) & 1u) rsel |= 1ull << (8 * r);
+			msp = (uint64_t)cols * rsel;
+			uint64_t om = 0;
+#pragma unroll
+			for (int i = 0; i < 4; i++) if (own_t[i] == (int)recN.y) om = own_m[i];
+			msp &= om;
+			rrec[c & 1][lane] = make_int4(dy0 * sPx + dx0, qt, (int)recN.z, (int)eN);
+		}
+#pragma unroll
+		for (int f = 0; f < TB; f++) {
+			const int dt = t0 + f - qt;
+			vm[f] = __ballot(rv && msp != 0 && dt >= 0 && dt < sPt);
+		}
+		return msp;
+	};
+	// staged slices of a chunk: float4 i = lane + 64 m of the chunk's rows
+	// (row i / QR), for the rows with a frame in the bin, the slices at the
+	// bin's frames; the rest of a row's slot is not read
+	float4 sv[SPL];
+#define AGG_BIN_STAGE_ISSUE(c, vm)                                                                  \
+	{                                                                                               \
+		const uint64_t any_ = vm[0] | vm[1] | vm[2] | vm[3];                                        \
+		_Pragma("unroll") for (int m = 0; m < SPL; m++) {                                         \
+			const int i = lane + 64 * m;                                                            \
+			const int j = i / QR, rq = i - j * QR;                                                  \
+			sv[m] = make_float4(0.f, 0.f, 0.f, 0.f);                                                \
+			if (j < NBR && ((any_ >> j) & 1ull)) {                                                  \
+				const int4 r = rrec[(c) & 1][j];                                                    \
+				const int dt = (rq % PQ) / SQ;                                                      \
+				if (dt >= t0 - r.y && dt <= t0 + TB - 1 - r.y)                                      \
+					sv[m] = est4[(size_t)(uint32_t)r.w * QR + rq];                                  \
+			}                                                                                       \
+		}                                                                                           \
+	}
+#define AGG_BIN_STAGE_WRITE()                                                                       \
+	{                                                                                               \
+		_Pragma("unroll") for (int m = 0; m < SPL; m++) {                                         \
+			const int i = lane + 64 * m;                                                            \
+			if (i < NBR * QR) stage4[i] = sv[m];                                                    \
+		}                                                                                           \
+	}
+	uint64_t vmC[TB], vmN[TB];
+	meta_load(0);
+	uint64_t mC = meta_commit(0, vmC);
+	meta_load(1);
+	__syncthreads();
+	AGG_BIN_STAGE_ISSUE(0, vmC);
+	AGG_BIN_STAGE_WRITE();
+	for (int c = 0; c < nch; c++) {
+		uint64_t mN = 0;
+		if (c + 1 < nch) mN = meta_commit(c + 1, vmN);
+		meta_load(c + 2);
+		__syncthreads();
+		if (c + 1 < nch) AGG_BIN_STAGE_ISSUE(c + 1, vmN);
+		// each lane folds the rows that cover its pixel, frame by frame, in
+		// row order, two rows per round (their loads overlap)
+		const uint64_t mine_sp = transpose_bits64(mC);
+#pragma unroll
+		for (int f = 0; f < TB; f++) {
+			uint64_t mine = mine_sp & vmC[f];
+			while (mine) {
+				const int j1 = __ffsll((unsigned long long)mine) - 1;
+				mine &= mine - 1;
+				const bool two = mine != 0;
+				const int j2 = two ? __ffsll((unsigned long long)mine) - 1 : j1;
+				if (two) mine &= mine - 1;
+				const int4 r1 = rrec[c & 1][j1], r2 = rrec[c & 1][j2];
+				const int s1 = (t0 + f - r1.y) * SL + pix_l - r1.x;
+				const int s2 = (t0 + f - r2.y) * SL + pix_l - r2.x;
+				const float v1 = swin[s1], v2 = swin[s2];
+				float x1[CC], x2[CC];
+#pragma unroll
+				for (int cc = 0; cc < CC; cc++) {
+					x1[cc] = stage[(j1 * CC + cc) * P + s1];
+					x2[cc] = stage[(j2 * CC + cc) * P + s2];
+				}
+				aw[f] = aw[f] + __int_as_float(r1.z) * v1;
+#pragma unroll
+				for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + x1[cc];
+				if (two) {
+					aw[f] = aw[f] + __int_as_float(r2.z) * v2;
+#pragma unroll
+					for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + x2[cc];
+				}
+			}
+		}
+		__syncthreads();
+		if (c + 1 < nch) AGG_BIN_STAGE_WRITE();
+		mC = mN;
+#pragma unroll
+		for (int f = 0; f < TB; f++) vmC[f] = vmN[f];
+	}
+#pragma unroll
+	for (int f = 0; f < TB; f++)
+		if (valid && t0 + f < sc.F) {
+			wsum[(size_t)(t0 + f) * sc.WH + (size_t)y * sc.W + x] = aw[f];
+#pragma unroll
+			for (int c = 0; c < CC; c++)
+				num[(size_t)(t0 + f) * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x] = an[f][c];
+		}
+}
+
+#undef AGG_BIN_STAGE_ISSUE
+#undef AGG_BIN_STAGE_WRITE
+
+#define AGGB_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, StepConst, \
+	int, int, int, float *, float *
+template __global__ void agg_bin_fold_kernel<1, 32, 16, 4>(AGGB_ARGS);
+template __global__ void agg_bin_fold_kernel<1, 32, 16, 16>(AGGB_ARGS);
+template __global__ void agg_bin_fold_kernel<3, 16, 16, 4>(AGGB_ARGS);
+template __global__ void agg_bin_fold_kernel<3, 16, 16, 16>(AGGB_ARGS);
+template __global__ void agg_bin_fold_kernel<4, 16, 16, 4>(AGGB_ARGS);
+template __global__ void agg_bin_fold_kernel<4, 16, 16, 16>(AGGB_ARGS);
+
 __global__ void normalize_kernel(const float *__restrict__ num, const float *__restrict__ wsum,
                                  const float *__restrict__ noisy, StepConst sc,
                                  float *__restrict__ out)

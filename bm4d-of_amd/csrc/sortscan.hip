@@ -225,4 +225,82 @@ __global__ void drop_groups_kernel(const uint32_t *__restrict__ drop, int n, uin
 	if (i < n) nsim[drop[i]] = 0u;
 }
 
+// ------------------------------------------------------- stable radix sort
+// One LSD pass of 8-bit digits (bits [shift, shift + 8)) over (key, value)
+// pairs; the pair count n is on the device, nblk tiles of RS_TILE pairs
+// bound it.  rs_hist_kernel: digit counts per tile, digit-major
+// (hist[d * nblk + tile]); an exclusive scan of them gives every tile's
+// first output slot per digit; rs_scatter_kernel: each tile writes its
+// pairs there in input order (equal digits keep their order: stable).
+__global__ __launch_bounds__(RS_THREADS)
+void rs_hist_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ np, int shift, int nblk,
+                    uint32_t *__restrict__ hist)
+{
+	__shared__ uint32_t h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const uint64_t n = *np;
+	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
+	if (b0 < n)
+		for (int i = 0; i < RS_IPT; i++) {
+			const uint64_t k = b0 + (uint64_t)i * RS_THREADS + threadIdx.x;
+			if (k < n) atomicAdd(&h[(keys[k] >> shift) & 255u], 1u);
+		}
+	__syncthreads();
+	hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS)
+void rs_scatter_kernel(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                       const uint32_t *__restrict__ np, int shift, int nblk, const uint32_t *__restrict__ offs,
+                       uint32_t *__restrict__ kout, uint32_t *__restrict__ vout)
+{
+	constexpr int NW = RS_THREADS / 64;
+	__shared__ uint32_t base[256];
+	__shared__ uint32_t wcnt[NW][256];
+	const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+	const uint64_t n = *np;
+	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
+	if (b0 >= n) return;
+	base[tid] = offs[(size_t)tid * nblk + blockIdx.x];
+	const uint64_t lt = (1ull << lane) - 1ull;
+	for (int r = 0; r < RS_IPT; r++) {
+		const uint64_t k = b0 + (uint64_t)r * RS_THREADS + tid;
+		if (b0 + (uint64_t)r * RS_THREADS >= n) break;   // uniform over the block
+#pragma unroll
+		for (int w = 0; w < NW; w++) wcnt[w][tid] = 0;
+		__syncthreads();
+		const bool valid = k < n;
+		uint32_t key = 0, val = 0, d = 0;
+		if (valid) {
+			key = kin[k];
+			val = vin[k];
+			d = (key >> shift) & 255u;
+		}
+		// the lanes of this wave with the same digit
+		uint64_t m = __ballot(valid);
+#pragma unroll
+		for (int bit = 0; bit < 8; bit++) {
+			const bool bs = (d >> bit) & 1u;
+			const uint64_t bb = __ballot(valid && bs);
+			m &= bs ? bb : ~bb;
+		}
+		const uint32_t rk = (uint32_t)__popcll(m & lt);
+		if (valid && rk == 0) wcnt[wv][d] = (uint32_t)__popcll(m);
+		__syncthreads();
+		if (valid) {
+			uint32_t pos = base[d] + rk;
+			for (int w = 0; w < wv; w++) pos += wcnt[w][d];
+			kout[pos] = key;
+			vout[pos] = val;
+		}
+		__syncthreads();
+		uint32_t t = 0;
+#pragma unroll
+		for (int w = 0; w < NW; w++) t += wcnt[w][tid];
+		base[tid] += t;
+		__syncthreads();
+	}
+}
+
 } // namespace bm4dof

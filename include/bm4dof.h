@@ -119,6 +119,13 @@ const char *bm4dof_global_error(void);
  *                  high); 0: the banded aggregation for every block.
  *   "agg_cap"      rows sorted per step of the chunked aggregation, for one
  *                  channel: 512, 768, 1024 (default) or 1536.
+ *   "agg_binned"   1 (default): where the chunked aggregation applies and
+ *                  patches have at most 64 samples (at most 8x8 pixels and
+ *                  4 frames, 1, 3 or 4 channels), each bin of 8 x 8 pixels
+ *                  x 4 frames gets its rows as one list in row order, built
+ *                  beside the filter, and one wave folds the list into the
+ *                  bin's four frames; 0: the chunked aggregation sorts the
+ *                  rows of every 8 x 8 x 1 block itself.
  *   "coef_budget"  bytes of device memory a step of the register filters may
  *                  hold in per-anchor coefficient tables (the forward
  *                  transform of every patch position, computed once per

@@ -184,6 +184,34 @@ def test_chunked_and_banded_aggregation_agree(ctx, w, h, f, nparts, px, pt):
     assert np.array_equal(b0, b1) and np.array_equal(f0, f1)
 
 
+@pytest.mark.parametrize("w,h,f,c,nparts,px,pt", [(96, 64, 8, 1, 4, 4, 4), (50, 38, 7, 1, 6, 4, 4),
+                                                   (80, 72, 1, 1, 4, 8, 1), (72, 56, 9, 1, 2, 4, 2),
+                                                   (61, 45, 6, 3, 2, 4, 4), (64, 48, 5, 4, 4, 4, 4),
+                                                   (88, 40, 12, 1, 2, 4, 3)])
+def test_binned_and_chunked_aggregation_agree(ctx, w, h, f, c, nparts, px, pt):
+    """The binned aggregation (row lists per 8 x 8 x 4 bin from a stable
+    radix sort of (bin, row) pairs, four frames folded per wave) and the
+    per-frame chunked one fold the same rows in the same order: bitwise equal
+    output for gray and colour videos, frame counts that do not fill the last
+    bin, blocks straddling 2 x 2 tiles."""
+    clean, noisy = _video(w, h, f, seed=w * h + f + c, channels=c)
+    ff = bf = None
+    if f > 1:
+        ff, bf = _smooth_flow(f, h, w, 15), _smooth_flow(f, h, w, 16)
+    a1 = A.cli_params(1, 20, noisy.shape, px=px, pt=pt, wx=9, np_=16)
+    a2 = A.cli_params(2, 20, noisy.shape, px=px, pt=pt, wx=9, np_=16)
+    try:
+        ctx.set_option("agg_binned", 0)
+        b0, f0, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=nparts)
+    finally:
+        ctx.set_option("agg_binned", 1)
+    b1, f1, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=nparts)
+    # bins take the 64-sample patches (4x4x4, 8x8x1); other shapes keep the
+    # per-frame blocks
+    assert ctx.stats()["agg_bt"] == (4 if (px, pt) in ((4, 4), (8, 1)) else 1)
+    assert np.array_equal(b0, b1, equal_nan=True) and np.array_equal(f0, f1, equal_nan=True)
+
+
 @pytest.mark.parametrize("px,pt,flow,nparts,wx,flat", [(4, 4, True, 4, 11, False), (8, 1, False, 2, 11, False),
                                                        (4, 4, False, 6, 11, False), (4, 4, True, 2, 11, True),
                                                        (8, 1, False, 1, 21, True), (8, 1, False, 2, 39, False),
