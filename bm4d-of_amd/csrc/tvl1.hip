@@ -219,13 +219,17 @@ __global__ void tv_cgrad_kernel(const float *__restrict__ I1, int nx, int ny, in
 // (tvl1flow_lib.c:142-161).
 __global__ void tv_warp_kernel(const float *__restrict__ I0, const float *__restrict__ I1,
                                const float *__restrict__ I1x, const float *__restrict__ I1y,
-                               const float *__restrict__ u1, const float *__restrict__ u2,
-                               int nx, int ny, int stride, float *__restrict__ I1wx,
-                               float *__restrict__ I1wy, float *__restrict__ grad,
+                               const float *__restrict__ u1a, const float *__restrict__ u2a,
+                               const float *__restrict__ u1b, const float *__restrict__ u2b,
+                               const int *__restrict__ cur, int nx, int ny, int stride,
+                               float *__restrict__ I1wx, float *__restrict__ I1wy, float *__restrict__ grad,
                                float *__restrict__ rho_c)
 {
 	const int pr = blockIdx.y;
 	const size_t off = (size_t)pr * stride;
+	const bool flip = cur[pr] != 0;
+	const float *__restrict__ u1 = flip ? u1b : u1a;
+	const float *__restrict__ u2 = flip ? u2b : u2a;
 	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nx * ny; p += gridDim.x * blockDim.x) {
 		const int i = p / nx, j = p - i * nx;
 		const float uu = (float)(j + u1[off + p]);
@@ -258,11 +262,12 @@ __global__ void tv_warp_kernel(const float *__restrict__ I0, const float *__rest
 //
 // The state (u1, u2, p11, p12, p21, p22) is double-buffered: a pair reads
 // buffer cur[pair] and writes the other.  ctl[pair] is the number of
-// iterations to run this round (0: idle).  tv_round_kernel then applies the
-// stopping test to each iteration of the round in order; if it fires before
-// the round's last iteration, the pair runs that many iterations again from
-// the same source buffer (an exact re-run), so the state and the iteration
-// count are those of the iteration the test fired on.
+// iterations to run this round (0: idle).  tv_round_kernel then sums the
+// tiles' partials and applies the stopping test to each iteration of the
+// round in order (tv_round_decide); if it fires before the round's last
+// iteration, the pair runs that many iterations again from the same source
+// buffer (an exact re-run), so the state and the iteration count are those
+// of the iteration the test fired on.
 // ---------------------------------------------------------------------------
 struct TVPlanes {
 	float *u1, *u2, *p11, *p12, *p21, *p22;
@@ -273,6 +278,46 @@ struct TVPlanes {
 #define TV_K 4
 #define TV_NT 512
 enum { TV_RUN = 0, TV_REDO = 1 };
+
+// The stopping test of every iteration of a round, in order
+// (tvl1flow_lib.c:165, 225), given errs[j] = the mean squared update of
+// iteration j: the loop ends when error <= epsilon^2 or the count reaches
+// MAX_ITERATIONS.  state[pair] receives the final count.  A round that
+// stops before its last iteration is run again from the same source buffer
+// with that many iterations (mode TV_REDO), so the state and the count are
+// those of the iteration the test fired on.
+__device__ void tv_round_decide(int pr, int kk, const float *errs, float eps2, int K, int *__restrict__ ctl,
+                                int *__restrict__ cur, int *__restrict__ iters, int *__restrict__ mode,
+                                int *__restrict__ state, int *__restrict__ n_active)
+{
+	const int done = iters[pr];
+	if (mode[pr] == TV_REDO) {   // the exact re-run of the iterations up to the test
+		iters[pr] = done + kk;
+		cur[pr] ^= 1;
+		state[pr] = done + kk;
+		ctl[pr] = 0;
+		atomicSub(n_active, 1);
+		return;
+	}
+	int stop = -1;
+	for (int j = 0; j < kk && stop < 0; j++)
+		if (!(errs[j] > eps2) || done + j + 1 >= TV_MAX_ITERATIONS) stop = j;
+	if (stop < 0) {
+		iters[pr] = done + kk;
+		cur[pr] ^= 1;
+		ctl[pr] = min(K, TV_MAX_ITERATIONS - (done + kk));
+	} else if (stop == kk - 1) {
+		iters[pr] = done + kk;
+		cur[pr] ^= 1;
+		state[pr] = done + kk;
+		ctl[pr] = 0;
+		atomicSub(n_active, 1);
+	} else {
+		mode[pr] = TV_REDO;
+		ctl[pr] = stop + 1;
+	}
+}
+
 
 // Per-slot flags of a tile pixel (the LDS index is in the low 16 bits).
 enum : unsigned {
@@ -471,10 +516,9 @@ void tv_iter_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1
 	}
 }
 
-// The stopping test of every iteration of a round, in order
-// (tvl1flow_lib.c:165, 225): error = (sum over the tiles, in tile order) /
-// size, and the loop ends when error <= epsilon^2 or the count reaches
-// MAX_ITERATIONS.  One block per pair.  state[pair] receives the final count.
+// The stopping test of a round: error of iteration j = (sum over the
+// tiles' partials, in tile order) / size, then tv_round_decide.  One block
+// per pair.
 __global__ void tv_round_kernel(const double *__restrict__ part, int K, int ntiles, int size, float eps2,
                                 int *__restrict__ ctl, int *__restrict__ cur, int *__restrict__ iters,
                                 int *__restrict__ mode, int *__restrict__ state, int *__restrict__ n_active)
@@ -496,60 +540,36 @@ __global__ void tv_round_kernel(const double *__restrict__ part, int K, int ntil
 		if (threadIdx.x == 0) errs[j] = (float)s[0] / (float)size;
 		__syncthreads();
 	}
-	if (threadIdx.x != 0) return;
-	const int done = iters[pr];
-	if (mode[pr] == TV_REDO) {   // the exact re-run of the iterations up to the test
-		iters[pr] = done + kk;
-		cur[pr] ^= 1;
-		state[pr] = done + kk;
-		ctl[pr] = 0;
-		atomicSub(n_active, 1);
-		return;
-	}
-	int stop = -1;
-	for (int j = 0; j < kk && stop < 0; j++)
-		if (!(errs[j] > eps2) || done + j + 1 >= TV_MAX_ITERATIONS) stop = j;
-	if (stop < 0) {
-		iters[pr] = done + kk;
-		cur[pr] ^= 1;
-		ctl[pr] = min(K, TV_MAX_ITERATIONS - (done + kk));
-	} else if (stop == kk - 1) {
-		iters[pr] = done + kk;
-		cur[pr] ^= 1;
-		state[pr] = done + kk;
-		ctl[pr] = 0;
-		atomicSub(n_active, 1);
-	} else {
-		mode[pr] = TV_REDO;
-		ctl[pr] = stop + 1;
-	}
+	if (threadIdx.x == 0) tv_round_decide(pr, kk, errs, eps2, K, ctl, cur, iters, mode, state, n_active);
 }
 
 // Per pair: run state of the solve of one warp.
+// The state stays in the buffer the last round wrote (cur) from one warp to
+// the next; reset_cur at the start of a scale, where it is in the first.
 __global__ void tv_round_init_kernel(int P, int K, int *__restrict__ ctl, int *__restrict__ cur,
                                      int *__restrict__ iters, int *__restrict__ mode, int *__restrict__ state,
-                                     int *__restrict__ n_active)
+                                     int *__restrict__ n_active, int reset_cur)
 {
 	const int pr = blockIdx.x * blockDim.x + threadIdx.x;
 	if (pr == 0) *n_active = P;
 	if (pr >= P) return;
 	ctl[pr] = min(K, TV_MAX_ITERATIONS);
-	cur[pr] = 0;
+	if (reset_cur) cur[pr] = 0;
 	iters[pr] = 0;
 	mode[pr] = TV_RUN;
 	state[pr] = -1;
 }
 
-// Pairs whose last round wrote the second buffer: copy it to the first.
+// End of a scale: pairs whose flow is in the second buffer copy it to the
+// first (the pyramid level the zoom and the output read).
 __global__ void tv_settle_kernel(TVPlanes A, TVPlanes B, int n, int stride, const int *__restrict__ cur)
 {
 	const int pr = blockIdx.y;
 	if (!cur[pr]) return;
 	const size_t off = (size_t)pr * stride;
 	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-		A.u1[off + p] = B.u1[off + p]; A.u2[off + p] = B.u2[off + p];
-		A.p11[off + p] = B.p11[off + p]; A.p12[off + p] = B.p12[off + p];
-		A.p21[off + p] = B.p21[off + p]; A.p22[off + p] = B.p22[off + p];
+		A.u1[off + p] = B.u1[off + p];
+		A.u2[off + p] = B.u2[off + p];
 	}
 }
 
@@ -827,9 +847,12 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	const float eps2 = prm.epsilon * prm.epsilon;
 	const float scale = (float)1.0 / prm.zfactor;
 	std::vector<int> h_state(P);
-	// rounds of TV_K iterations between looks at the active count
-	constexpr int kRoundsPerCheck = 3;
+	// Rounds of TV_K iterations run until the pair's test stops them; the
+	// host reads the active count after a batch of rounds: the first batch
+	// long enough for the previous warp's iteration count, then two rounds
+	// at a time (a round of a finished pair is a launch whose blocks exit).
 	const int max_rounds = (TV_MAX_ITERATIONS + TV_K - 1) / TV_K + 1;
+	int guess = 0;   // most iterations of a pair in the previous warp
 
 	for (int s = nscales - 1; s >= prm.fscale; s--) {
 		// Dual_TVL1_optic_flow, tvl1flow_lib.c:93-275
@@ -845,36 +868,46 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		for (float *q : {p11, p12, p21, p22})
 			hipLaunchKernelGGL(tv_fill_kernel, dim3(tv_grid(nP)), dim3(TV_THREADS), 0, st, q, nP, 0.f);
 		for (int w = 0; w < prm.warps; w++) {
-			hipLaunchKernelGGL(tv_warp_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I0, I1, I1x, I1y, u1, u2,
-			                   nx, ny, n, I1wx, I1wy, grad, rho_c);
 			hipLaunchKernelGGL(tv_round_init_kernel, dim3(nblk_tv(P)), dim3(TV_THREADS), 0, st, P, TV_K, d_ctl,
-			                   d_cur, d_iters, d_mode, d_state, d_nact);
+			                   d_cur, d_iters, d_mode, d_state, d_nact, w == 0 ? 1 : 0);
+			hipLaunchKernelGGL(tv_warp_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I0, I1, I1x, I1y, SA.u1, SA.u2,
+			                   SB.u1, SB.u2, d_cur, nx, ny, n, I1wx, I1wy, grad, rho_c);
 			HIPCHK(hipGetLastError());
-			for (int r = 0; r < max_rounds; r++) {
-				hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0, st,
-				                   I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut,
-				                   d_ctl, d_cur, d_part);
-				hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n,
-				                   eps2, d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
-				if ((r + 1) % kRoundsPerCheck == 0 || r == max_rounds - 1) {
-					HIPCHK(hipGetLastError());
-					int nact = 0;
-					HIPCHK(hipMemcpyAsync(&nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
-					HIPCHK(hipStreamSynchronize(st));
-					if (nact == 0) break;
+			int batch = guess > 0 ? (guess + TV_K - 1) / TV_K : 3;
+			for (int r = 0; r < max_rounds;) {
+				const int rb = std::min(batch, max_rounds - r);
+				for (int i = 0; i < rb; i++) {
+					hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0,
+					                   st, I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut,
+					                   d_ctl, d_cur, d_part);
+					hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n,
+					                   eps2, d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
 				}
+				r += rb;
+				HIPCHK(hipGetLastError());
+				int nact = 0;
+				HIPCHK(hipMemcpyAsync(&nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
+				HIPCHK(hipStreamSynchronize(st));
+				if (nact == 0) break;
+				batch = 2;
 			}
-			hipLaunchKernelGGL(tv_settle_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, SA, SB, n, n, d_cur);
-			HIPCHK(hipGetLastError());
 			HIPCHK(hipMemcpyAsync(h_state.data(), d_state, P * sizeof(int), hipMemcpyDeviceToHost, st));
 			HIPCHK(hipStreamSynchronize(st));
-			for (int pr = 0; pr < P; pr++) ctx->tv_pixel_iters += (uint64_t)std::max(h_state[pr], 0) * (uint64_t)n;
+			guess = 0;
+			for (int pr = 0; pr < P; pr++) {
+				ctx->tv_pixel_iters += (uint64_t)std::max(h_state[pr], 0) * (uint64_t)n;
+				guess = std::max(guess, h_state[pr]);
+			}
 			if (iters)
 				for (int pr = 0; pr < P; pr++) iters[((size_t)pr * nscales + s) * prm.warps + w] = h_state[pr];
 			if (prm.verbose)
 				for (int pr = 0; pr < P; pr++)
 					fprintf(stderr, "Pair %d scale %d warping %d: iterations %d\n", pr, s, w, h_state[pr]);
 		}
+		// the flow of every pair into the pyramid level
+		if (prm.warps > 0)
+			hipLaunchKernelGGL(tv_settle_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, SA, SB, n, n, d_cur);
+		HIPCHK(hipGetLastError());
 		if (!s) break;
 		hipLaunchKernelGGL(tv_zoom_in_kernel, dim3(tv_grid(pl.nx[s - 1] * pl.ny[s - 1]), P), dim3(TV_THREADS), 0, st,
 		                   u1, nx, ny, n, U1 + pl.off[s - 1], pl.nx[s - 1], pl.ny[s - 1], pl.nx[s - 1] * pl.ny[s - 1], scale);
