@@ -3173,8 +3173,8 @@ __global__ void bin_bounds_kernel(const uint32_t *__restrict__ keys, const uint3
 	for (uint32_t b = lo; b <= hi; b++) boff[b] = (uint32_t)i;
 }
 
-// CC channels; NBR rows per chunk; patches of 4 PQ samples, frame slices of
-// 4 SQ samples (PQ <= 16).
+// CC channels; NBR (<= 32) rows per chunk; patches of 4 PQ samples, frame
+// slices of 4 SQ samples (PQ <= 16).
 template <int CC, int NBR, int PQ, int SQ>
 __global__ __launch_bounds__(64)
 void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__restrict__ blist,
@@ -3182,15 +3182,19 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
                          const float *__restrict__ win, StepConst sc, int nbx, int nby, int nbins,
                          float *__restrict__ num, float *__restrict__ wsum)
 {
+	static_assert(NBR <= 32, "row masks are 32 bits");
 	constexpr int TB = AGG_TB;
 	constexpr int P = 4 * PQ, SL = 4 * SQ, QR = CC * PQ;   // QR: float4 of one row
 	constexpr int SPL = (NBR * QR + 63) / 64;              // float4 staged per lane per chunk
 	__shared__ __align__(16) float4 stage4[NBR * QR];
 	const float *stage = reinterpret_cast<const float *>(stage4);
 	__shared__ float swin[P];
-	// per row of a chunk: slice offset of the block pixel (0, 0)
-	// (dy0 sPx + dx0), patch frame qt, weight bits, row index e
+	// per row of a chunk (two chunks in flight): x = stage index and y =
+	// window index of the block pixel (0, 0) at the bin's first frame (add
+	// f SL + the lane's slice index), z = weight bits; rowm = (row index e,
+	// bit dt set for the slices at the bin's frames)
 	__shared__ int4 rrec[2][NBR];
+	__shared__ uint2 rowm[2][NBR];
 	const int lane = threadIdx.x;
 	// bins are dealt to the 8 XCDs in contiguous eighths (x fastest)
 	const uint32_t per_xcd = ((uint32_t)nbins + 7) / 8;
@@ -3248,14 +3252,14 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	uint4 recN = make_uint4(0, 0, 0, 0);
 	auto meta_load = [&](int c) {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
-		if (c < nch && kc + lane < k1 && lane < NBR) {
+		if (c < nch && lane < NBR && kc + lane < k1) {
 			eN = blist[kc + lane];
 			recN = erec[eN];
 		}
 	};
-	// the chunk's records into rrec[buf]; returns the lane's coverage mask
-	// (lanes = rows) and the rows present at each frame of the bin
-	auto meta_commit = [&](int c, uint64_t (&vm)[TB]) -> uint64_t {
+	// the chunk's records into rrec / rowm [c & 1]; returns the lane's
+	// coverage mask (lanes = rows) and the rows present at each frame
+	auto meta_commit = [&](int c, uint32_t (&vm)[TB]) -> uint64_t {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
 		const bool rv = lane < NBR && kc + lane < k1;
 		uint64_t msp = 0;
@@ -3276,75 +3280,86 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 #pragma unroll
 			for (int i = 0; i < 4; i++) if (own_t[i] == (int)recN.y) om = own_m[i];
 			msp &= om;
-			rrec[c & 1][lane] = make_int4(dy0 * sPx + dx0, qt, (int)recN.z, (int)eN);
+			// slices dt of the bin's frames t0 .. t0 + TB - 1
+			const int dlo = max(0, t0 - qt), dhi = min(sPt - 1, t0 + TB - 1 - qt);
+			const uint32_t dm = (msp != 0 && dhi >= dlo) ? ((2u << dhi) - (1u << dlo)) : 0u;
+			const int wb = (t0 - qt) * SL - (dy0 * sPx + dx0);
+			rrec[c & 1][lane] = make_int4(lane * CC * P + wb, wb, (int)recN.z, 0);
+			rowm[c & 1][lane] = make_uint2(eN, dm);
+		} else if (lane < NBR) {
+			rowm[c & 1][lane] = make_uint2(0u, 0u);   // past the list's end: nothing staged
 		}
 #pragma unroll
 		for (int f = 0; f < TB; f++) {
 			const int dt = t0 + f - qt;
-			vm[f] = __ballot(rv && msp != 0 && dt >= 0 && dt < sPt);
+			vm[f] = (uint32_t)__ballot(rv && msp != 0 && dt >= 0 && dt < sPt);
 		}
 		return msp;
 	};
 	// staged slices of a chunk: float4 i = lane + 64 m of the chunk's rows
-	// (row i / QR), for the rows with a frame in the bin, the slices at the
-	// bin's frames; the rest of a row's slot is not read
+	// (row i / QR); only the slices at the bin's frames are loaded and
+	// stored, the rest of a row's slot is never read
 	float4 sv[SPL];
-#define AGG_BIN_STAGE_ISSUE(c, vm)                                                                  \
+#pragma unroll
+	for (int m = 0; m < SPL; m++) sv[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+	uint32_t sneed = 0;
+#define AGG_BIN_STAGE_ISSUE(c)                                                                      \
 	{                                                                                               \
-		const uint64_t any_ = vm[0] | vm[1] | vm[2] | vm[3];                                        \
+		sneed = 0;                                                                                  \
+		const int nbc_ = (int)min<uint32_t>(NBR, k1 - (k0 + (uint32_t)(c) * NBR));                 \
 		_Pragma("unroll") for (int m = 0; m < SPL; m++) {                                         \
 			const int i = lane + 64 * m;                                                            \
 			const int j = i / QR, rq = i - j * QR;                                                  \
-			sv[m] = make_float4(0.f, 0.f, 0.f, 0.f);                                                \
-			if (j < NBR && ((any_ >> j) & 1ull)) {                                                  \
-				const int4 r = rrec[(c) & 1][j];                                                    \
-				const int dt = (rq % PQ) / SQ;                                                      \
-				if (dt >= t0 - r.y && dt <= t0 + TB - 1 - r.y)                                      \
-					sv[m] = est4[(size_t)(uint32_t)r.w * QR + rq];                                  \
+			if (j < nbc_) {                                                                         \
+				const uint2 rm = rowm[(c) & 1][j];                                                  \
+				if ((rm.y >> ((rq % PQ) / SQ)) & 1u) {                                              \
+					sv[m] = est4[(size_t)rm.x * QR + rq];                                           \
+					sneed |= 1u << m;                                                               \
+				}                                                                                   \
 			}                                                                                       \
 		}                                                                                           \
 	}
 #define AGG_BIN_STAGE_WRITE()                                                                       \
 	{                                                                                               \
-		_Pragma("unroll") for (int m = 0; m < SPL; m++) {                                         \
-			const int i = lane + 64 * m;                                                            \
-			if (i < NBR * QR) stage4[i] = sv[m];                                                    \
-		}                                                                                           \
+		_Pragma("unroll") for (int m = 0; m < SPL; m++)                                           \
+			if ((sneed >> m) & 1u) stage4[lane + 64 * m] = sv[m];                                  \
 	}
-	uint64_t vmC[TB], vmN[TB];
+	uint32_t vmC[TB], vmN[TB];
 	meta_load(0);
 	uint64_t mC = meta_commit(0, vmC);
 	meta_load(1);
 	__syncthreads();
-	AGG_BIN_STAGE_ISSUE(0, vmC);
+	AGG_BIN_STAGE_ISSUE(0);
 	AGG_BIN_STAGE_WRITE();
+	// the lane's slice index at each frame of the bin
+	int plf[TB];
+#pragma unroll
+	for (int f = 0; f < TB; f++) plf[f] = f * SL + pix_l;
 	for (int c = 0; c < nch; c++) {
 		uint64_t mN = 0;
 		if (c + 1 < nch) mN = meta_commit(c + 1, vmN);
 		meta_load(c + 2);
 		__syncthreads();
-		if (c + 1 < nch) AGG_BIN_STAGE_ISSUE(c + 1, vmN);
+		if (c + 1 < nch) AGG_BIN_STAGE_ISSUE(c + 1);
 		// each lane folds the rows that cover its pixel, frame by frame, in
 		// row order, two rows per round (their loads overlap)
-		const uint64_t mine_sp = transpose_bits64(mC);
+		const uint32_t mine_sp = (uint32_t)transpose_bits64(mC);
 #pragma unroll
 		for (int f = 0; f < TB; f++) {
-			uint64_t mine = mine_sp & vmC[f];
+			uint32_t mine = mine_sp & vmC[f];
 			while (mine) {
-				const int j1 = __ffsll((unsigned long long)mine) - 1;
+				const int j1 = __ffs(mine) - 1;
 				mine &= mine - 1;
 				const bool two = mine != 0;
-				const int j2 = two ? __ffsll((unsigned long long)mine) - 1 : j1;
-				if (two) mine &= mine - 1;
+				const int j2 = two ? __ffs(mine) - 1 : j1;
+				mine &= mine - 1;
 				const int4 r1 = rrec[c & 1][j1], r2 = rrec[c & 1][j2];
-				const int s1 = (t0 + f - r1.y) * SL + pix_l - r1.x;
-				const int s2 = (t0 + f - r2.y) * SL + pix_l - r2.x;
-				const float v1 = swin[s1], v2 = swin[s2];
+				const float v1 = swin[r1.y + plf[f]], v2 = swin[r2.y + plf[f]];
 				float x1[CC], x2[CC];
 #pragma unroll
 				for (int cc = 0; cc < CC; cc++) {
-					x1[cc] = stage[(j1 * CC + cc) * P + s1];
-					x2[cc] = stage[(j2 * CC + cc) * P + s2];
+					x1[cc] = stage[r1.x + cc * P + plf[f]];
+					x2[cc] = stage[r2.x + cc * P + plf[f]];
 				}
 				aw[f] = aw[f] + __int_as_float(r1.z) * v1;
 #pragma unroll
@@ -3371,7 +3386,6 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 				num[(size_t)(t0 + f) * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x] = an[f][c];
 		}
 }
-
 #undef AGG_BIN_STAGE_ISSUE
 #undef AGG_BIN_STAGE_WRITE
 
