@@ -3186,6 +3186,7 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	constexpr int TB = AGG_TB;
 	constexpr int P = 4 * PQ, SL = 4 * SQ, QR = CC * PQ;   // QR: float4 of one row
 	constexpr int SPL = (NBR * QR + 63) / 64;              // float4 staged per lane per chunk
+	constexpr int RPR = CC == 1 ? 4 : 2;                   // rows folded per round
 	__shared__ __align__(16) float4 stage4[NBR * QR];
 	const float *stage = reinterpret_cast<const float *>(stage4);
 	__shared__ float swin[P];
@@ -3342,33 +3343,37 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 		__syncthreads();
 		if (c + 1 < nch) AGG_BIN_STAGE_ISSUE(c + 1);
 		// each lane folds the rows that cover its pixel, frame by frame, in
-		// row order, two rows per round (their loads overlap)
+		// row order, up to RPR rows per round (their LDS reads overlap)
 		const uint32_t mine_sp = (uint32_t)transpose_bits64(mC);
 #pragma unroll
 		for (int f = 0; f < TB; f++) {
 			uint32_t mine = mine_sp & vmC[f];
 			while (mine) {
-				const int j1 = __ffs(mine) - 1;
-				mine &= mine - 1;
-				const bool two = mine != 0;
-				const int j2 = two ? __ffs(mine) - 1 : j1;
-				mine &= mine - 1;
-				const int4 r1 = rrec[c & 1][j1], r2 = rrec[c & 1][j2];
-				const float v1 = swin[r1.y + plf[f]], v2 = swin[r2.y + plf[f]];
-				float x1[CC], x2[CC];
+				int jj[RPR];
+				bool has[RPR];
 #pragma unroll
-				for (int cc = 0; cc < CC; cc++) {
-					x1[cc] = stage[r1.x + cc * P + plf[f]];
-					x2[cc] = stage[r2.x + cc * P + plf[f]];
+				for (int u = 0; u < RPR; u++) {
+					has[u] = mine != 0;
+					jj[u] = has[u] ? __ffs(mine) - 1 : jj[0];
+					mine &= mine - 1;
 				}
-				aw[f] = aw[f] + __int_as_float(r1.z) * v1;
+				int4 rr[RPR];
+				float vv[RPR], xx[RPR][CC];
 #pragma unroll
-				for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + x1[cc];
-				if (two) {
-					aw[f] = aw[f] + __int_as_float(r2.z) * v2;
+				for (int u = 0; u < RPR; u++) rr[u] = rrec[c & 1][jj[u]];
 #pragma unroll
-					for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + x2[cc];
+				for (int u = 0; u < RPR; u++) {
+					vv[u] = swin[rr[u].y + plf[f]];
+#pragma unroll
+					for (int cc = 0; cc < CC; cc++) xx[u][cc] = stage[rr[u].x + cc * P + plf[f]];
 				}
+#pragma unroll
+				for (int u = 0; u < RPR; u++)
+					if (has[u]) {
+						aw[f] = aw[f] + __int_as_float(rr[u].z) * vv[u];
+#pragma unroll
+						for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + xx[u][cc];
+					}
 			}
 		}
 		__syncthreads();
