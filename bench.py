@@ -72,7 +72,13 @@ def _stage_work(stats, anchors, pixels, P=64, C=1):
     coef = ("valu", 2.0 * anchors * P * P * C, "forward transform of every patch anchor, 2 A P^2")
     f1 = 2.0 * n1 * P * P * C + 8.0 * P * C * (n1 - g1) + 2.0 * P * C * n1
     f2 = 2.0 * n2 * P * P * C + 12.0 * P * C * (n2 - g2) + 5.0 * P * C * n2
-    return {"s1_coef": coef, "s2_coef": coef,
+    # block matching: the sequential distance of every candidate, a
+    # subtraction, a multiply and an add per patch sample (the windows of
+    # 11 x 11 x 9 candidates; border windows are smaller, so an upper bound)
+    cand = 11 * 11 * 9
+    bm1 = ("valu", 3.0 * P * cand * g1, "distances 3 P Cand per group, Cand = 11 x 11 x 9")
+    bm2 = ("valu", 3.0 * P * cand * g2, "distances 3 P Cand per group, Cand = 11 x 11 x 9")
+    return {"s1_coef": coef, "s2_coef": coef, "s1_bm": bm1, "s2_bm": bm2,
             "s1_filter": ("valu", f1, "executed: inverse transform 2 N P^2 + Haar 8 P (N - 1) + threshold 2 P N"),
             "s2_filter": ("valu", f2, "executed: inverse transform 2 N P^2 + Haar 12 P (N - 1) + Wiener 5 P N"),
             "s1_aggregate": ("hbm", n1 * (4.0 * P * C + 4) + 4.0 * pixels * (C + 1),
@@ -422,7 +428,7 @@ def main():
     kern = {"s1_bm": "bm_reg_kernel<4, 4, false>", "s2_bm": "bm_reg_kernel<4, 4, false>",
             "s1_coef": "coef_kernel<4>", "s2_coef": "coef_kernel<4>",
             "s1_filter": "filter1_fast_kernel<4>", "s2_filter": "filter2_fast_kernel<4>",
-            "s1_aggregate": "agg_chunk_kernel<1, 1024, 64>", "s2_aggregate": "agg_chunk_kernel<1, 1024, 64>"}
+            "s1_aggregate": "agg_bin_fold_kernel<1, 32, 16, 4>", "s2_aggregate": "agg_bin_fold_kernel<1, 32, 16, 4>"}
     work = _stage_work(stats, (W - 3) * (H - 3) * (F - 3), W * H * F)
     ranked = sorted((k for k in stage_ms if k in kern), key=lambda k: -stage_ms[k])
     roofline = _roofline(ranked[0], stage_ms, launches, args.steps, work, kern)
@@ -447,11 +453,17 @@ def main():
         "step_roofline": step_roofline,
         "stages_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_ms.items()},
         "tvl1_sequence_s": round(tvl1_s, 3),
-        # TV-L1 iterations: 64 B per pixel and iteration (u1, u2, p11..p22 read
-        # and written, rho_c, I1wx, I1wy, grad read) over the executed iterations
-        "tvl1_roofline": {"bound": "hbm", "pixel_iterations": int(tvl1_pix_it),
-                          "achieved": round(64.0 * tvl1_pix_it / tvl1_s / 1e9, 1), "peak": PEAK_HBM_GBS,
-                          "unit": "GB/s", "frac": round(64.0 * tvl1_pix_it / tvl1_s / 1e9 / PEAK_HBM_GBS, 4)},
+        # TV-L1 iterations (tvl1flow_lib.c:165-246) over the executed
+        # pixel-iterations: about 60 FLOP each (TH step, divergences, u, the
+        # squared update, forward gradients, two hypot, four divisions; a
+        # division or square root counted as one), the whole flow computation
+        # timed; and the bytes an iteration that keeps nothing on chip moves,
+        # 64 B (u1, u2, p11..p22 read and written, rho_c, I1wx, I1wy, grad
+        # read), which the LDS tiles of 4 iterations cut to about a third
+        "tvl1_roofline": {"bound": "valu", "pixel_iterations": int(tvl1_pix_it), "flop_per_pixel_iteration": 60,
+                          "achieved": round(60.0 * tvl1_pix_it / tvl1_s / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                          "unit": "TFLOP/s", "frac": round(60.0 * tvl1_pix_it / tvl1_s / 1e12 / PEAK_FP32_TFLOPS, 4),
+                          "untiled_equivalent_gbs": round(64.0 * tvl1_pix_it / tvl1_s / 1e9, 1)},
         "e2e_mpix_s": round(W * H * F / (elapsed / args.steps + tvl1_s) / 1e6, 4),
         "psnr_db": {"noisy": round(_psnr(noisy, clean), 3), "basic": round(_psnr(basic, clean), 3),
                     "final": round(_psnr(final, clean), 3)},
