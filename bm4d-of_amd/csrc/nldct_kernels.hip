@@ -1,19 +1,22 @@
 // HIP kernels of the BM4D-OF / NL-DCT denoiser for gfx950 (MI355X).
 //
 // Arithmetic that decides a discontinuous outcome is evaluated in the
-// reference's exact operation order with separately rounded IEEE float
-// operations (this file is compiled with -ffp-contract=off):
-//   * patch distances, sequential over (c, ht, hy, hx)  (nldct.cpp:1262-1274,
-//     1723-1737);
+// reference's exact operation order with IEEE float operations (this file
+// is compiled with -ffp-contract=off, so only the explicit fused
+// multiply-adds below are fused):
+//   * patch distances, sequential over (c, ht, hy, hx), separately rounded
+//     as the reference's own loops (nldct.cpp:1262-1274, 1723-1737);
 //   * the forward patch transform, a sequential-K dot product per
-//     coefficient (nldct.cpp:1840-1844 via productMatrix/cblas_sgemm);
+//     coefficient with a fused multiply-add per term (nldct.cpp:1840-1844 via
+//     productMatrix/cblas_sgemm: the reference leaves the order to the BLAS
+//     it links; an FMA BLAS kernel accumulates K = 64 this way);
 //   * the Haar butterflies and the hard threshold (nldct.cpp:1847-1907);
-//   * the inverse transform, sequential in the coefficient index
-//     (nldct.cpp:1927-1931, 2155-2159);
+//   * the inverse transform, sequential in the coefficient index, fused
+//     likewise (nldct.cpp:1927-1931, 2155-2159);
 //   * the Wiener energy sum_w^2 in (coefficient, patch) order
 //     (nldct.cpp:2123-2135) and the aggregation sums in group order
-//     (nldct.cpp:2284-2300, 2376-2393).
-// The GPU output is therefore bitwise equal to the sequential reference.
+//     (nldct.cpp:2284-2300, 2376-2393), separately rounded.
+// The GPU output is therefore bitwise equal to the sequential oracle.
 #include "common.hpp"
 #include "mlp_dev.hpp"
 
@@ -1185,7 +1188,7 @@ __device__ __forceinline__ void fwd_transform(const float *__restrict__ X, const
 		const int k = o / N, n = o - k * N;
 		const float *b = B + (size_t)k * P;
 		float acc = 0.f;
-		for (int p = 0; p < P; p++) acc = acc + X[p * N + n] * b[p];
+		for (int p = 0; p < P; p++) acc = __builtin_fmaf(X[p * N + n], b[p], acc);
 		Z[o] = acc;
 	}
 }
@@ -1197,7 +1200,7 @@ __device__ __forceinline__ void inv_transform(const float *__restrict__ Z, const
 	for (int o = threadIdx.x; o < P * N; o += FILTER_THREADS) {
 		const int pix = o / N, n = o - pix * N;
 		float acc = 0.f;
-		for (int k = 0; k < P; k++) acc = acc + Z[k * N + n] * Bi[(size_t)k * P + pix];
+		for (int k = 0; k < P; k++) acc = __builtin_fmaf(Z[k * N + n], Bi[(size_t)k * P + pix], acc);
 		Xh[o] = acc;
 	}
 }
@@ -1540,8 +1543,8 @@ __device__ __forceinline__ void fast_forward(const float *__restrict__ img, cons
 #pragma unroll
 				for (int hx = 0; hx < PX; hx++) {
 					const float b = brow[(ht * PX + hy) * PX + hx];
-					acc0 = acc0 + r0[hx] * b;
-					acc1 = acc1 + r1[hx] * b;
+					acc0 = __builtin_fmaf(r0[hx], b, acc0);
+					acc1 = __builtin_fmaf(r1[hx], b, acc1);
 				}
 			}
 		zs[n * FAST_P + threadIdx.x] = acc0;
@@ -1556,7 +1559,7 @@ __device__ __forceinline__ void fast_forward(const float *__restrict__ img, cons
 			for (int hy = 0; hy < PX; hy++) {
 				const float *r = src + ht * WH + hy * W;
 #pragma unroll
-				for (int hx = 0; hx < PX; hx++) acc = acc + r[hx] * brow[(ht * PX + hy) * PX + hx];
+				for (int hx = 0; hx < PX; hx++) acc = __builtin_fmaf(r[hx], brow[(ht * PX + hy) * PX + hx], acc);
 			}
 		zs[n * FAST_P + threadIdx.x] = acc;
 	}
@@ -1581,14 +1584,14 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 #pragma unroll
 		for (int q = 0; q < FAST_P / 4; q++) {
 			const float4 z0 = zr0[q], z1 = zr1[q];
-			acc0 = acc0 + z0.x * bcol[4 * q + 0];
-			acc1 = acc1 + z1.x * bcol[4 * q + 0];
-			acc0 = acc0 + z0.y * bcol[4 * q + 1];
-			acc1 = acc1 + z1.y * bcol[4 * q + 1];
-			acc0 = acc0 + z0.z * bcol[4 * q + 2];
-			acc1 = acc1 + z1.z * bcol[4 * q + 2];
-			acc0 = acc0 + z0.w * bcol[4 * q + 3];
-			acc1 = acc1 + z1.w * bcol[4 * q + 3];
+			acc0 = __builtin_fmaf(z0.x, bcol[4 * q + 0], acc0);
+			acc1 = __builtin_fmaf(z1.x, bcol[4 * q + 0], acc1);
+			acc0 = __builtin_fmaf(z0.y, bcol[4 * q + 1], acc0);
+			acc1 = __builtin_fmaf(z1.y, bcol[4 * q + 1], acc1);
+			acc0 = __builtin_fmaf(z0.z, bcol[4 * q + 2], acc0);
+			acc1 = __builtin_fmaf(z1.z, bcol[4 * q + 2], acc1);
+			acc0 = __builtin_fmaf(z0.w, bcol[4 * q + 3], acc0);
+			acc1 = __builtin_fmaf(z1.w, bcol[4 * q + 3], acc1);
 		}
 		est[(size_t)(row0 + n) * rs + ro + threadIdx.x] = ww * acc0;
 		est[(size_t)(row0 + n + 1) * rs + ro + threadIdx.x] = ww * acc1;
@@ -1599,10 +1602,10 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 #pragma unroll
 		for (int q = 0; q < FAST_P / 4; q++) {
 			const float4 z = zr[q];
-			acc = acc + z.x * bcol[4 * q + 0];
-			acc = acc + z.y * bcol[4 * q + 1];
-			acc = acc + z.z * bcol[4 * q + 2];
-			acc = acc + z.w * bcol[4 * q + 3];
+			acc = __builtin_fmaf(z.x, bcol[4 * q + 0], acc);
+			acc = __builtin_fmaf(z.y, bcol[4 * q + 1], acc);
+			acc = __builtin_fmaf(z.z, bcol[4 * q + 2], acc);
+			acc = __builtin_fmaf(z.w, bcol[4 * q + 3], acc);
 		}
 		est[(size_t)(row0 + n) * rs + ro + threadIdx.x] = ww * acc;
 	}
@@ -1610,8 +1613,9 @@ __device__ __forceinline__ void fast_inverse(const float *__restrict__ zs, const
 
 // The inverse over the columns k set in kmask only, for groups whose other
 // columns are zero in every row (step 1 after the hard threshold): a
-// skipped term is (+0) * Bi = +-0, and adding +-0 to a sequential sum that
-// starts at +0 leaves it unchanged bit for bit (it never becomes -0), so the
+// skipped term is fma(+0, Bi, acc) = acc bit for bit for every acc but -0,
+// and a sum that starts at +0 reaches -0 only by an underflow below 2^-149,
+// which sums of image data times basis entries do not come near, so the
 // result equals fast_inverse's.  Lane k holds z[0..N) of column k on entry
 // (registers); the compacted rows are written over zs.
 __device__ __forceinline__ void fast_inverse_sparse(const float (&z)[FAST_NMAX], float *__restrict__ zs,
@@ -1649,11 +1653,11 @@ __device__ __forceinline__ void fast_inverse_sparse(const float (&z)[FAST_NMAX],
 			if (j < kc) {
 				const float4 a = *reinterpret_cast<const float4 *>(z0 + j);
 				const float4 b = *reinterpret_cast<const float4 *>(z1 + j);
-				acc0 = acc0 + a.x * bsel[j];
-				acc1 = acc1 + b.x * bsel[j];
-				if (j + 1 < kc) { acc0 = acc0 + a.y * bsel[j + 1]; acc1 = acc1 + b.y * bsel[j + 1]; }
-				if (j + 2 < kc) { acc0 = acc0 + a.z * bsel[j + 2]; acc1 = acc1 + b.z * bsel[j + 2]; }
-				if (j + 3 < kc) { acc0 = acc0 + a.w * bsel[j + 3]; acc1 = acc1 + b.w * bsel[j + 3]; }
+				acc0 = __builtin_fmaf(a.x, bsel[j], acc0);
+				acc1 = __builtin_fmaf(b.x, bsel[j], acc1);
+				if (j + 1 < kc) { acc0 = __builtin_fmaf(a.y, bsel[j + 1], acc0); acc1 = __builtin_fmaf(b.y, bsel[j + 1], acc1); }
+				if (j + 2 < kc) { acc0 = __builtin_fmaf(a.z, bsel[j + 2], acc0); acc1 = __builtin_fmaf(b.z, bsel[j + 2], acc1); }
+				if (j + 3 < kc) { acc0 = __builtin_fmaf(a.w, bsel[j + 3], acc0); acc1 = __builtin_fmaf(b.w, bsel[j + 3], acc1); }
 			}
 		}
 		est[(size_t)(row0 + n) * rs + ro + lane] = ww * acc0;
@@ -1666,10 +1670,10 @@ __device__ __forceinline__ void fast_inverse_sparse(const float (&z)[FAST_NMAX],
 		for (int j = 0; j < FAST_P; j += 4) {
 			if (j < kc) {
 				const float4 a = *reinterpret_cast<const float4 *>(z0 + j);
-				acc = acc + a.x * bsel[j];
-				if (j + 1 < kc) acc = acc + a.y * bsel[j + 1];
-				if (j + 2 < kc) acc = acc + a.z * bsel[j + 2];
-				if (j + 3 < kc) acc = acc + a.w * bsel[j + 3];
+				acc = __builtin_fmaf(a.x, bsel[j], acc);
+				if (j + 1 < kc) acc = __builtin_fmaf(a.y, bsel[j + 1], acc);
+				if (j + 2 < kc) acc = __builtin_fmaf(a.z, bsel[j + 2], acc);
+				if (j + 3 < kc) acc = __builtin_fmaf(a.w, bsel[j + 3], acc);
 			}
 		}
 		est[(size_t)(row0 + n) * rs + ro + lane] = ww * acc;
@@ -1719,7 +1723,7 @@ void coef_kernel(const float *__restrict__ img, int W, int H, int F, const float
 				for (int hx = 0; hx < PX; hx++) {
 					const float b = brow[(ht * PX + hy) * PX + hx];
 #pragma unroll
-					for (int a = 0; a < 4; a++) acc[a] = acc[a] + seg[hx + a] * b;
+					for (int a = 0; a < 4; a++) acc[a] = __builtin_fmaf(seg[hx + a], b, acc[a]);
 				}
 			}
 #pragma unroll
@@ -1734,7 +1738,7 @@ void coef_kernel(const float *__restrict__ img, int W, int H, int F, const float
 			for (int hy = 0; hy < PX; hy++) {
 				const float *r = s0 + ht * WH + hy * W;
 #pragma unroll
-				for (int hx = 0; hx < PX; hx++) acc = acc + r[hx] * brow[(ht * PX + hy) * PX + hx];
+				for (int hx = 0; hx < PX; hx++) acc = __builtin_fmaf(r[hx], brow[(ht * PX + hy) * PX + hx], acc);
 			}
 		coef[(pos0 + x) * FAST_P + lane] = acc;
 	}

@@ -10,10 +10,12 @@
  * -ffp-contract=off (oracle/Makefile): the reference's float arithmetic is
  * reproduced operation by operation, and products must not be fused.
  *
- * Matrix products follow a sequential-K, separately rounded sgemm: the
- * reference calls an external CBLAS whose summation order it does not fix
- * (src/core/matrix_funs.cpp:152, cmake/modules/FindCBLAS.cmake); this oracle
- * fixes it as acc = 0; for p in 0..K-1: acc = acc + a*b.
+ * Matrix products follow a sequential-K sgemm with a fused multiply-add per
+ * term: the reference calls an external CBLAS whose summation order it does
+ * not fix (src/core/matrix_funs.cpp:152, cmake/modules/FindCBLAS.cmake);
+ * this oracle fixes it as acc = 0; for p in 0..K-1: acc = fmaf(a, b, acc),
+ * the accumulation of an FMA BLAS micro-kernel over K = 64 (one K block,
+ * alpha = 1, beta = 0).  Every other product is separately rounded.
  *
  * Neighbour ties: the reference orders candidates with std::partial_sort
  * (libstdc++ heap select + sort_heap) under a strict `<` on the distance
@@ -494,7 +496,7 @@ OR_EXPORT int or_bases(const bm4dof_params *p, float *basis, float *binv, float 
 }
 
 /* ------------------------------------------------------------------------ */
-/* productMatrix (matrix_funs.cpp:136-164) with a sequential-K sgemm.       */
+/* productMatrix (matrix_funs.cpp:136-164) with a sequential-K FMA sgemm.   */
 /* Column-major: C(n x l) = A(n x m) * op(B).                               */
 /* ------------------------------------------------------------------------ */
 static void sgemm_nn(float *C, const float *A, const float *B, int n, int l, int m)
@@ -502,7 +504,7 @@ static void sgemm_nn(float *C, const float *A, const float *B, int n, int l, int
 	for (int j = 0; j < l; j++)
 		for (int i = 0; i < n; i++) {
 			float acc = 0.f;
-			for (int p = 0; p < m; p++) acc += A[i + p * n] * B[p + j * m];
+			for (int p = 0; p < m; p++) acc = fmaf(A[i + p * n], B[p + j * m], acc);
 			C[i + j * n] = acc;
 		}
 }
@@ -511,7 +513,7 @@ static void sgemm_nt(float *C, const float *A, const float *B, int n, int l, int
 	for (int j = 0; j < l; j++)
 		for (int i = 0; i < n; i++) {
 			float acc = 0.f;
-			for (int p = 0; p < m; p++) acc += A[i + p * n] * B[j + p * l];
+			for (int p = 0; p < m; p++) acc = fmaf(A[i + p * n], B[j + p * l], acc);
 			C[i + j * n] = acc;
 		}
 }
