@@ -718,8 +718,8 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 		std::swap(vin, vout);
 	}
 	const int64_t nub = 8 * Mub;
-	hipLaunchKernelGGL(bin_bounds_kernel, dim3(nblk(nub + 1, 256)), dim3(256), 0, st, q.k0, np, nub, bg.nbins,
-	                   q.boff);
+	hipLaunchKernelGGL(bin_bounds_kernel, dim3((unsigned)std::min<int64_t>(nblk(nub + 1, 256), 4096)), dim3(256), 0, st,
+	                   q.k0, np, nub, bg.nbins, q.boff);
 	HIPCHK(hipGetLastError());
 }
 

@@ -3165,16 +3165,18 @@ __global__ void bin_pair_emit_kernel(const uint32_t *__restrict__ epos, const ui
 			}
 }
 
-// boff[b] = number of sorted keys below b, b = 0 .. nbins
+// boff[b] = number of sorted keys below b, b = 0 .. nbins (grid-stride over
+// the pair index i <= n: i writes the starts of the bins from the key before
+// it, exclusive, to its own key, inclusive)
 __global__ void bin_bounds_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ np, int64_t nub,
                                   int nbins, uint32_t *__restrict__ boff)
 {
-	const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	const uint32_t n = *np;
-	if (i > nub || i > (int64_t)n) return;
-	const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1u;
-	const uint32_t hi = i == (int64_t)n ? (uint32_t)nbins : keys[i];
-	for (uint32_t b = lo; b <= hi; b++) boff[b] = (uint32_t)i;
+	const int64_t n = min<int64_t>((int64_t)*np, nub);
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
+		const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1u;
+		const uint32_t hi = i == n ? (uint32_t)nbins : keys[i];
+		for (uint32_t b = lo; b <= hi; b++) boff[b] = (uint32_t)i;
+	}
 }
 
 // CC channels; NBR (<= 32) rows per chunk; patches of 4 PQ samples, frame

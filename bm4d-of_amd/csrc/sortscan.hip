@@ -257,26 +257,32 @@ void rs_scatter_kernel(const uint32_t *__restrict__ kin, const uint32_t *__restr
 {
 	constexpr int NW = RS_THREADS / 64;
 	__shared__ uint32_t base[256];
-	__shared__ uint32_t wcnt[NW][256];
+	__shared__ uint32_t wcnt[2][NW][256];   // per round (two in flight): pairs of each digit per wave
 	const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 	const uint64_t n = *np;
 	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
 	if (b0 >= n) return;
-	base[tid] = offs[(size_t)tid * nblk + blockIdx.x];
-	const uint64_t lt = (1ull << lane) - 1ull;
+	// the tile's pairs in registers (round r: pairs b0 + r RS_THREADS + tid)
+	uint32_t key[RS_IPT], val[RS_IPT];
+#pragma unroll
 	for (int r = 0; r < RS_IPT; r++) {
 		const uint64_t k = b0 + (uint64_t)r * RS_THREADS + tid;
-		if (b0 + (uint64_t)r * RS_THREADS >= n) break;   // uniform over the block
+		key[r] = k < n ? kin[k] : 0u;
+		val[r] = k < n ? vin[k] : 0u;
+	}
+	base[tid] = offs[(size_t)tid * nblk + blockIdx.x];
 #pragma unroll
-		for (int w = 0; w < NW; w++) wcnt[w][tid] = 0;
-		__syncthreads();
-		const bool valid = k < n;
-		uint32_t key = 0, val = 0, d = 0;
-		if (valid) {
-			key = kin[k];
-			val = vin[k];
-			d = (key >> shift) & 255u;
-		}
+	for (int w = 0; w < NW; w++) {
+		wcnt[0][w][tid] = 0;
+		wcnt[1][w][tid] = 0;
+	}
+	__syncthreads();
+	const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+	for (int r = 0; r < RS_IPT; r++) {
+		const int pb = r & 1;
+		const bool valid = b0 + (uint64_t)r * RS_THREADS + tid < n;
+		const uint32_t d = valid ? (key[r] >> shift) & 255u : 0u;
 		// the lanes of this wave with the same digit
 		uint64_t m = __ballot(valid);
 #pragma unroll
@@ -286,20 +292,24 @@ void rs_scatter_kernel(const uint32_t *__restrict__ kin, const uint32_t *__restr
 			m &= bs ? bb : ~bb;
 		}
 		const uint32_t rk = (uint32_t)__popcll(m & lt);
-		if (valid && rk == 0) wcnt[wv][d] = (uint32_t)__popcll(m);
+		if (valid && rk == 0) wcnt[pb][wv][d] = (uint32_t)__popcll(m);
 		__syncthreads();
 		if (valid) {
 			uint32_t pos = base[d] + rk;
-			for (int w = 0; w < wv; w++) pos += wcnt[w][d];
-			kout[pos] = key;
-			vout[pos] = val;
+			for (int w = 0; w < wv; w++) pos += wcnt[pb][w][d];
+			kout[pos] = key[r];
+			vout[pos] = val[r];
 		}
 		__syncthreads();
+		// the round's totals: advance the digit offsets, clear the counts for
+		// round r + 2
 		uint32_t t = 0;
 #pragma unroll
-		for (int w = 0; w < NW; w++) t += wcnt[w][tid];
+		for (int w = 0; w < NW; w++) {
+			t += wcnt[pb][w][tid];
+			wcnt[pb][w][tid] = 0;
+		}
 		base[tid] += t;
-		__syncthreads();
 	}
 }
 
