@@ -185,8 +185,8 @@ def cpu_baseline(noisy, clean, ff, bf, crop, threads, gpu_run, reps=3, tv_pairs=
                      f"{threads} OpenMP threads of {os.cpu_count()} ({_cpu_model()}), "
                      f"nParts={2 * threads}, median of {reps}: {dt:.1f} s "
                      f"(runs {', '.join(f'{t:.1f}' for t in times)} s); the C restatement of the "
-                     "reference with a sequential-K sgemm (the reference itself needs cblas.h and "
-                     "fftw3.h, absent here)"}
+                     "reference with a sequential-K fused-multiply-add sgemm (the reference itself needs "
+                     "cblas.h and fftw3.h, absent here)"}
     if tv_pairs > 0:
         pairs = [(i, i + 1) for i in range(tv_pairs)]
         t0 = time.perf_counter()

@@ -1133,7 +1133,6 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		}
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
-		mark("l_sort", "s1_sort", "s2_sort");
 
 		if (la)
 			for (int c = 0; c < C; c++)   // channel planes, the first one sets the weights
@@ -1175,7 +1174,10 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 			HIPCHK(hipGetLastError());
 		}
 		mark("l_filter", "s1_filter", "s2_filter");
+		// the stage "sort": what the row sort on the second stream still
+		// takes once the filter is done
 		HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+		mark("l_sort", "s1_sort", "s2_sort");
 
 		std::vector<uint32_t> band_groups;
 		for (uint32_t gf : pl.geo.frame_first)

@@ -207,7 +207,10 @@ int bm4dof_bm(bm4dof_ctx *ctx, const bm4dof_params *p, const float *image,
 
 /* Per-stage timings of the last bm4dof_run / bm4dof_run_dev call, in ms,
  * measured with HIP events on the context's stream.  `names` receives a
- * static string naming each slot.  Returns the number of slots filled. */
+ * static string naming each slot (per step: setup, bm, coef, filter, sort,
+ * aggregate, normalize; "sort" is the time the stream still waits, after the
+ * filter, for the row sort that runs on the second stream).  Returns the
+ * number of slots filled. */
 int bm4dof_last_timings(const bm4dof_ctx *ctx, float *ms, const char **names, int max_slots);
 
 /* Statistics of the last bm4dof_run / bm4dof_run_dev call: stats[0] =

@@ -501,21 +501,29 @@ OR_EXPORT int or_bases(const bm4dof_params *p, float *basis, float *binv, float 
 /* ------------------------------------------------------------------------ */
 static void sgemm_nn(float *C, const float *A, const float *B, int n, int l, int m)
 {
-	for (int j = 0; j < l; j++)
-		for (int i = 0; i < n; i++) {
-			float acc = 0.f;
-			for (int p = 0; p < m; p++) acc = fmaf(A[i + p * n], B[p + j * m], acc);
-			C[i + j * n] = acc;
+	/* j, p, i: every C(i, j) is accumulated in p order; the i loop runs over
+	 * independent sums (the CPU overlaps them) */
+	for (int j = 0; j < l; j++) {
+		float *c = C + (size_t)j * n;
+		for (int i = 0; i < n; i++) c[i] = 0.f;
+		for (int p = 0; p < m; p++) {
+			const float b = B[p + j * m];
+			const float *a = A + (size_t)p * n;
+			for (int i = 0; i < n; i++) c[i] = fmaf(a[i], b, c[i]);
 		}
+	}
 }
 static void sgemm_nt(float *C, const float *A, const float *B, int n, int l, int m)
 {
-	for (int j = 0; j < l; j++)
-		for (int i = 0; i < n; i++) {
-			float acc = 0.f;
-			for (int p = 0; p < m; p++) acc = fmaf(A[i + p * n], B[j + p * l], acc);
-			C[i + j * n] = acc;
+	for (int j = 0; j < l; j++) {
+		float *c = C + (size_t)j * n;
+		for (int i = 0; i < n; i++) c[i] = 0.f;
+		for (int p = 0; p < m; p++) {
+			const float b = B[j + p * l];
+			const float *a = A + (size_t)p * n;
+			for (int i = 0; i < n; i++) c[i] = fmaf(a[i], b, c[i]);
 		}
+	}
 }
 
 /* ------------------------------------------------------------------------ */
