@@ -71,6 +71,8 @@ __global__ void fill_u8_kernel(uint8_t *, uint8_t, int64_t);
 template <int OP> __global__ void scan_block_kernel(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
 template <int OP> __global__ void scan_top_kernel(uint32_t *, int64_t);
 template <int OP> __global__ void scan_add_kernel(uint32_t *, int64_t, const uint32_t *);
+template <int OP> __global__ void scan_block_w_kernel(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
+template <int OP> __global__ void scan_top_w_kernel(uint32_t *, int64_t);
 __global__ void count_rows_kernel(const uint32_t *, const uint32_t *, int64_t, uint32_t *);
 __global__ void scatter_rows_kernel(const uint32_t *, const uint32_t *, int64_t, const uint32_t *, uint32_t *,
                                     uint32_t *);
@@ -629,7 +631,19 @@ void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n,
 	const int64_t nb = (n + 1023) / 1024;
 	uint32_t *bsum = (side ? ctx->scan_tmp2 : ctx->scan_tmp).as<uint32_t>((size_t)nb + 2);
 	hipStream_t st = side ? ctx->side : ctx->stream;
-	if (op == 0) {
+	if (side) {
+		// one-wave blocks on the second stream, so that they get dispatched
+		// beside the one-wave blocks of the filter
+		if (op == 0) {
+			hipLaunchKernelGGL(scan_block_w_kernel<0>, dim3((unsigned)nb), dim3(64), 0, st, in, n_in, n, out, bsum);
+			hipLaunchKernelGGL(scan_top_w_kernel<0>, dim3(1), dim3(64), 0, st, bsum, nb);
+			hipLaunchKernelGGL(scan_add_kernel<0>, dim3(nblk(n, 64)), dim3(64), 0, st, out, n, bsum);
+		} else {
+			hipLaunchKernelGGL(scan_block_w_kernel<1>, dim3((unsigned)nb), dim3(64), 0, st, in, n_in, n, out, bsum);
+			hipLaunchKernelGGL(scan_top_w_kernel<1>, dim3(1), dim3(64), 0, st, bsum, nb);
+			hipLaunchKernelGGL(scan_add_kernel<1>, dim3(nblk(n, 64)), dim3(64), 0, st, out, n, bsum);
+		}
+	} else if (op == 0) {
 		hipLaunchKernelGGL(scan_block_kernel<0>, dim3((unsigned)nb), dim3(256), 0, st, in, n_in, n, out, bsum);
 		hipLaunchKernelGGL(scan_top_kernel<0>, dim3(1), dim3(256), 0, st, bsum, nb);
 		hipLaunchKernelGGL(scan_add_kernel<0>, dim3(nblk(n, 256)), dim3(256), 0, st, out, n, bsum);
@@ -693,7 +707,7 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 	const BinGrid bg = bin_grid(sc);
 	const BinBufs q = bin_bufs(ctx, sc, Mub);
 	hipStream_t st = ctx->side;
-	hipLaunchKernelGGL(bin_pair_count_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, st, epos, Mp, Mub, sc, q.pcnt);
+	hipLaunchKernelGGL(bin_pair_count_kernel, dim3(nblk(Mub, 64)), dim3(64), 0, st, epos, Mp, Mub, sc, q.pcnt);
 	dscan(ctx, 0, q.pcnt, Mub, Mub + 1, q.poff, true);
 	const uint32_t *np = q.poff + Mub;   // pair count
 	int bits = 1;
@@ -705,7 +719,7 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 		std::swap(kin, kout);
 		std::swap(vin, vout);
 	}
-	hipLaunchKernelGGL(bin_pair_emit_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, st, epos, Mp, Mub, sc, bg.nbx,
+	hipLaunchKernelGGL(bin_pair_emit_kernel, dim3(nblk(Mub, 64)), dim3(64), 0, st, epos, Mp, Mub, sc, bg.nbx,
 	                   bg.nby, q.poff, kin, vin);
 	HIPCHK(hipGetLastError());
 	for (int ps = 0; ps < passes; ps++) {
@@ -718,7 +732,7 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 		std::swap(vin, vout);
 	}
 	const int64_t nub = 8 * Mub;
-	hipLaunchKernelGGL(bin_bounds_kernel, dim3((unsigned)std::min<int64_t>(nblk(nub + 1, 256), 4096)), dim3(256), 0, st,
+	hipLaunchKernelGGL(bin_bounds_kernel, dim3((unsigned)std::min<int64_t>(nblk(nub + 1, 64), 16384)), dim3(64), 0, st,
 	                   q.k0, np, nub, bg.nbins, q.boff);
 	HIPCHK(hipGetLastError());
 }
@@ -1114,7 +1128,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		// the side stream, beside the filter (only the aggregation reads them)
 		HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
 		HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->side, d_idx, d_nsim,
+		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk(Mub, 64)), dim3(64), 0, ctx->side, d_idx, d_nsim,
 		                   d_base, np, g0, Gc, d_epos);
 		HIPCHK(hipGetLastError());
 		hipLaunchKernelGGL(add_rows_kernel, dim3(1), dim3(64), 0, ctx->side, d_M, d_rows);

@@ -103,6 +103,72 @@ void scan_top_kernel(uint32_t *__restrict__ bsum, int64_t nb)
 	}
 }
 
+// One-wave forms of the two scan passes (1024 elements per block, 16 per
+// lane), for the second stream: blocks of one wave get dispatched between
+// the one-wave blocks of a kernel running beside them.
+template <int OP> __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v)
+{
+	const int lane = threadIdx.x & 63;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) {
+		const uint32_t o = (uint32_t)__shfl_up((int)v, d, 64);
+		if (lane >= d) v = scan_op<OP>(v, o);
+	}
+	return v;
+}
+
+template <int OP>
+__global__ __launch_bounds__(64)
+void scan_block_w_kernel(const uint32_t *__restrict__ in, int64_t n_in, int64_t n, uint32_t *__restrict__ out,
+                         uint32_t *__restrict__ bsum)
+{
+	constexpr int PER = SCAN_ITEMS / 64;
+	const int64_t b0 = (int64_t)blockIdx.x * SCAN_ITEMS + (int64_t)threadIdx.x * PER;
+	uint32_t v[PER];
+	uint32_t acc = 0;
+#pragma unroll
+	for (int i = 0; i < PER; i++) {
+		const int64_t j = b0 + i;
+		v[i] = j < n_in ? in[j] : 0u;
+		acc = scan_op<OP>(acc, v[i]);
+	}
+	const uint32_t incl = wave_incl_scan_u32<OP>(acc);
+	const uint32_t before = (uint32_t)__shfl_up((int)incl, 1, 64);
+	uint32_t run = OP == 0 ? incl - acc : (threadIdx.x ? before : 0u);
+#pragma unroll
+	for (int i = 0; i < PER; i++) {
+		const int64_t j = b0 + i;
+		if (OP == 0) {
+			if (j < n) out[j] = run;   // exclusive
+			run += v[i];
+		} else {
+			run = scan_op<OP>(run, v[i]);
+			if (j < n) out[j] = run;   // inclusive
+		}
+	}
+	if (threadIdx.x == 63) bsum[blockIdx.x] = incl;
+}
+
+template <int OP>
+__global__ __launch_bounds__(64)
+void scan_top_w_kernel(uint32_t *__restrict__ bsum, int64_t nb)
+{
+	uint32_t carry = 0;
+	for (int64_t c0 = 0; c0 < nb; c0 += 64) {
+		const int64_t j = c0 + threadIdx.x;
+		const uint32_t v = j < nb ? bsum[j] : 0u;
+		const uint32_t incl = wave_incl_scan_u32<OP>(v);
+		const uint32_t before = (uint32_t)__shfl_up((int)incl, 1, 64);
+		const uint32_t ex = threadIdx.x ? before : 0u;
+		if (j < nb) bsum[j] = scan_op<OP>(carry, OP == 0 ? incl - v : ex);
+		carry = scan_op<OP>(carry, (uint32_t)__shfl((int)incl, 63, 64));
+	}
+}
+template __global__ void scan_block_w_kernel<0>(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
+template __global__ void scan_block_w_kernel<1>(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
+template __global__ void scan_top_w_kernel<0>(uint32_t *, int64_t);
+template __global__ void scan_top_w_kernel<1>(uint32_t *, int64_t);
+
 template <int OP>
 __global__ void scan_add_kernel(uint32_t *__restrict__ out, int64_t n, const uint32_t *__restrict__ bsum)
 {
@@ -236,8 +302,10 @@ __global__ __launch_bounds__(RS_THREADS)
 void rs_hist_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ np, int shift, int nblk,
                     uint32_t *__restrict__ hist)
 {
+	constexpr int DPT = 256 / RS_THREADS;   // digits per thread
 	__shared__ uint32_t h[256];
-	h[threadIdx.x] = 0;
+#pragma unroll
+	for (int i = 0; i < DPT; i++) h[threadIdx.x + i * RS_THREADS] = 0;
 	__syncthreads();
 	const uint64_t n = *np;
 	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
@@ -247,69 +315,62 @@ void rs_hist_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restric
 			if (k < n) atomicAdd(&h[(keys[k] >> shift) & 255u], 1u);
 		}
 	__syncthreads();
-	hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+#pragma unroll
+	for (int i = 0; i < DPT; i++) {
+		const int d = threadIdx.x + i * RS_THREADS;
+		hist[(size_t)d * nblk + blockIdx.x] = h[d];
+	}
 }
 
+// One wave per tile: rounds of 64 pairs in input order; the lanes of a
+// round with the same digit take consecutive slots after the digit's
+// running offset, and the lowest of them advances it.
 __global__ __launch_bounds__(RS_THREADS)
 void rs_scatter_kernel(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                        const uint32_t *__restrict__ np, int shift, int nblk, const uint32_t *__restrict__ offs,
                        uint32_t *__restrict__ kout, uint32_t *__restrict__ vout)
 {
-	constexpr int NW = RS_THREADS / 64;
+	static_assert(RS_THREADS == 64, "one wave per tile");
+	constexpr int DPT = 256 / RS_THREADS, BATCH = 16;
 	__shared__ uint32_t base[256];
-	__shared__ uint32_t wcnt[2][NW][256];   // per round (two in flight): pairs of each digit per wave
-	const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+	const int lane = threadIdx.x;
 	const uint64_t n = *np;
 	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
 	if (b0 >= n) return;
-	// the tile's pairs in registers (round r: pairs b0 + r RS_THREADS + tid)
-	uint32_t key[RS_IPT], val[RS_IPT];
 #pragma unroll
-	for (int r = 0; r < RS_IPT; r++) {
-		const uint64_t k = b0 + (uint64_t)r * RS_THREADS + tid;
-		key[r] = k < n ? kin[k] : 0u;
-		val[r] = k < n ? vin[k] : 0u;
-	}
-	base[tid] = offs[(size_t)tid * nblk + blockIdx.x];
-#pragma unroll
-	for (int w = 0; w < NW; w++) {
-		wcnt[0][w][tid] = 0;
-		wcnt[1][w][tid] = 0;
-	}
+	for (int i = 0; i < DPT; i++) base[lane + i * 64] = offs[(size_t)(lane + i * 64) * nblk + blockIdx.x];
 	__syncthreads();
 	const uint64_t lt = (1ull << lane) - 1ull;
+	for (int r0 = 0; r0 < RS_IPT; r0 += BATCH) {
+		if (b0 + (uint64_t)r0 * 64 >= n) break;
+		uint32_t key[BATCH], val[BATCH];
 #pragma unroll
-	for (int r = 0; r < RS_IPT; r++) {
-		const int pb = r & 1;
-		const bool valid = b0 + (uint64_t)r * RS_THREADS + tid < n;
-		const uint32_t d = valid ? (key[r] >> shift) & 255u : 0u;
-		// the lanes of this wave with the same digit
-		uint64_t m = __ballot(valid);
+		for (int r = 0; r < BATCH; r++) {
+			const uint64_t k = b0 + (uint64_t)(r0 + r) * 64 + lane;
+			key[r] = k < n ? kin[k] : 0u;
+			val[r] = k < n ? vin[k] : 0u;
+		}
 #pragma unroll
-		for (int bit = 0; bit < 8; bit++) {
-			const bool bs = (d >> bit) & 1u;
-			const uint64_t bb = __ballot(valid && bs);
-			m &= bs ? bb : ~bb;
-		}
-		const uint32_t rk = (uint32_t)__popcll(m & lt);
-		if (valid && rk == 0) wcnt[pb][wv][d] = (uint32_t)__popcll(m);
-		__syncthreads();
-		if (valid) {
-			uint32_t pos = base[d] + rk;
-			for (int w = 0; w < wv; w++) pos += wcnt[pb][w][d];
-			kout[pos] = key[r];
-			vout[pos] = val[r];
-		}
-		__syncthreads();
-		// the round's totals: advance the digit offsets, clear the counts for
-		// round r + 2
-		uint32_t t = 0;
+		for (int r = 0; r < BATCH; r++) {
+			const bool valid = b0 + (uint64_t)(r0 + r) * 64 + lane < n;
+			const uint32_t d = valid ? (key[r] >> shift) & 255u : 0u;
+			uint64_t m = __ballot(valid);
 #pragma unroll
-		for (int w = 0; w < NW; w++) {
-			t += wcnt[pb][w][tid];
-			wcnt[pb][w][tid] = 0;
+			for (int bit = 0; bit < 8; bit++) {
+				const bool bs = (d >> bit) & 1u;
+				const uint64_t bb = __ballot(valid && bs);
+				m &= bs ? bb : ~bb;
+			}
+			const uint32_t rk = (uint32_t)__popcll(m & lt);
+			const uint32_t bd = base[d];
+			__syncthreads();   // every lane has read its digit's offset
+			if (valid) {
+				kout[bd + rk] = key[r];
+				vout[bd + rk] = val[r];
+				if (rk == 0) base[d] = bd + (uint32_t)__popcll(m);
+			}
+			__syncthreads();
 		}
-		base[tid] += t;
 	}
 }
 
