@@ -3193,14 +3193,22 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	constexpr int P = 4 * PQ, SL = 4 * SQ, QR = CC * PQ;   // QR: float4 of one row
 	constexpr int SPL = (NBR * QR + 63) / 64;              // float4 staged per lane per chunk
 	constexpr int RPR = CC == 1 ? 4 : 2;                   // rows folded per round
-	__shared__ __align__(16) float4 stage4[NBR * QR];
+	static_assert(SPL * 64 == NBR * QR, "the staging fills the chunk's slots exactly");
+	// a round's missing rows read the record NBR: weight +0 and samples +0
+	// from a zeroed area after the chunk's slots, so that every round adds
+	// RPR terms without a select (adding +0 leaves a sum that is not -0
+	// unchanged, and these sums start at +0 and never become -0)
+	constexpr int ZR = (TB - 1) * SL + 64;                 // span of the lane offsets f SL + pix_l
+	constexpr int ZQ = ((CC - 1) * P + ZR + 3) / 4;
+	constexpr int SWN = P > ZR ? P : ZR;
+	__shared__ __align__(16) float4 stage4[NBR * QR + ZQ];
 	const float *stage = reinterpret_cast<const float *>(stage4);
-	__shared__ float swin[P];
+	__shared__ float swin[SWN];
 	// per row of a chunk (two chunks in flight): x = stage index and y =
 	// window index of the block pixel (0, 0) at the bin's first frame (add
 	// f SL + the lane's slice index), z = weight bits; rowm = (row index e,
 	// bit dt set for the slices at the bin's frames)
-	__shared__ int4 rrec[2][NBR];
+	__shared__ int4 rrec[2][NBR + 1];
 	__shared__ uint2 rowm[2][NBR];
 	const int lane = threadIdx.x;
 	// bins are dealt to the 8 XCDs in contiguous eighths (x fastest)
@@ -3247,7 +3255,9 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 				an[f][c] = num[(size_t)(t0 + f) * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x];
 		}
 	}
-	for (int i = lane; i < P; i += 64) swin[i] = win[i];
+	for (int i = lane; i < SWN; i += 64) swin[i] = i < P ? win[i] : 0.f;
+	for (int i = lane; i < ZQ; i += 64) stage4[NBR * QR + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	if (lane < 2) rrec[lane][NBR] = make_int4(NBR * QR * 4, 0, 0, 0);
 	const int pix_l = ly * sPx + lx;
 	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
 	const float4 *est4 = reinterpret_cast<const float4 *>(est);
@@ -3349,18 +3359,16 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 		__syncthreads();
 		if (c + 1 < nch) AGG_BIN_STAGE_ISSUE(c + 1);
 		// each lane folds the rows that cover its pixel, frame by frame, in
-		// row order, up to RPR rows per round (their LDS reads overlap)
+		// row order, RPR rows per round (their LDS reads overlap)
 		const uint32_t mine_sp = (uint32_t)transpose_bits64(mC);
 #pragma unroll
 		for (int f = 0; f < TB; f++) {
 			uint32_t mine = mine_sp & vmC[f];
 			while (mine) {
 				int jj[RPR];
-				bool has[RPR];
 #pragma unroll
 				for (int u = 0; u < RPR; u++) {
-					has[u] = mine != 0;
-					jj[u] = has[u] ? __ffs(mine) - 1 : jj[0];
+					jj[u] = mine ? __ffs(mine) - 1 : NBR;
 					mine &= mine - 1;
 				}
 				int4 rr[RPR];
@@ -3374,12 +3382,11 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 					for (int cc = 0; cc < CC; cc++) xx[u][cc] = stage[rr[u].x + cc * P + plf[f]];
 				}
 #pragma unroll
-				for (int u = 0; u < RPR; u++)
-					if (has[u]) {
-						aw[f] = aw[f] + __int_as_float(rr[u].z) * vv[u];
+				for (int u = 0; u < RPR; u++) {
+					aw[f] = aw[f] + __int_as_float(rr[u].z) * vv[u];
 #pragma unroll
-						for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + xx[u][cc];
-					}
+					for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + xx[u][cc];
+				}
 			}
 		}
 		__syncthreads();
