@@ -240,8 +240,23 @@ def batch32(A, S, ctx, torch, nparts):
     runs = [batch.run() for _ in range(3)]
     batch.close()
     ms = float(np.median(runs))
+    # the same sequences one at a time through the single-video entry point
+    # (default options, each run once to warm the buffers, then timed)
+    single_ms = 0.0
+    for q, p1, p2 in zip(seqs, p1s, p2s):
+        shape = (q.f, 1, q.h, q.w)
+
+        def one():
+            ctx.run_dev(p1, p2, q.noisy, shape, 0, 0, q.basic, q.final, nparts=nparts)
+        one()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one()
+        torch.cuda.synchronize()
+        single_ms += (time.perf_counter() - t0) * 1e3
     return {"value": round(pix / (ms * 1e-3) / 1e6, 4), "unit": "Mpix/s", "sequences": 32,
             "mpix": round(pix / 1e6, 3), "ms_per_batch": round(ms, 3), "replays_ms": [round(r, 3) for r in runs],
+            "singles_sum_ms": round(single_ms, 3), "speedup_vs_singles": round(single_ms / ms, 3),
             "create_s_incl_autotune": round(create_s, 2), "generation_s": round(gen_s, 1)}
 
 
@@ -336,6 +351,9 @@ def main():
     ap.add_argument("--no-batch32", action="store_true",
                     help="skip the batched workload (32 uneven gray sequences in one hipGraph)")
     ap.add_argument("--cpu-reps", type=int, default=3, help="CPU baseline runs (median)")
+    ap.add_argument("--cpu-sweep", default="",
+                    help="comma-separated thread counts (e.g. 1,8,16): time the CPU baseline at each "
+                         "(nParts = 2 T, median of --cpu-reps) and report the fastest as cpu_baseline")
     ap.add_argument("--cpu-tvl1-pairs", type=int, default=16,
                     help="full-size pairs of the CPU TV-L1 sample (0: none)")
     ap.add_argument("--learned-eval", type=int, default=0, metavar="N",
@@ -522,9 +540,22 @@ def main():
             q2 = A.cli_params(2, SIGMA, n.shape, px=4, pt=4, wx=11, np_=32)
             return ctx.run(q1, q2, n, f1, b1, nparts=nparts)[1]
 
-        out["cpu_baseline"], out["parity_vs_cpu"] = cpu_baseline(
-            noisy, clean, d_ff.cpu().numpy(), d_bf.cpu().numpy(), (cw, ch), thr, gpu_run,
-            reps=args.cpu_reps, tv_pairs=args.cpu_tvl1_pairs)
+        h_ff, h_bf = d_ff.cpu().numpy(), d_bf.cpu().numpy()
+        if args.cpu_sweep:
+            sweep = []
+            for t in (int(v) for v in args.cpu_sweep.split(",")):
+                print(f"[bench] CPU sweep: {t} threads", file=sys.stderr, flush=True)
+                cb, par = cpu_baseline(noisy, clean, h_ff, h_bf, (cw, ch), t, gpu_run, reps=args.cpu_reps,
+                                       tv_pairs=args.cpu_tvl1_pairs if t == thr else 0)
+                sweep.append((cb, par))
+            out["cpu_sweep"] = [{"threads": cb["cores"], "nparts": 2 * cb["cores"], "value": cb["value"],
+                                 "sample": cb["sample"], "identical_to_gpu": par["identical"]}
+                                for cb, par in sweep]
+            out["cpu_baseline"], out["parity_vs_cpu"] = max(sweep, key=lambda q: q[0]["value"])
+        else:
+            out["cpu_baseline"], out["parity_vs_cpu"] = cpu_baseline(
+                noisy, clean, h_ff, h_bf, (cw, ch), thr, gpu_run,
+                reps=args.cpu_reps, tv_pairs=args.cpu_tvl1_pairs)
     else:
         out["cpu_baseline"] = None
     ctx.close()

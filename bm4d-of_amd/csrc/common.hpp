@@ -47,7 +47,7 @@ struct AggRegion { int RX, RY, RT, tb0; };
 // stable radix sort passes (sortscan.hip): threads per block, pairs per
 // thread and per block (tile)
 #define RS_THREADS 64
-#define RS_IPT 64
+#define RS_IPT 256
 #define RS_TILE (RS_THREADS * RS_IPT)
 
 // Decode a 3-index position t*WH + y*W + x.
