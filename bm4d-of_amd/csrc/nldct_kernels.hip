@@ -112,6 +112,22 @@ __device__ __forceinline__ bool bm_has_tie(const uint32_t *__restrict__ skey, in
 	return __any(eq) || le > n;
 }
 
+// The same test over candidates held as (distance << 32 | rank + 1).
+__device__ __forceinline__ bool bm_has_tie(const uint64_t *__restrict__ cand, int ncand,
+                                           const uint32_t *__restrict__ out_key, int nSimP, float tau2)
+{
+	const int lane = threadIdx.x & 63;
+	int n = 1;
+	while (2 * n <= nSimP && __uint_as_float(out_key[n]) < tau2) n *= 2;
+	const uint32_t last = out_key[n - 1];
+	const bool eq = lane + 1 < n && out_key[lane] == out_key[lane + 1];
+	int le = 0;
+	for (int s = lane; s < ncand; s += 64) le += (uint32_t)(cand[s] >> 32) <= last ? 1 : 0;
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) le += __shfl_xor(le, o, 64);
+	return __any(eq) || le > n;
+}
+
 __device__ __forceinline__ void bm_tie_append(int g, uint32_t *__restrict__ tie_count,
                                               uint32_t *__restrict__ tie_list, int tie_cap)
 {
@@ -956,12 +972,14 @@ void bm_fast_kernel(const float *__restrict__ img, const int4 *__restrict__ wins
 // (vector loads from global memory, cached) and computes one distance, the
 // sequential (ht, hy, hx) sum of the reference (nldct.cpp:1262-1274,
 // 1723-1737), with the reference patch in scalar registers.  Distances go to
-// LDS by scan index.  Selection: the kk-th smallest of the 64 per-lane
-// minima bounds the kk-th smallest distance from above (kk lanes each own a
-// distance at or below it), so only the candidates at or below that bound
-// are ranked, in the total order of bm_radix_select; more than BMR_CAP of
-// them (or fewer than kk lanes with candidates) falls back to the radix
-// select over all distances.  Same output as bm_fast_kernel / bm_kernel.
+// LDS by scan index.  Selection: every lane keeps its two smallest
+// distances; the kk-th smallest of these 128 bounds the kk-th smallest
+// distance from above (they are kk distances of distinct candidates at or
+// below it), so only the candidates at or below that bound are ranked, in
+// the total order of bm_radix_select (distance, then scan index with the
+// reference patch first) as one 64-bit key; more than BMR_CAP of them (or
+// fewer than kk finite values) falls back to the radix select over all
+// distances.  Same output as bm_fast_kernel / bm_kernel.
 // ---------------------------------------------------------------------------
 
 #define BMR_CAP 128
@@ -997,8 +1015,7 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	const int maxCand = sc.sWx * sc.sWx * sWt;
 	uint32_t *skey = (uint32_t *)smem;          // maxCand distances
 	int *stie = (int *)(skey + maxCand);         // maxCand (radix fallback)
-	__shared__ uint32_t cand_key[BMR_CAP];
-	__shared__ int cand_rank[BMR_CAP];
+	__shared__ uint64_t cand[BMR_CAP];   // (distance << 32) | (scan index + 1; 0: the reference patch)
 	__shared__ uint32_t out_key[64];
 	__shared__ int out_rank[64];
 
@@ -1022,7 +1039,7 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	// items: (frame j, column cx, row chunk r), frame-major
 	int total = 0;
 	for (int j = 0; j < nframes; j++) total += (gw[j].z & 0xffff) * rchunks;
-	uint32_t lmin = 0xffffffffu;
+	uint32_t lmin = 0xffffffffu, lmin2 = 0xffffffffu;   // the lane's two smallest distances
 	// channels in order (step 1: channel 0; step 2: all): a channel's sums
 	// continue the previous channel's, which the same lane stored in skey
 	// (MC: more than one channel; the one-channel form keeps no such state)
@@ -1076,7 +1093,10 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 								}
 						const uint32_t kb = __float_as_uint(d);
 						kout[(y + u) * nx] = kb;
-						if (!MC || last) lmin = min(lmin, kb);
+						if (!MC || last) {
+							lmin2 = min(lmin2, max(lmin, kb));
+							lmin = min(lmin, kb);
+						}
 					}
 				}
 			}
@@ -1092,20 +1112,31 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	int kk = 1;
 	while (kk * 2 <= nSimP) kk *= 2;
 
-	// bound: the kk-th smallest lane minimum (lanes without candidates hold
-	// ~0, which then means no bound), by a bitonic sort across the wave
+	// bound: the kk-th smallest of the lanes' two smallest distances (a lane
+	// with fewer candidates holds ~0, which then means no bound), by a
+	// bitonic sort of the 128 values (value i = lane + 64 k in v[k])
 	uint32_t T0;
 	{
-		uint32_t v = lmin;
+		uint32_t v[2] = {lmin, lmin2};
 #pragma unroll
-		for (int k = 2; k <= 64; k <<= 1)
+		for (int size = 2; size <= 128; size <<= 1)
 #pragma unroll
-			for (int j = k >> 1; j > 0; j >>= 1) {
-				const uint32_t o = (uint32_t)__shfl_xor((int)v, j, 64);
-				const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-				v = keep_min ? min(v, o) : max(v, o);
+			for (int j = size >> 1; j > 0; j >>= 1) {
+				if (j == 64) {
+					const uint32_t lo = min(v[0], v[1]), hi = max(v[0], v[1]);
+					v[0] = lo;
+					v[1] = hi;
+				} else {
+#pragma unroll
+					for (int k = 0; k < 2; k++) {
+						const int i = lane + 64 * k;
+						const uint32_t o = (uint32_t)__shfl_xor((int)v[k], j, 64);
+						const bool keep_min = ((i & j) == 0) == ((i & size) == 0);
+						v[k] = keep_min ? min(v[k], o) : max(v[k], o);
+					}
+				}
 			}
-		T0 = (uint32_t)__shfl((int)v, kk - 1, 64);
+		T0 = (uint32_t)__shfl((int)v[0], kk - 1, 64);
 	}
 	int cnt = 0;
 	bool fits = T0 != 0xffffffffu;
@@ -1117,10 +1148,8 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 			const bool take = s2 < nsrch && k <= T0;
 			const uint64_t m = __ballot(take);
 			const int before = __popcll(m & ((1ull << lane) - 1ull));
-			if (take && cnt + before < BMR_CAP) {
-				cand_key[cnt + before] = k;
-				cand_rank[cnt + before] = s2 == s_self ? -1 : s2;
-			}
+			if (take && cnt + before < BMR_CAP)
+				cand[cnt + before] = ((uint64_t)k << 32) | (uint32_t)(s2 == s_self ? 0 : s2 + 1);
 			cnt += __popcll(m);
 		}
 		fits = cnt <= BMR_CAP;
@@ -1128,16 +1157,12 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	__syncthreads();
 	if (fits) {
 		for (int e = lane; e < cnt; e += 64) {
-			const uint32_t k = cand_key[e];
-			const int r = cand_rank[e];
+			const uint64_t c = cand[e];
 			int p = 0;
-			for (int j = 0; j < cnt; j++) {
-				const uint32_t kj = cand_key[j];
-				p += (kj < k) || (kj == k && cand_rank[j] < r);
-			}
+			for (int j = 0; j < cnt; j++) p += cand[j] < c;
 			if (p < kk) {
-				out_key[p] = k;
-				out_rank[p] = r;
+				out_key[p] = (uint32_t)(c >> 32);
+				out_rank[p] = (int)(uint32_t)c - 1;
 			}
 		}
 		__syncthreads();
@@ -1146,7 +1171,7 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	}
 	__syncthreads();
 	// every distance up to the bound is among the ranked candidates
-	const bool tie = tie_list ? (fits ? bm_has_tie(cand_key, cnt, out_key, nSimP, sc.tau2)
+	const bool tie = tie_list ? (fits ? bm_has_tie(cand, cnt, out_key, nSimP, sc.tau2)
 	                                  : bm_has_tie(skey, nsrch, out_key, nSimP, sc.tau2))
 	                          : false;
 	if (tie && lane == 0) bm_tie_append(g, tie_count, tie_list, tie_cap);
