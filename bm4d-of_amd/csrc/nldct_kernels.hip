@@ -1036,9 +1036,15 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 	const int nframes = meta.x, nsrch = meta.y;
 	const int4 *gw = wins + (size_t)g * sWt;
 
-	// items: (frame j, column cx, row chunk r), frame-major
+	// items: (frame j, column cx, row chunk r), frame-major; lane j <
+	// nframes holds frame j's item count and the inclusive sum up to it, so
+	// that an item finds its frame without a chain of dependent loads
+	const int my_nit = lane < nframes ? (gw[lane].z & 0xffff) * rchunks : 0;
+	const int my_cum = wave_incl_scan(my_nit);
 	int total = 0;
-	for (int j = 0; j < nframes; j++) total += (gw[j].z & 0xffff) * rchunks;
+	if (nframes <= 64) total = __shfl(my_cum, nframes - 1, 64);
+	else
+		for (int j = 0; j < nframes; j++) total += (gw[j].z & 0xffff) * rchunks;
 	uint32_t lmin = 0xffffffffu, lmin2 = 0xffffffffu;   // the lane's two smallest distances
 	// channels in order (step 1: channel 0; step 2: all): a channel's sums
 	// continue the previous channel's, which the same lane stored in skey
@@ -1059,12 +1065,24 @@ void bm_reg_kernel(const float *__restrict__ img, const int4 *__restrict__ wins,
 					    imc[(size_t)(gt + ht) * sc.WHC + (size_t)(gy + hy) * sc.W + gx + hx];
 		for (int it = lane; it < total; it += 64) {
 			int j = 0, loc = it;
-			int4 f = gw[0];
-			for (;;) {
-				const int nit = (f.z & 0xffff) * rchunks;
-				if (loc < nit) break;
-				loc -= nit;
-				f = gw[++j];
+			int4 f;
+			if (nframes <= 64) {
+				// (readlane reads lane jj whatever the branch's active lanes)
+				int base = 0;
+				for (int jj = 0; jj + 1 < nframes; jj++) {
+					const int c = __builtin_amdgcn_readlane(my_cum, jj);   // first item of frame jj + 1
+					if (it >= c) { j = jj + 1; base = c; }
+				}
+				loc = it - base;
+				f = gw[j];
+			} else {
+				f = gw[0];
+				for (;;) {
+					const int nit = (f.z & 0xffff) * rchunks;
+					if (loc < nit) break;
+					loc -= nit;
+					f = gw[++j];
+				}
 			}
 			const int nx = f.z & 0xffff, ny = f.z >> 16;
 			const int r = loc / nx, cx = loc - r * nx;
