@@ -741,6 +741,28 @@ constexpr int TV_WORK = 16;     // working planes at the finest size
 
 inline int tv_tiles(int nx, int ny) { return ((nx + TV_TX - 1) / TV_TX) * ((ny + TV_TY - 1) / TV_TY); }
 
+// Scales whose 64 x 32 tiles give fewer than TV_SMALL_BLOCKS blocks over all
+// pairs run 32 x 16 tiles in blocks of three waves: more blocks to spread over
+// the CUs where the launch is too small to fill them.  A pixel's arithmetic
+// does not depend on the tile it is in.
+#define TV_STX 32
+#define TV_STY 16
+#define TV_SNT 192
+#define TV_SMALL_BLOCKS 1024
+inline bool tv_small(int nx, int ny, int P) { return (int64_t)tv_tiles(nx, ny) * P < TV_SMALL_BLOCKS; }
+inline int tv_tiles_x(int nx, int ny, int P) { return (nx + (tv_small(nx, ny, P) ? TV_STX : TV_TX) - 1) / (tv_small(nx, ny, P) ? TV_STX : TV_TX); }
+inline int tv_tiles_at(int nx, int ny, int P)
+{
+	return tv_small(nx, ny, P) ? ((nx + TV_STX - 1) / TV_STX) * ((ny + TV_STY - 1) / TV_STY) : tv_tiles(nx, ny);
+}
+// most tiles of any scale: the per-iteration partial sums per pair
+inline int tv_part_tiles(const TVPlan &pl)
+{
+	int m = 0;
+	for (int s = 0; s < pl.nscales; s++) m = std::max(m, tv_tiles_at(pl.nx[s], pl.ny[s], pl.P));
+	return m;
+}
+
 // Byte offsets of the parts of a TVScratch carved from one region, and the
 // region's size (the last entry), each part 256-byte aligned.
 std::array<size_t, 8> tv_scratch_layout(const TVPlan &pl, size_t size0)
@@ -753,7 +775,7 @@ std::array<size_t, 8> tv_scratch_layout(const TVPlan &pl, size_t size0)
 	o[2] = o[1] + al(pl.pyr * 4 * sizeof(float));                      // working planes
 	o[3] = o[2] + al(P * size0 * TV_WORK * sizeof(float));             // min/max
 	o[4] = o[3] + al(P * TV_MMB * 4 * sizeof(float));                  // iteration partial sums
-	o[5] = o[4] + al(P * TV_K * (size_t)tv_tiles(pl.nx[0], pl.ny[0]) * sizeof(double));
+	o[5] = o[4] + al(P * TV_K * (size_t)tv_part_tiles(pl) * sizeof(double));
 	o[6] = o[5] + al((5 * P + 1) * sizeof(int));                       // taps
 	o[7] = o[6] + al(64 * sizeof(double));                             // total
 	return o;
@@ -780,7 +802,7 @@ TVScratch tv_scratch_ctx(bm4dof_ctx *ctx, const TVPlan &pl, size_t size0)
 	t.pyr = ctx->tv_pyr.as<float>(pl.pyr * 4);
 	t.work = ctx->tv_work.as<float>(P * size0 * TV_WORK);
 	t.mm = ctx->tv_mm.as<float>(P * TV_MMB * 4);
-	t.part = ctx->tv_part.as<double>(P * TV_K * (size_t)tv_tiles(pl.nx[0], pl.ny[0]));
+	t.part = ctx->tv_part.as<double>(P * TV_K * (size_t)tv_part_tiles(pl));
 	t.state = ctx->tv_state.as<int>(5 * P + 1);
 	t.taps = ctx->tv_taps.as<double>(64);
 	return t;
@@ -881,7 +903,8 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		const int g = tv_grid(n);
 		const TVPlanes SA = {u1, u2, p11, p12, p21, p22};
 		const TVPlanes SB = {tmp, tmp2, work + 12 * W, work + 13 * W, work + 14 * W, work + 15 * W};
-		const int tiles_x = (nx + TV_TX - 1) / TV_TX, ntiles = tv_tiles(nx, ny);
+		const bool small = tv_small(nx, ny, P);
+		const int tiles_x = tv_tiles_x(nx, ny, P), ntiles = tv_tiles_at(nx, ny, P);
 		hipLaunchKernelGGL(tv_cgrad_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1, nx, ny, n, I1x, I1y);
 		const size_t nP = (size_t)P * n;
 		for (float *q : {p11, p12, p21, p22})
@@ -896,9 +919,14 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 			for (int r = 0; r < max_rounds;) {
 				const int rb = std::min(batch, max_rounds - r);
 				for (int i = 0; i < rb; i++) {
-					hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0,
-					                   st, I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut,
-					                   d_ctl, d_cur, d_part);
+					if (small)
+						hipLaunchKernelGGL((tv_iter_kernel<TV_STX, TV_STY, TV_K, TV_SNT>), dim3(ntiles, P), dim3(TV_SNT),
+						                   0, st, I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta,
+						                   taut, d_ctl, d_cur, d_part);
+					else
+						hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0,
+						                   st, I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut,
+						                   d_ctl, d_cur, d_part);
 					hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n,
 					                   eps2, d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
 				}
