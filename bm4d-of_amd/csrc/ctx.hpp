@@ -97,7 +97,8 @@ struct bm4dof_ctx {
 	int bm_frames = 2;         // frames of search window staged per round of the staged matching
 	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
 	int agg_cap = 1024;        // rows per e-range of the chunked aggregation (one channel)
-	bool agg_binned = true;    // binned aggregation (row lists per 8 x 8 x 4 bin) where the shape allows it
+	bool agg_binned = true;
+	int side_blocks = 2048;    // most blocks of a second-stream kernel (0: no limit)    // binned aggregation (row lists per 8 x 8 x 4 bin) where the shape allows it
 	bool vbm3d_search = false; // block matching of a VBM3D_SEARCH build (predictive search)
 	// learned gain network (mlp.hip)
 	bm4dof::DevBuf mlp_theta, mlp_m, mlp_v, mlp_wq, mlp_inf, mlp_ws, mlp_grad, mlp_xb, mlp_tb, mlp_sse;

@@ -616,6 +616,15 @@ void fill_f32(bm4dof_ctx *ctx, float *p, float v, int64_t n)
 	HIPCHK(hipGetLastError());
 }
 
+// Grid of a second-stream kernel: at most side_blocks one-wave blocks (the
+// kernels loop over their work), so that they take few of the wave slots
+// the filter beside them uses; 0 = one block per unit of work.
+static unsigned side_grid(const bm4dof_ctx *ctx, int64_t blocks)
+{
+	if (ctx->side_blocks > 0) blocks = std::min<int64_t>(blocks, ctx->side_blocks);
+	return (unsigned)std::max<int64_t>(1, blocks);
+}
+
 // out[0..n) = scan of in[0..n_in) (identity past n_in): op 0 exclusive sum,
 // op 1 inclusive maximum.  Needs ctx->scan_tmp reserved (reserve_scan).
 void reserve_scan(bm4dof_ctx *ctx, int64_t n)
@@ -635,13 +644,13 @@ void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n,
 		// one-wave blocks on the second stream, so that they get dispatched
 		// beside the one-wave blocks of the filter
 		if (op == 0) {
-			hipLaunchKernelGGL(scan_block_w_kernel<0>, dim3((unsigned)nb), dim3(64), 0, st, in, n_in, n, out, bsum);
+			hipLaunchKernelGGL(scan_block_w_kernel<0>, dim3(side_grid(ctx, nb)), dim3(64), 0, st, in, n_in, n, out, bsum);
 			hipLaunchKernelGGL(scan_top_w_kernel<0>, dim3(1), dim3(64), 0, st, bsum, nb);
-			hipLaunchKernelGGL(scan_add_kernel<0>, dim3(nblk(n, 64)), dim3(64), 0, st, out, n, bsum);
+			hipLaunchKernelGGL(scan_add_kernel<0>, dim3(side_grid(ctx, nblk(n, 64))), dim3(64), 0, st, out, n, bsum);
 		} else {
-			hipLaunchKernelGGL(scan_block_w_kernel<1>, dim3((unsigned)nb), dim3(64), 0, st, in, n_in, n, out, bsum);
+			hipLaunchKernelGGL(scan_block_w_kernel<1>, dim3(side_grid(ctx, nb)), dim3(64), 0, st, in, n_in, n, out, bsum);
 			hipLaunchKernelGGL(scan_top_w_kernel<1>, dim3(1), dim3(64), 0, st, bsum, nb);
-			hipLaunchKernelGGL(scan_add_kernel<1>, dim3(nblk(n, 64)), dim3(64), 0, st, out, n, bsum);
+			hipLaunchKernelGGL(scan_add_kernel<1>, dim3(side_grid(ctx, nblk(n, 64))), dim3(64), 0, st, out, n, bsum);
 		}
 	} else if (op == 0) {
 		hipLaunchKernelGGL(scan_block_kernel<0>, dim3((unsigned)nb), dim3(256), 0, st, in, n_in, n, out, bsum);
@@ -707,7 +716,8 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 	const BinGrid bg = bin_grid(sc);
 	const BinBufs q = bin_bufs(ctx, sc, Mub);
 	hipStream_t st = ctx->side;
-	hipLaunchKernelGGL(bin_pair_count_kernel, dim3(nblk(Mub, 64)), dim3(64), 0, st, epos, Mp, Mub, sc, q.pcnt);
+	hipLaunchKernelGGL(bin_pair_count_kernel, dim3(side_grid(ctx, nblk(Mub, 64))), dim3(64), 0, st, epos, Mp, Mub, sc,
+	                   q.pcnt);
 	dscan(ctx, 0, q.pcnt, Mub, Mub + 1, q.poff, true);
 	const uint32_t *np = q.poff + Mub;   // pair count
 	int bits = 1;
@@ -719,20 +729,21 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 		std::swap(kin, kout);
 		std::swap(vin, vout);
 	}
-	hipLaunchKernelGGL(bin_pair_emit_kernel, dim3(nblk(Mub, 64)), dim3(64), 0, st, epos, Mp, Mub, sc, bg.nbx,
+	hipLaunchKernelGGL(bin_pair_emit_kernel, dim3(side_grid(ctx, nblk(Mub, 64))), dim3(64), 0, st, epos, Mp, Mub, sc, bg.nbx,
 	                   bg.nby, q.poff, kin, vin);
 	HIPCHK(hipGetLastError());
 	for (int ps = 0; ps < passes; ps++) {
-		hipLaunchKernelGGL(rs_hist_kernel, dim3(q.nblk), dim3(RS_THREADS), 0, st, kin, np, 8 * ps, q.nblk, q.hist);
+		hipLaunchKernelGGL(rs_hist_kernel, dim3(side_grid(ctx, q.nblk)), dim3(RS_THREADS), 0, st, kin, np, 8 * ps,
+		                   q.nblk, q.hist);
 		dscan(ctx, 0, q.hist, (int64_t)256 * q.nblk, (int64_t)256 * q.nblk, q.hoff, true);
-		hipLaunchKernelGGL(rs_scatter_kernel, dim3(q.nblk), dim3(RS_THREADS), 0, st, kin, vin, np, 8 * ps, q.nblk,
+		hipLaunchKernelGGL(rs_scatter_kernel, dim3(side_grid(ctx, q.nblk)), dim3(RS_THREADS), 0, st, kin, vin, np, 8 * ps, q.nblk,
 		                   q.hoff, kout, vout);
 		HIPCHK(hipGetLastError());
 		std::swap(kin, kout);
 		std::swap(vin, vout);
 	}
 	const int64_t nub = 8 * Mub;
-	hipLaunchKernelGGL(bin_bounds_kernel, dim3((unsigned)std::min<int64_t>(nblk(nub + 1, 64), 16384)), dim3(64), 0, st,
+	hipLaunchKernelGGL(bin_bounds_kernel, dim3(side_grid(ctx, std::min<int64_t>(nblk(nub + 1, 64), 16384))), dim3(64), 0, st,
 	                   q.k0, np, nub, bg.nbins, q.boff);
 	HIPCHK(hipGetLastError());
 }
@@ -1128,7 +1139,7 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		// the side stream, beside the filter (only the aggregation reads them)
 		HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
 		HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-		hipLaunchKernelGGL(emit_entries_kernel, dim3(nblk(Mub, 64)), dim3(64), 0, ctx->side, d_idx, d_nsim,
+		hipLaunchKernelGGL(emit_entries_kernel, dim3(side_grid(ctx, nblk(Mub, 64))), dim3(64), 0, ctx->side, d_idx, d_nsim,
 		                   d_base, np, g0, Gc, d_epos);
 		HIPCHK(hipGetLastError());
 		hipLaunchKernelGGL(add_rows_kernel, dim3(1), dim3(64), 0, ctx->side, d_M, d_rows);
@@ -1848,6 +1859,7 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
 		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
 		else if (k == "agg_binned") ctx->agg_binned = value != 0;
+		else if (k == "side_blocks") ctx->side_blocks = (int)value;
 		else if (k == "fast_bm") {
 			if (value < 0 || value > 2) throw ArgError("fast_bm must be 0, 1 or 2");
 			ctx->fast_bm = (int)value;

@@ -2413,12 +2413,26 @@ __global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint
                                     const uint32_t *__restrict__ base, int npmax, int g0, int Gc,
                                     uint32_t *__restrict__ keys)
 {
-	const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (o >= (int64_t)Gc * npmax) return;
-	const int gl = (int)(o / npmax), n = (int)(o - (int64_t)gl * npmax);
-	const int g = g0 + gl;
-	if (n >= (int)nsim[g]) return;
-	keys[base[gl] + n] = idx[(size_t)g * npmax + n];
+	// four entries per thread and round, their loads issued together
+	const int64_t tot = (int64_t)Gc * npmax, gs = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t o0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o0 < tot; o0 += 4 * gs) {
+		uint32_t ns[4], bs[4], ix[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t o = o0 + u * gs;
+			const int gl = o < tot ? (int)(o / npmax) : 0, n = (int)(o - (int64_t)gl * npmax);
+			ns[u] = o < tot ? nsim[g0 + gl] : 0u;
+			bs[u] = o < tot ? base[gl] : 0u;
+			ix[u] = o < tot ? idx[(size_t)(g0 + gl) * npmax + n] : 0u;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t o = o0 + u * gs;
+			if (o >= tot) break;
+			const int n = (int)(o % npmax);
+			if (n < (int)ns[u]) keys[bs[u] + n] = ix[u];
+		}
+	}
 }
 
 // erec[e] = (position, tile, weight bits, 0): the row metadata the chunked
@@ -3178,15 +3192,28 @@ __device__ __forceinline__ void bin_span(uint32_t q, const StepConst &sc, int &b
 __global__ void bin_pair_count_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
                                       int64_t Mub, StepConst sc, uint32_t *__restrict__ cnt)
 {
-	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (e >= Mub) return;
-	uint32_t c = 0;
-	if (e < (int64_t)*Mp) {
-		int bx0, bx1, by0, by1, bt0, bt1;
-		bin_span(epos[e], sc, bx0, bx1, by0, by1, bt0, bt1);
-		c = (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1) * (bt1 - bt0 + 1));
+	// four rows per thread and round, their loads issued together
+	const int64_t M = (int64_t)*Mp, gs = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e0 < Mub; e0 += 4 * gs) {
+		uint32_t q[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t e = e0 + u * gs;
+			q[u] = e < M ? epos[e] : 0u;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t e = e0 + u * gs;
+			if (e >= Mub) break;
+			uint32_t c = 0;
+			if (e < M) {
+				int bx0, bx1, by0, by1, bt0, bt1;
+				bin_span(q[u], sc, bx0, bx1, by0, by1, bt0, bt1);
+				c = (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1) * (bt1 - bt0 + 1));
+			}
+			cnt[e] = c;
+		}
 	}
-	cnt[e] = c;
 }
 
 __global__ void bin_pair_emit_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
@@ -3194,18 +3221,32 @@ __global__ void bin_pair_emit_kernel(const uint32_t *__restrict__ epos, const ui
                                      const uint32_t *__restrict__ poff, uint32_t *__restrict__ keys,
                                      uint32_t *__restrict__ vals)
 {
-	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (e >= Mub || e >= (int64_t)*Mp) return;
-	int bx0, bx1, by0, by1, bt0, bt1;
-	bin_span(epos[e], sc, bx0, bx1, by0, by1, bt0, bt1);
-	uint32_t o = poff[e];
-	for (int bt = bt0; bt <= bt1; bt++)
-		for (int by = by0; by <= by1; by++)
-			for (int bx = bx0; bx <= bx1; bx++) {
-				keys[o] = (uint32_t)((bt * nby + by) * nbx + bx);
-				vals[o] = (uint32_t)e;
-				o++;
-			}
+	// four rows per thread and round, their loads issued together
+	const int64_t M = min<int64_t>(Mub, (int64_t)*Mp), gs = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e0 < M; e0 += 4 * gs) {
+		uint32_t q[4], o4[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t e = e0 + u * gs;
+			q[u] = e < M ? epos[e] : 0u;
+			o4[u] = e < M ? poff[e] : 0u;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t e = e0 + u * gs;
+			if (e >= M) break;
+			int bx0, bx1, by0, by1, bt0, bt1;
+			bin_span(q[u], sc, bx0, bx1, by0, by1, bt0, bt1);
+			uint32_t o = o4[u];
+			for (int bt = bt0; bt <= bt1; bt++)
+				for (int by = by0; by <= by1; by++)
+					for (int bx = bx0; bx <= bx1; bx++) {
+						keys[o] = (uint32_t)((bt * nby + by) * nbx + bx);
+						vals[o] = (uint32_t)e;
+						o++;
+					}
+		}
+	}
 }
 
 // boff[b] = number of sorted keys below b, b = 0 .. nbins (grid-stride over

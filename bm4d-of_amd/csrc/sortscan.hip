@@ -123,30 +123,33 @@ void scan_block_w_kernel(const uint32_t *__restrict__ in, int64_t n_in, int64_t 
                          uint32_t *__restrict__ bsum)
 {
 	constexpr int PER = SCAN_ITEMS / 64;
-	const int64_t b0 = (int64_t)blockIdx.x * SCAN_ITEMS + (int64_t)threadIdx.x * PER;
-	uint32_t v[PER];
-	uint32_t acc = 0;
+	const int64_t nb = (n + SCAN_ITEMS - 1) / SCAN_ITEMS;
+	for (int64_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+		const int64_t b0 = blk * SCAN_ITEMS + (int64_t)threadIdx.x * PER;
+		uint32_t v[PER];
+		uint32_t acc = 0;
 #pragma unroll
-	for (int i = 0; i < PER; i++) {
-		const int64_t j = b0 + i;
-		v[i] = j < n_in ? in[j] : 0u;
-		acc = scan_op<OP>(acc, v[i]);
-	}
-	const uint32_t incl = wave_incl_scan_u32<OP>(acc);
-	const uint32_t before = (uint32_t)__shfl_up((int)incl, 1, 64);
-	uint32_t run = OP == 0 ? incl - acc : (threadIdx.x ? before : 0u);
-#pragma unroll
-	for (int i = 0; i < PER; i++) {
-		const int64_t j = b0 + i;
-		if (OP == 0) {
-			if (j < n) out[j] = run;   // exclusive
-			run += v[i];
-		} else {
-			run = scan_op<OP>(run, v[i]);
-			if (j < n) out[j] = run;   // inclusive
+		for (int i = 0; i < PER; i++) {
+			const int64_t j = b0 + i;
+			v[i] = j < n_in ? in[j] : 0u;
+			acc = scan_op<OP>(acc, v[i]);
 		}
+		const uint32_t incl = wave_incl_scan_u32<OP>(acc);
+		const uint32_t before = (uint32_t)__shfl_up((int)incl, 1, 64);
+		uint32_t run = OP == 0 ? incl - acc : (threadIdx.x ? before : 0u);
+#pragma unroll
+		for (int i = 0; i < PER; i++) {
+			const int64_t j = b0 + i;
+			if (OP == 0) {
+				if (j < n) out[j] = run;   // exclusive
+				run += v[i];
+			} else {
+				run = scan_op<OP>(run, v[i]);
+				if (j < n) out[j] = run;   // inclusive
+			}
+		}
+		if (threadIdx.x == 63) bsum[blk] = incl;
 	}
-	if (threadIdx.x == 63) bsum[blockIdx.x] = incl;
 }
 
 template <int OP>
@@ -172,8 +175,21 @@ template __global__ void scan_top_w_kernel<1>(uint32_t *, int64_t);
 template <int OP>
 __global__ void scan_add_kernel(uint32_t *__restrict__ out, int64_t n, const uint32_t *__restrict__ bsum)
 {
-	const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (j < n) out[j] = scan_op<OP>(bsum[j / SCAN_ITEMS], out[j]);
+	const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t j0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j0 < n; j0 += 4 * gs) {
+		uint32_t a[4], o[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t j = j0 + u * gs;
+			a[u] = j < n ? bsum[j / SCAN_ITEMS] : 0u;
+			o[u] = j < n ? out[j] : 0u;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int64_t j = j0 + u * gs;
+			if (j < n) out[j] = scan_op<OP>(a[u], o[u]);
+		}
+	}
 }
 
 template __global__ void scan_block_kernel<0>(const uint32_t *, int64_t, int64_t, uint32_t *, uint32_t *);
@@ -304,21 +320,33 @@ void rs_hist_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restric
 {
 	constexpr int DPT = 256 / RS_THREADS;   // digits per thread
 	__shared__ uint32_t h[256];
-#pragma unroll
-	for (int i = 0; i < DPT; i++) h[threadIdx.x + i * RS_THREADS] = 0;
-	__syncthreads();
 	const uint64_t n = *np;
-	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
-	if (b0 < n)
-		for (int i = 0; i < RS_IPT; i++) {
-			const uint64_t k = b0 + (uint64_t)i * RS_THREADS + threadIdx.x;
-			if (k < n) atomicAdd(&h[(keys[k] >> shift) & 255u], 1u);
-		}
-	__syncthreads();
+	for (int tile = blockIdx.x; tile < nblk; tile += gridDim.x) {
 #pragma unroll
-	for (int i = 0; i < DPT; i++) {
-		const int d = threadIdx.x + i * RS_THREADS;
-		hist[(size_t)d * nblk + blockIdx.x] = h[d];
+		for (int i = 0; i < DPT; i++) h[threadIdx.x + i * RS_THREADS] = 0;
+		__syncthreads();
+		const uint64_t b0 = (uint64_t)tile * RS_TILE;
+		if (b0 < n)
+			for (int i0 = 0; i0 < RS_IPT; i0 += 16) {   // sixteen loads in flight, then their counts
+				uint32_t kv[16];
+#pragma unroll
+				for (int i = 0; i < 16; i++) {
+					const uint64_t k = b0 + (uint64_t)(i0 + i) * RS_THREADS + threadIdx.x;
+					kv[i] = k < n ? keys[k] : 0u;
+				}
+#pragma unroll
+				for (int i = 0; i < 16; i++) {
+					const uint64_t k = b0 + (uint64_t)(i0 + i) * RS_THREADS + threadIdx.x;
+					if (k < n) atomicAdd(&h[(kv[i] >> shift) & 255u], 1u);
+				}
+			}
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < DPT; i++) {
+			const int d = threadIdx.x + i * RS_THREADS;
+			hist[(size_t)d * nblk + tile] = h[d];
+		}
+		__syncthreads();
 	}
 }
 
@@ -335,42 +363,45 @@ void rs_scatter_kernel(const uint32_t *__restrict__ kin, const uint32_t *__restr
 	__shared__ uint32_t base[256];
 	const int lane = threadIdx.x;
 	const uint64_t n = *np;
-	const uint64_t b0 = (uint64_t)blockIdx.x * RS_TILE;
-	if (b0 >= n) return;
+	for (int tile = blockIdx.x; tile < nblk; tile += gridDim.x) {
+		const uint64_t b0 = (uint64_t)tile * RS_TILE;
+		if (b0 >= n) break;
 #pragma unroll
-	for (int i = 0; i < DPT; i++) base[lane + i * 64] = offs[(size_t)(lane + i * 64) * nblk + blockIdx.x];
-	__syncthreads();
-	const uint64_t lt = (1ull << lane) - 1ull;
-	for (int r0 = 0; r0 < RS_IPT; r0 += BATCH) {
-		if (b0 + (uint64_t)r0 * 64 >= n) break;
-		uint32_t key[BATCH], val[BATCH];
+		for (int i = 0; i < DPT; i++) base[lane + i * 64] = offs[(size_t)(lane + i * 64) * nblk + tile];
+		__syncthreads();
+		const uint64_t lt = (1ull << lane) - 1ull;
+		for (int r0 = 0; r0 < RS_IPT; r0 += BATCH) {
+			if (b0 + (uint64_t)r0 * 64 >= n) break;
+			uint32_t key[BATCH], val[BATCH];
 #pragma unroll
-		for (int r = 0; r < BATCH; r++) {
-			const uint64_t k = b0 + (uint64_t)(r0 + r) * 64 + lane;
-			key[r] = k < n ? kin[k] : 0u;
-			val[r] = k < n ? vin[k] : 0u;
-		}
-#pragma unroll
-		for (int r = 0; r < BATCH; r++) {
-			const bool valid = b0 + (uint64_t)(r0 + r) * 64 + lane < n;
-			const uint32_t d = valid ? (key[r] >> shift) & 255u : 0u;
-			uint64_t m = __ballot(valid);
-#pragma unroll
-			for (int bit = 0; bit < 8; bit++) {
-				const bool bs = (d >> bit) & 1u;
-				const uint64_t bb = __ballot(valid && bs);
-				m &= bs ? bb : ~bb;
+			for (int r = 0; r < BATCH; r++) {
+				const uint64_t k = b0 + (uint64_t)(r0 + r) * 64 + lane;
+				key[r] = k < n ? kin[k] : 0u;
+				val[r] = k < n ? vin[k] : 0u;
 			}
-			const uint32_t rk = (uint32_t)__popcll(m & lt);
-			const uint32_t bd = base[d];
-			__syncthreads();   // every lane has read its digit's offset
-			if (valid) {
-				kout[bd + rk] = key[r];
-				vout[bd + rk] = val[r];
-				if (rk == 0) base[d] = bd + (uint32_t)__popcll(m);
+#pragma unroll
+			for (int r = 0; r < BATCH; r++) {
+				const bool valid = b0 + (uint64_t)(r0 + r) * 64 + lane < n;
+				const uint32_t d = valid ? (key[r] >> shift) & 255u : 0u;
+				uint64_t m = __ballot(valid);
+#pragma unroll
+				for (int bit = 0; bit < 8; bit++) {
+					const bool bs = (d >> bit) & 1u;
+					const uint64_t bb = __ballot(valid && bs);
+					m &= bs ? bb : ~bb;
+				}
+				const uint32_t rk = (uint32_t)__popcll(m & lt);
+				const uint32_t bd = base[d];
+				__syncthreads();   // every lane has read its digit's offset
+				if (valid) {
+					kout[bd + rk] = key[r];
+					vout[bd + rk] = val[r];
+					if (rk == 0) base[d] = bd + (uint32_t)__popcll(m);
+				}
+				__syncthreads();
 			}
-			__syncthreads();
 		}
+		__syncthreads();   // the next tile rewrites base
 	}
 }
 

@@ -126,6 +126,10 @@ const char *bm4dof_global_error(void);
  *                  beside the filter, and one wave folds the list into the
  *                  bin's four frames; 0: the chunked aggregation sorts the
  *                  rows of every 8 x 8 x 1 block itself.
+ *   "side_blocks"  most blocks (one wave each) of a kernel that builds the
+ *                  bin lists on the second stream, beside the filter; the
+ *                  kernels loop over their work.  2048 (default); 0: one
+ *                  block per unit of work.
  *   "coef_budget"  bytes of device memory a step of the register filters may
  *                  hold in per-anchor coefficient tables (the forward
  *                  transform of every patch position, computed once per
