@@ -1,7 +1,7 @@
 """CPU float64 restatement of the learned gain network (test infrastructure only).
 
-The network has no counterpart in the reference; it is the north star's
-learned single pass: per group row, 66 features -> 128 -> 128 -> 64 gains,
+The network has no counterpart in the reference; it is the learned single
+pass that replaces the second step: per group row, 66 features -> 128 -> 128 -> 64 gains,
 ReLU hidden layers, sigmoid output, weights stored as bfloat16 with float32
 master copies, trained with Adam on the mean squared error against the
 step-2 Wiener gains y^2 / (y^2 + (beta sigma)^2) (src/core/nldct.cpp:2131)

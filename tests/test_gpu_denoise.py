@@ -212,6 +212,26 @@ def test_binned_and_chunked_aggregation_agree(ctx, w, h, f, c, nparts, px, pt):
     assert np.array_equal(b0, b1, equal_nan=True) and np.array_equal(f0, f1, equal_nan=True)
 
 
+@pytest.mark.parametrize("blocks", [1, 3, 64])
+def test_bin_lists_with_few_second_stream_blocks(ctx, blocks):
+    """The bin-list kernels loop over their work when the second stream's
+    grid is capped (option side_blocks): one block, three, 64 blocks give
+    bitwise the output of the default grid, colour and gray."""
+    for c in (1, 3):
+        clean, noisy = _video(88, 64, 9, seed=500 + blocks + c, channels=c)
+        ff, bf = _smooth_flow(9, 64, 88, 21), _smooth_flow(9, 64, 88, 22)
+        a1 = A.cli_params(1, 20, noisy.shape, px=4, pt=4, wx=9, np_=16)
+        a2 = A.cli_params(2, 20, noisy.shape, px=4, pt=4, wx=9, np_=16)
+        b0, f0, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+        try:
+            ctx.set_option("side_blocks", blocks)
+            b1, f1, _ = ctx.run(a1, a2, noisy, ff, bf, nparts=2)
+        finally:
+            ctx.set_option("side_blocks", 2048)
+        assert ctx.stats()["agg_bt"] == 4
+        assert np.array_equal(b0, b1, equal_nan=True) and np.array_equal(f0, f1, equal_nan=True)
+
+
 @pytest.mark.parametrize("px,pt,flow,nparts,wx,flat", [(4, 4, True, 4, 11, False), (8, 1, False, 2, 11, False),
                                                        (4, 4, False, 6, 11, False), (4, 4, True, 2, 11, True),
                                                        (8, 1, False, 1, 21, True), (8, 1, False, 2, 39, False),

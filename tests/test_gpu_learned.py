@@ -76,7 +76,7 @@ def test_adam_step_matches_oracle(ctx):
 @pytest.fixture(scope="module")
 def trained(ctx):
     """A network trained on pairs from the plain two-step method over four
-    small seeded videos at sigma 10..40 (the driver's own code path)."""
+    small seeded videos at sigma 10..40 (the training script's own code path)."""
     import torch
     L = _learned_module()
     xs, ts = [], []
