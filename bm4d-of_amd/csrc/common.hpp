@@ -49,6 +49,8 @@ struct AggRegion { int RX, RY, RT, tb0; };
 #define RS_THREADS 64
 #define RS_IPT 256
 #define RS_TILE (RS_THREADS * RS_IPT)
+// rows per tile of the first bin-sort pass (at most 8 pairs per row)
+#define BIN_RT 4096
 
 // Decode a 3-index position t*WH + y*W + x.
 __host__ __device__ inline void pos_decode(uint32_t pos, int W, int WH, int &x, int &y, int &t)

@@ -70,7 +70,7 @@ struct bm4dof_ctx {
 	    rank_of, etile, entry_w, est, early, gweight, counters, stale, elist, num, wsum,
 	    basis, binv, win, noisy_w, basic_w, out_w, io_noisy, io_fflow, io_bflow, io_basic,
 	    io_final, planes, planes_b, coef_n, coef_b, gacc, bsplit, ties, bm_image, bm_wins, bm_wmeta, erec, gflag, fmask, lp_idx, stat64, lastf, fidx, geo, scan_tmp,
-	    scan_tmp2, bin_cnt, bin_poff, bin_k0, bin_k1, bin_v1, bin_hist, bin_hoff, bin_off, bin_list;
+	    scan_tmp2, bin_cnt, bin_k0, bin_k1, bin_v1, bin_hist, bin_hoff, bin_off, bin_list;
 	// timings of the last run
 	std::vector<std::pair<const char *, float>> timings;
 	std::vector<hipEvent_t> ev;
@@ -122,7 +122,7 @@ struct bm4dof_ctx {
 		        &wsum, &basis, &binv, &win, &noisy_w, &basic_w, &out_w, &io_noisy, &io_fflow, &io_bflow,
 		        &io_basic, &io_final, &planes, &planes_b, &coef_n, &coef_b, &gacc, &bsplit, &ties, &bm_image,
 		        &bm_wins, &bm_wmeta, &erec, &gflag, &fmask, &lp_idx, &stat64, &lastf, &fidx, &geo, &scan_tmp,
-		        &scan_tmp2, &bin_cnt, &bin_poff, &bin_k0, &bin_k1, &bin_v1, &bin_hist, &bin_hoff, &bin_off,
+		        &scan_tmp2, &bin_cnt, &bin_k0, &bin_k1, &bin_v1, &bin_hist, &bin_hoff, &bin_off,
 		        &bin_list, &mlp_theta, &mlp_m, &mlp_v, &mlp_wq, &mlp_inf,
 		        &mlp_ws, &mlp_grad, &mlp_xb, &mlp_tb, &mlp_sse, &tv_frames, &tv_pairs, &tv_part, &tv_mm,
 		        &tv_state, &tv_work, &tv_pyr, &tv_taps};

@@ -96,9 +96,9 @@ template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
 	StepConst, int, int, float *, float *, uint32_t *
 __global__ void pack_rows_kernel(const uint32_t *, const uint32_t *, const float *, int64_t, uint4 *);
 template <int CC, int CAP, int WP> __global__ void agg_chunk_kernel(AGG2_ARGS);
-__global__ void bin_pair_count_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, uint32_t *);
-__global__ void bin_pair_emit_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, int, int,
-                                     const uint32_t *, uint32_t *, uint32_t *);
+__global__ void bin_hist1_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, int, int, int, uint32_t *);
+__global__ void bin_scatter1_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, int, int, int,
+                                    const uint32_t *, uint32_t *, uint32_t *);
 __global__ void bin_bounds_kernel(const uint32_t *, const uint32_t *, int64_t, int, uint32_t *);
 __global__ void rs_hist_kernel(const uint32_t *, const uint32_t *, int, int, uint32_t *);
 __global__ void rs_scatter_kernel(const uint32_t *, const uint32_t *, const uint32_t *, int, int, const uint32_t *,
@@ -684,12 +684,13 @@ BinGrid bin_grid(const StepConst &sc)
 	return g;
 }
 
-// Device buffers of the bin lists for Mub rows: per-row pair counts and
-// offsets, pairs (two key / value buffers), digit counts per radix tile,
-// list starts per bin.  The sorted values end in bin_v0.
+// Device buffers of the bin lists for Mub rows: the first pass's digit
+// offsets per row tile (and the pair count after them), pairs (two key /
+// value buffers, 8 Mub each), digit counts per radix tile, list starts per
+// bin.  The sorted values end in bin_list (v0).
 struct BinBufs {
-	uint32_t *pcnt, *poff, *k0, *v0, *k1, *v1, *hist, *hoff, *boff;
-	int nblk;
+	uint32_t *off1, *k0, *v0, *k1, *v1, *hist, *hoff, *boff;
+	int nt1, nblk;   // tiles of the first pass (rows) and of the later ones (pairs)
 };
 BinBufs bin_bufs(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub)
 {
@@ -697,8 +698,8 @@ BinBufs bin_bufs(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub)
 	const size_t npair = (size_t)8 * Mub;
 	BinBufs q;
 	q.nblk = (int)((npair + RS_TILE - 1) / RS_TILE);
-	q.pcnt = ctx->bin_cnt.as<uint32_t>((size_t)Mub + 1);
-	q.poff = ctx->bin_poff.as<uint32_t>((size_t)Mub + 1);
+	q.nt1 = (int)((Mub + BIN_RT - 1) / BIN_RT);
+	q.off1 = ctx->bin_cnt.as<uint32_t>((size_t)256 * q.nt1 + 1);
 	q.k0 = ctx->bin_k0.as<uint32_t>(npair);
 	q.v0 = ctx->bin_list.as<uint32_t>(npair);
 	q.k1 = ctx->bin_k1.as<uint32_t>(npair);
@@ -716,23 +717,23 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 	const BinGrid bg = bin_grid(sc);
 	const BinBufs q = bin_bufs(ctx, sc, Mub);
 	hipStream_t st = ctx->side;
-	hipLaunchKernelGGL(bin_pair_count_kernel, dim3(side_grid(ctx, nblk(Mub, 64))), dim3(64), 0, st, epos, Mp, Mub, sc,
-	                   q.pcnt);
-	dscan(ctx, 0, q.pcnt, Mub, Mub + 1, q.poff, true);
-	const uint32_t *np = q.poff + Mub;   // pair count
 	int bits = 1;
 	while ((1ll << bits) < (int64_t)bg.nbins) bits++;
 	const int passes = (bits + 7) / 8;
 	// the passes alternate between the buffers and end in bin_k0 / bin_list
-	uint32_t *kin = q.k0, *vin = q.v0, *kout = q.k1, *vout = q.v1;
-	if (passes % 2) {
-		std::swap(kin, kout);
-		std::swap(vin, vout);
-	}
-	hipLaunchKernelGGL(bin_pair_emit_kernel, dim3(side_grid(ctx, nblk(Mub, 64))), dim3(64), 0, st, epos, Mp, Mub, sc, bg.nbx,
-	                   bg.nby, q.poff, kin, vin);
+	uint32_t *kout = passes % 2 ? q.k0 : q.k1, *vout = passes % 2 ? q.v0 : q.v1;
+	// pass 0 from the rows; off1[256 nt1] = the pair count
+	hipLaunchKernelGGL(bin_hist1_kernel, dim3(side_grid(ctx, q.nt1)), dim3(64), 0, st, epos, Mp, Mub, sc, bg.nbx,
+	                   bg.nby, q.nt1, q.hist);
+	dscan(ctx, 0, q.hist, (int64_t)256 * q.nt1, (int64_t)256 * q.nt1 + 1, q.off1, true);
+	const uint32_t *np = q.off1 + (size_t)256 * q.nt1;
+	hipLaunchKernelGGL(bin_scatter1_kernel, dim3(side_grid(ctx, q.nt1)), dim3(64), 0, st, epos, Mp, Mub, sc,
+	                   bg.nbx, bg.nby, q.nt1, q.off1, kout, vout);
 	HIPCHK(hipGetLastError());
-	for (int ps = 0; ps < passes; ps++) {
+	uint32_t *kin = kout, *vin = vout;
+	kout = kin == q.k0 ? q.k1 : q.k0;
+	vout = vin == q.v0 ? q.v1 : q.v0;
+	for (int ps = 1; ps < passes; ps++) {
 		hipLaunchKernelGGL(rs_hist_kernel, dim3(side_grid(ctx, q.nblk)), dim3(RS_THREADS), 0, st, kin, np, 8 * ps,
 		                   q.nblk, q.hist);
 		dscan(ctx, 0, q.hist, (int64_t)256 * q.nblk, (int64_t)256 * q.nblk, q.hoff, true);
@@ -1031,7 +1032,12 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 			ctx->coef_b.release();
 		}
 	}
-	reserve_scan(ctx, (int64_t)std::max(pl.WHF + 1, G + 1));
+	int64_t nscan = std::max(pl.WHF + 1, G + 1);
+	if (agg_binned_ok(ctx, pl.sc, (int64_t)Mub)) {
+		const BinBufs q = bin_bufs(ctx, pl.sc, (int64_t)Mub);   // the bin sort's scans
+		nscan = std::max<int64_t>(nscan, (int64_t)256 * std::max(q.nt1, q.nblk) + 1);
+	}
+	reserve_scan(ctx, nscan);
 }
 
 // Launches one step on ctx->stream.  With `tm` it records stage events; a

@@ -3161,11 +3161,12 @@ __global__ void band_bounds_kernel(const uint32_t *__restrict__ base, const uint
 // ---------------------------------------------------------------------------
 // Ordered aggregation by bins of 8 x 8 pixels x AGG_TB frames.
 //
-// bin_pair_count_kernel / bin_pair_emit_kernel (side stream, beside the
-// filter): every row e emits one (bin, e) pair per bin its patch overlaps,
-// in increasing e; a stable radix sort by bin (sortscan.hip) then leaves
-// each bin's rows in increasing e, and bin_bounds_kernel marks where each
-// bin's list starts.  agg_bin_fold_kernel: one wave per bin, lane = pixel
+// bin_hist1_kernel / bin_scatter1_kernel (side stream, beside the filter):
+// every row e gives one (bin, e) pair per bin its patch overlaps, in
+// increasing e, sorted stably by the bin's low 8 bits straight from the
+// rows; the further 8-bit passes (sortscan.hip) then leave each bin's rows
+// in increasing e, and bin_bounds_kernel marks where each bin's list
+// starts.  agg_bin_fold_kernel: one wave per bin, lane = pixel
 // of the 8 x 8 block, AGG_TB running sums per lane (one per frame of the
 // bin).  The bin's list is read in chunks of NBR rows; each row's slices at
 // the bin's frames are staged once in LDS and added, frame by frame, into
@@ -3189,65 +3190,114 @@ __device__ __forceinline__ void bin_span(uint32_t q, const StepConst &sc, int &b
 	bt1 = (qt + sc.sPt - 1) / AGG_TB;
 }
 
-__global__ void bin_pair_count_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
-                                      int64_t Mub, StepConst sc, uint32_t *__restrict__ cnt)
+// bin key of the pairs of a row, in the order they are listed: frame bins,
+// then y, then x
+#define BIN_FOR_PAIRS(q, KEY, BODY)                                                                  \
+	{                                                                                                \
+		int bx0_, bx1_, by0_, by1_, bt0_, bt1_;                                                      \
+		bin_span(q, sc, bx0_, bx1_, by0_, by1_, bt0_, bt1_);                                         \
+		for (int bt_ = bt0_; bt_ <= bt1_; bt_++)                                                     \
+			for (int by_ = by0_; by_ <= by1_; by_++)                                                 \
+				for (int bx_ = bx0_; bx_ <= bx1_; bx_++) {                                           \
+					const uint32_t KEY = (uint32_t)((bt_ * nby + by_) * nbx + bx_);                  \
+					BODY;                                                                            \
+				}                                                                                    \
+	}
+
+// First radix pass straight from the rows (no pair list in memory): tiles
+// of BIN_RT rows, one wave each (grid-stride over the tiles).
+// bin_hist1_kernel: the low-digit counts of each tile's pairs, digit-major
+// (hist[d * ntiles + tile]).
+__global__ __launch_bounds__(64)
+void bin_hist1_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp, int64_t Mub,
+                      StepConst sc, int nbx, int nby, int ntiles, uint32_t *__restrict__ hist)
 {
-	// four rows per thread and round, their loads issued together
-	const int64_t M = (int64_t)*Mp, gs = (int64_t)gridDim.x * blockDim.x;
-	for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e0 < Mub; e0 += 4 * gs) {
-		uint32_t q[4];
+	__shared__ uint32_t h[256];
+	const int lane = threadIdx.x;
+	const int64_t M = min<int64_t>(Mub, (int64_t)*Mp);
+	for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
-			const int64_t e = e0 + u * gs;
-			q[u] = e < M ? epos[e] : 0u;
-		}
+		for (int i = 0; i < 4; i++) h[lane + 64 * i] = 0;
+		__syncthreads();
+		const int64_t e0 = (int64_t)tile * BIN_RT;
+		for (int r0 = 0; r0 < BIN_RT && e0 + r0 < M; r0 += 256) {   // four rows per lane, loads together
+			uint32_t q[4];
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
-			const int64_t e = e0 + u * gs;
-			if (e >= Mub) break;
-			uint32_t c = 0;
-			if (e < M) {
-				int bx0, bx1, by0, by1, bt0, bt1;
-				bin_span(q[u], sc, bx0, bx1, by0, by1, bt0, bt1);
-				c = (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1) * (bt1 - bt0 + 1));
+			for (int u = 0; u < 4; u++) {
+				const int64_t e = e0 + r0 + 64 * u + lane;
+				q[u] = e < M ? epos[e] : 0u;
 			}
-			cnt[e] = c;
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				if (e0 + r0 + 64 * u + lane < M) BIN_FOR_PAIRS(q[u], key, atomicAdd(&h[key & 255u], 1u));
 		}
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < 4; i++) hist[(size_t)(lane + 64 * i) * ntiles + tile] = h[lane + 64 * i];
+		__syncthreads();
 	}
 }
 
-__global__ void bin_pair_emit_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp,
-                                     int64_t Mub, StepConst sc, int nbx, int nby,
-                                     const uint32_t *__restrict__ poff, uint32_t *__restrict__ keys,
-                                     uint32_t *__restrict__ vals)
+// bin_scatter1_kernel: each tile's pairs to the slots the scan of the counts
+// gives, in row order: a round's 64 rows list their pairs in LDS in (row,
+// bin) order, then rounds of 64 pairs take consecutive slots per digit (the
+// lanes of a round with the same digit by rank, the lowest advancing the
+// digit's offset), as rs_scatter_kernel does for the later passes.
+__global__ __launch_bounds__(64)
+void bin_scatter1_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ Mp, int64_t Mub,
+                         StepConst sc, int nbx, int nby, int ntiles, const uint32_t *__restrict__ offs,
+                         uint32_t *__restrict__ kout, uint32_t *__restrict__ vout)
 {
-	// four rows per thread and round, their loads issued together
-	const int64_t M = min<int64_t>(Mub, (int64_t)*Mp), gs = (int64_t)gridDim.x * blockDim.x;
-	for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e0 < M; e0 += 4 * gs) {
-		uint32_t q[4], o4[4];
+	__shared__ uint32_t base[256];
+	__shared__ uint32_t pk[8 * 64], pe[8 * 64];
+	const int lane = threadIdx.x;
+	const uint64_t lt = (1ull << lane) - 1ull;
+	const int64_t M = min<int64_t>(Mub, (int64_t)*Mp);
+	for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		const int64_t e0 = (int64_t)tile * BIN_RT;
+		if (e0 >= M) break;
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
-			const int64_t e = e0 + u * gs;
-			q[u] = e < M ? epos[e] : 0u;
-			o4[u] = e < M ? poff[e] : 0u;
-		}
+		for (int i = 0; i < 4; i++) base[lane + 64 * i] = offs[(size_t)(lane + 64 * i) * ntiles + tile];
+		uint32_t qn = e0 + lane < M ? epos[e0 + lane] : 0u;   // the next round's position
+		__syncthreads();
+		for (int r0 = 0; r0 < BIN_RT && e0 + r0 < M; r0 += 64) {
+			const int64_t e = e0 + r0 + lane;
+			const uint32_t q = qn;
+			qn = r0 + 64 < BIN_RT && e + 64 < M ? epos[e + 64] : 0u;
+			int c = 0;
+			if (e < M) BIN_FOR_PAIRS(q, key, (void)key; c++);
+			const int incl = wave_incl_scan(c);
+			const int T = __shfl(incl, 63, 64);
+			int o = incl - c;
+			if (e < M) BIN_FOR_PAIRS(q, key, pk[o] = key; pe[o] = (uint32_t)e; o++);
+			__syncthreads();
+			for (int p0 = 0; p0 < T; p0 += 64) {
+				const int i = p0 + lane;
+				const bool valid = i < T;
+				const uint32_t key = valid ? pk[i] : 0u, val = valid ? pe[i] : 0u;
+				const uint32_t d = key & 255u;
+				uint64_t m = __ballot(valid);
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
-			const int64_t e = e0 + u * gs;
-			if (e >= M) break;
-			int bx0, bx1, by0, by1, bt0, bt1;
-			bin_span(q[u], sc, bx0, bx1, by0, by1, bt0, bt1);
-			uint32_t o = o4[u];
-			for (int bt = bt0; bt <= bt1; bt++)
-				for (int by = by0; by <= by1; by++)
-					for (int bx = bx0; bx <= bx1; bx++) {
-						keys[o] = (uint32_t)((bt * nby + by) * nbx + bx);
-						vals[o] = (uint32_t)e;
-						o++;
-					}
+				for (int bit = 0; bit < 8; bit++) {
+					const bool bs = (d >> bit) & 1u;
+					const uint64_t bb = __ballot(valid && bs);
+					m &= bs ? bb : ~bb;
+				}
+				const uint32_t rk = (uint32_t)__popcll(m & lt);
+				const uint32_t bd = base[d];
+				__syncthreads();   // every lane has read its digit's offset
+				if (valid) {
+					kout[bd + rk] = key;
+					vout[bd + rk] = val;
+					if (rk == 0) base[d] = bd + (uint32_t)__popcll(m);
+				}
+				__syncthreads();
+			}
 		}
+		__syncthreads();   // the next tile rewrites base
 	}
 }
+#undef BIN_FOR_PAIRS
 
 // boff[b] = number of sorted keys below b, b = 0 .. nbins (grid-stride over
 // the pair index i <= n: i writes the starts of the bins from the key before
