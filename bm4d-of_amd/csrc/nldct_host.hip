@@ -744,7 +744,7 @@ void agg_bin_lists(bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub, const uint
 		std::swap(vin, vout);
 	}
 	const int64_t nub = 8 * Mub;
-	hipLaunchKernelGGL(bin_bounds_kernel, dim3(side_grid(ctx, std::min<int64_t>(nblk(nub + 1, 64), 16384))), dim3(64), 0, st,
+	hipLaunchKernelGGL(bin_bounds_kernel, dim3(side_grid(ctx, (nub + 1 + 1023) / 1024)), dim3(64), 0, st,
 	                   q.k0, np, nub, bg.nbins, q.boff);
 	HIPCHK(hipGetLastError());
 }

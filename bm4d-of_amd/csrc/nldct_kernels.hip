@@ -3299,17 +3299,42 @@ void bin_scatter1_kernel(const uint32_t *__restrict__ epos, const uint32_t *__re
 }
 #undef BIN_FOR_PAIRS
 
-// boff[b] = number of sorted keys below b, b = 0 .. nbins (grid-stride over
-// the pair index i <= n: i writes the starts of the bins from the key before
-// it, exclusive, to its own key, inclusive)
-__global__ void bin_bounds_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ np, int64_t nub,
-                                  int nbins, uint32_t *__restrict__ boff)
+// boff[b] = number of sorted keys below b, b = 0 .. nbins: pair index
+// i <= n writes the starts of the bins from the key before it, exclusive, to
+// its own key, inclusive.  One wave per segment of 1024 indices, 16
+// consecutive ones per lane (grid-stride over the segments).
+__global__ __launch_bounds__(64)
+void bin_bounds_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ np, int64_t nub,
+                       int nbins, uint32_t *__restrict__ boff)
 {
 	const int64_t n = min<int64_t>((int64_t)*np, nub);
-	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
-		const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1u;
-		const uint32_t hi = i == n ? (uint32_t)nbins : keys[i];
-		for (uint32_t b = lo; b <= hi; b++) boff[b] = (uint32_t)i;
+	const int64_t nseg = (n + 1 + 1023) / 1024;
+	for (int64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+		const int64_t i0 = sg * 1024 + (int64_t)threadIdx.x * 16;
+		uint32_t k[17];   // keys[i0 - 1 .. i0 + 15]
+		k[0] = i0 > 0 && i0 - 1 < n ? keys[i0 - 1] : 0u;
+		if (i0 + 16 <= n) {
+			const uint4 *p = reinterpret_cast<const uint4 *>(keys + i0);
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const uint4 v = p[q];
+				k[4 * q + 1] = v.x;
+				k[4 * q + 2] = v.y;
+				k[4 * q + 3] = v.z;
+				k[4 * q + 4] = v.w;
+			}
+		} else {
+#pragma unroll
+			for (int j = 0; j < 16; j++) k[j + 1] = i0 + j < n ? keys[i0 + j] : 0u;
+		}
+#pragma unroll
+		for (int j = 0; j < 16; j++) {
+			const int64_t i = i0 + j;
+			if (i > n) break;
+			const uint32_t lo = i == 0 ? 0u : k[j] + 1u;
+			const uint32_t hi = i == n ? (uint32_t)nbins : k[j + 1];
+			for (uint32_t b = lo; b <= hi; b++) boff[b] = (uint32_t)i;
+		}
 	}
 }
 
