@@ -64,6 +64,11 @@ struct bm4dof_ctx {
 	// rows, which only the aggregation needs: it runs beside the filter
 	hipStream_t side = nullptr;
 	hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+	// coefficient tables of the current gray step launched on the second
+	// stream beside its block matching (ev_coef marks their end)
+	hipEvent_t ev_coef = nullptr;
+	bool coef_on_side = false;
+	const float *coef_side_n = nullptr, *coef_side_b = nullptr;
 	std::string err;
 	// step buffers
 	bm4dof::DevBuf gpos, gtile, tiles, idx, nsim, base, keys, vals, skeys, svals, sort_tmp, lstart, lend,
@@ -98,6 +103,7 @@ struct bm4dof_ctx {
 	bool agg_chunked = true;   // chunked (mask transpose) ordered aggregation where tiles allow it
 	int agg_cap = 1024;        // rows per e-range of the chunked aggregation (one channel)
 	bool agg_binned = true;
+	bool coef_beside_bm = true;   // gray steps: coefficient tables on the second stream beside block matching
 	int side_blocks = 2048;    // most blocks of a second-stream kernel (0: no limit)    // binned aggregation (row lists per 8 x 8 x 4 bin) where the shape allows it
 	bool vbm3d_search = false; // block matching of a VBM3D_SEARCH build (predictive search)
 	// learned gain network (mlp.hip)

@@ -1040,6 +1040,45 @@ void reserve_step(bm4dof_ctx *ctx, const StepPlan &pl)
 	reserve_scan(ctx, nscan);
 }
 
+// The per-anchor coefficient tables of a step on stream st (the ones
+// reserve_step allocated; nothing allocates here): the noisy planes' table,
+// and in step 2 the basic planes' one, unless the tables do not fit
+// coef_budget (then both pointers stay null).
+static void coef_tables(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const float *d_nplanes,
+                        const float *d_bplanes, hipStream_t st, const float *&d_coef_n, const float *&d_coef_b)
+{
+	const StepConst &sc = pl.sc;
+	const bool step1 = pl.step1;
+	const int C = pl.C;
+	const size_t WHF = pl.WHF;
+	const int nimg = step1 ? 1 : 2;
+	const size_t per = coef_floats(pl, ctx->coef_budget);
+	const int64_t waves = coef_waves(sc);
+	d_coef_n = d_coef_b = nullptr;
+	if (per > 0 && ctx->coef_n.n >= per * sizeof(float) && (step1 || ctx->coef_b.n >= per * sizeof(float))) {
+		const bool have_n = !step1 && ctx->coef_n_keep && ctx->coef_n_ok && ctx->coef_n_whf == WHF &&
+		                    ctx->coef_n_shape[0] == sc.sPx && ctx->coef_n_shape[1] == sc.sPt &&
+		                    ctx->coef_n_shape[2] == C && ctx->coef_n_basis == pl.hb;
+		ctx->coef_n_ok = false;
+		for (int i = 0; i < nimg; i++) {
+			float *tab = (float *)(i ? ctx->coef_b : ctx->coef_n).p;
+			const float *src = i ? d_bplanes : d_nplanes;
+			(i ? d_coef_b : d_coef_n) = tab;
+			if (i == 0 && have_n) continue;   // the first step's table of the same planes
+			for (int c = 0; c < C; c++)
+				hipLaunchKernelGGL(sc.sPx == 4 ? coef_kernel<4> : coef_kernel<8>, dim3((unsigned)waves), dim3(64), 0,
+				                   st, src + (size_t)c * WHF, sc.W, sc.H, sc.F, dv.basis, tab + (size_t)c * WHF * sc.P);
+		}
+		HIPCHK(hipGetLastError());
+		ctx->coef_n_ok = true;
+		ctx->coef_n_whf = WHF;
+		ctx->coef_n_shape[0] = sc.sPx;
+		ctx->coef_n_shape[1] = sc.sPt;
+		ctx->coef_n_shape[2] = C;
+		ctx->coef_n_basis = pl.hb;
+	}
+}
+
 // Launches one step on ctx->stream.  With `tm` it records stage events; a
 // captured enqueue passes none.  Needs reserve_step(plan) first.
 // parts: 1 set-up and block matching, 2 the rest (filter, aggregation,
@@ -1070,6 +1109,16 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 		fill_f32(ctx, d_num, 0.f, (int64_t)N_all);
 		fill_f32(ctx, d_wsum, 0.f, (int64_t)WHF);
 		mark("l_setup", "s1_setup", "s2_setup");
+		// a gray step's coefficient tables depend on the images only: on the
+		// second stream beside the block matching
+		ctx->coef_on_side = false;
+		if (pl.fast && !la && C == 1 && ctx->coef_beside_bm) {
+			HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
+			HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+			coef_tables(ctx, pl, dv, noisy_w, basic_w, ctx->side, ctx->coef_side_n, ctx->coef_side_b);
+			HIPCHK(hipEventRecord(ctx->ev_coef, ctx->side));
+			ctx->coef_on_side = true;
+		}
 		run_bm(ctx, sc, step1 ? 1 : 2, step1 ? noisy_w : basic_w, ff, bf, dv.gpos, dv.gtile, dv.tiles, G, d_idx,
 		       d_nsim);
 		mark("l_bm", "s1_bm", "s2_bm");
@@ -1096,36 +1145,17 @@ void enqueue_step(bm4dof_ctx *ctx, const StepPlan &pl, const StepDev &dv, const 
 	// forward transforms of every patch anchor, once per image, when the
 	// tables fit coef_budget: the register filters then read a group row's
 	// coefficients instead of transforming the patch again in every group
-	// that holds it (bit for bit the same sums)
+	// that holds it (bit for bit the same sums); a gray step launched them
+	// beside its block matching
 	const float *d_coef_n = nullptr, *d_coef_b = nullptr;
 	if (pl.fast && !la) {
-		// the tables reserve_step allocated (nothing allocates here)
-		const int nimg = step1 ? 1 : 2;
-		const size_t per = coef_floats(pl, ctx->coef_budget);
-		const int64_t waves = coef_waves(sc);
-		if (per > 0 && ctx->coef_n.n >= per * sizeof(float) && (step1 || ctx->coef_b.n >= per * sizeof(float))) {
-			const bool have_n = !step1 && ctx->coef_n_keep && ctx->coef_n_ok && ctx->coef_n_whf == WHF &&
-			                    ctx->coef_n_shape[0] == sc.sPx && ctx->coef_n_shape[1] == sc.sPt &&
-			                    ctx->coef_n_shape[2] == C && ctx->coef_n_basis == pl.hb;
-			ctx->coef_n_ok = false;
-			for (int i = 0; i < nimg; i++) {
-				float *tab = (float *)(i ? ctx->coef_b : ctx->coef_n).p;
-				const float *src = i ? d_bplanes : d_nplanes;
-				(i ? d_coef_b : d_coef_n) = tab;
-				if (i == 0 && have_n) continue;   // the first step's table of the same planes
-				for (int c = 0; c < C; c++)
-					hipLaunchKernelGGL(sc.sPx == 4 ? coef_kernel<4> : coef_kernel<8>, dim3((unsigned)waves), dim3(64),
-					                   0, ctx->stream, src + (size_t)c * WHF, sc.W, sc.H, sc.F, dv.basis,
-					                   tab + (size_t)c * WHF * sc.P);
-			}
-			HIPCHK(hipGetLastError());
-			ctx->coef_n_ok = true;
-			ctx->coef_n_whf = WHF;
-			ctx->coef_n_shape[0] = sc.sPx;
-			ctx->coef_n_shape[1] = sc.sPt;
-			ctx->coef_n_shape[2] = C;
-			ctx->coef_n_basis = pl.hb;
-		}
+		if (ctx->coef_on_side) {
+			HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_coef, 0));
+			d_coef_n = ctx->coef_side_n;
+			d_coef_b = ctx->coef_side_b;
+			ctx->coef_on_side = false;
+		} else
+			coef_tables(ctx, pl, dv, d_nplanes, d_bplanes, ctx->stream, d_coef_n, d_coef_b);
 		mark("l_coef", "s1_coef", "s2_coef");
 	}
 
@@ -1821,6 +1851,7 @@ int bm4dof_ctx_create(bm4dof_ctx **out, int device)
 		HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+		HIPCHK(hipEventCreateWithFlags(&ctx->ev_coef, hipEventDisableTiming));
 		tv_scratch_init(device);
 		// coefficient tables: at most 2/5 of the device memory free now
 		size_t free_b = 0, total_b = 0;
@@ -1848,6 +1879,7 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	for (DevBuf *b : ctx->buffers()) b->release();
 	if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
 	if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+	if (ctx->ev_coef) (void)hipEventDestroy(ctx->ev_coef);
 	if (ctx->side) (void)hipStreamDestroy(ctx->side);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
@@ -1865,6 +1897,7 @@ int bm4dof_ctx_set_option(bm4dof_ctx *ctx, const char *key, long long value)
 		if (k == "fast_filter") ctx->fast_filter = value != 0;
 		else if (k == "agg_chunked") ctx->agg_chunked = value != 0;
 		else if (k == "agg_binned") ctx->agg_binned = value != 0;
+		else if (k == "coef_beside_bm") ctx->coef_beside_bm = value != 0;
 		else if (k == "side_blocks") ctx->side_blocks = (int)value;
 		else if (k == "fast_bm") {
 			if (value < 0 || value > 2) throw ArgError("fast_bm must be 0, 1 or 2");

@@ -126,6 +126,9 @@ const char *bm4dof_global_error(void);
  *                  beside the filter, and one wave folds the list into the
  *                  bin's four frames; 0: the chunked aggregation sorts the
  *                  rows of every 8 x 8 x 1 block itself.
+ *   "coef_beside_bm" 1 (default): a gray step's coefficient tables are
+ *                  computed on the second stream beside its block matching;
+ *                  0: after it, on the main stream.
  *   "side_blocks"  most blocks (one wave each) of a kernel that builds the
  *                  bin lists on the second stream, beside the filter; the
  *                  kernels loop over their work.  2048 (default); 0: one
