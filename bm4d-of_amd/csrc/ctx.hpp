@@ -67,6 +67,12 @@ struct bm4dof_ctx {
 	// coefficient tables of the current gray step launched on the second
 	// stream beside its block matching (ev_coef marks their end)
 	hipEvent_t ev_coef = nullptr;
+	// TV-L1: pinned host words the solve reads its active counts and
+	// iteration counts into while the GPU runs on (1 + 2 tv_pin_pairs ints,
+	// allocated at context creation), and the event of a poll
+	int *tv_pin = nullptr;
+	int tv_pin_pairs = 0;
+	hipEvent_t ev_tvpoll = nullptr;
 	bool coef_on_side = false;
 	const float *coef_side_n = nullptr, *coef_side_b = nullptr;
 	std::string err;

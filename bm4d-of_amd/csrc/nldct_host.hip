@@ -1852,6 +1852,15 @@ int bm4dof_ctx_create(bm4dof_ctx **out, int device)
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
 		HIPCHK(hipEventCreateWithFlags(&ctx->ev_coef, hipEventDisableTiming));
+		HIPCHK(hipEventCreateWithFlags(&ctx->ev_tvpoll, hipEventDisableTiming));
+		// TV-L1 polls (no allocation in the solve itself, which the
+		// allocation-free flow entry point shares); without it the solve
+		// waits for the stream at each poll
+		if (hipHostMalloc((void **)&ctx->tv_pin, (1 + 2 * (size_t)8192) * sizeof(int), hipHostMallocDefault) ==
+		    hipSuccess)
+			ctx->tv_pin_pairs = 8192;
+		else
+			ctx->tv_pin = nullptr;
 		tv_scratch_init(device);
 		// coefficient tables: at most 2/5 of the device memory free now
 		size_t free_b = 0, total_b = 0;
@@ -1880,6 +1889,8 @@ void bm4dof_ctx_destroy(bm4dof_ctx *ctx)
 	if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
 	if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
 	if (ctx->ev_coef) (void)hipEventDestroy(ctx->ev_coef);
+	if (ctx->ev_tvpoll) (void)hipEventDestroy(ctx->ev_tvpoll);
+	if (ctx->tv_pin) (void)hipHostFree(ctx->tv_pin);
 	if (ctx->side) (void)hipStreamDestroy(ctx->side);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;

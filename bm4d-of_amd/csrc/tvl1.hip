@@ -892,8 +892,32 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	// host reads the active count after a batch of rounds: the first batch
 	// long enough for the previous warp's iteration count, then two rounds
 	// at a time (a round of a finished pair is a launch whose blocks exit).
+	// With the pinned words, a poll copies the count and the GPU runs one
+	// more round while the host waits for the copy, and a warp's iteration
+	// counts are copied at its end and read at the next poll.
 	const int max_rounds = (TV_MAX_ITERATIONS + TV_K - 1) / TV_K + 1;
-	int guess = 0;   // most iterations of a pair in the previous warp
+	int guess = 0;   // most iterations of a pair in the last warp read back
+	const bool piped = ctx->tv_pin && P <= ctx->tv_pin_pairs;
+	int pend_s = -1, pend_w = -1, pend_slot = 0, nwarp = 0;
+	size_t pend_n = 0;
+	auto take_counts = [&](const int *hs, int s, int w, size_t n) {
+		guess = 0;
+		for (int pr = 0; pr < P; pr++) {
+			ctx->tv_pixel_iters += (uint64_t)std::max(hs[pr], 0) * (uint64_t)n;
+			guess = std::max(guess, hs[pr]);
+		}
+		if (iters)
+			for (int pr = 0; pr < P; pr++) iters[((size_t)pr * nscales + s) * prm.warps + w] = hs[pr];
+		if (prm.verbose)
+			for (int pr = 0; pr < P; pr++)
+				fprintf(stderr, "Pair %d scale %d warping %d: iterations %d\n", pr, s, w, hs[pr]);
+	};
+	// the counts of a warp whose copy a completed poll has passed
+	auto take_pending = [&]() {
+		if (pend_w < 0) return;
+		take_counts(ctx->tv_pin + 1 + (size_t)pend_slot * P, pend_s, pend_w, pend_n);
+		pend_w = -1;
+	};
 
 	for (int s = nscales - 1; s >= prm.fscale; s--) {
 		// Dual_TVL1_optic_flow, tvl1flow_lib.c:93-275
@@ -905,11 +929,23 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 		const TVPlanes SB = {tmp, tmp2, work + 12 * W, work + 13 * W, work + 14 * W, work + 15 * W};
 		const bool small = tv_small(nx, ny, P);
 		const int tiles_x = tv_tiles_x(nx, ny, P), ntiles = tv_tiles_at(nx, ny, P);
+		auto launch_round = [&]() {
+			if (small)
+				hipLaunchKernelGGL((tv_iter_kernel<TV_STX, TV_STY, TV_K, TV_SNT>), dim3(ntiles, P), dim3(TV_SNT), 0, st,
+				                   I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut, d_ctl,
+				                   d_cur, d_part);
+			else
+				hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0, st,
+				                   I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut, d_ctl,
+				                   d_cur, d_part);
+			hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n, eps2,
+			                   d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
+		};
 		hipLaunchKernelGGL(tv_cgrad_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1, nx, ny, n, I1x, I1y);
 		const size_t nP = (size_t)P * n;
 		for (float *q : {p11, p12, p21, p22})
 			hipLaunchKernelGGL(tv_fill_kernel, dim3(tv_grid(nP)), dim3(TV_THREADS), 0, st, q, nP, 0.f);
-		for (int w = 0; w < prm.warps; w++) {
+		for (int w = 0; w < prm.warps; w++, nwarp++) {
 			hipLaunchKernelGGL(tv_round_init_kernel, dim3(nblk_tv(P)), dim3(TV_THREADS), 0, st, P, TV_K, d_ctl,
 			                   d_cur, d_iters, d_mode, d_state, d_nact, w == 0 ? 1 : 0);
 			hipLaunchKernelGGL(tv_warp_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I0, I1, I1x, I1y, SA.u1, SA.u2,
@@ -918,38 +954,40 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 			int batch = guess > 0 ? (guess + TV_K - 1) / TV_K : 3;
 			for (int r = 0; r < max_rounds;) {
 				const int rb = std::min(batch, max_rounds - r);
-				for (int i = 0; i < rb; i++) {
-					if (small)
-						hipLaunchKernelGGL((tv_iter_kernel<TV_STX, TV_STY, TV_K, TV_SNT>), dim3(ntiles, P), dim3(TV_SNT),
-						                   0, st, I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta,
-						                   taut, d_ctl, d_cur, d_part);
-					else
-						hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0,
-						                   st, I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut,
-						                   d_ctl, d_cur, d_part);
-					hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n,
-					                   eps2, d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
-				}
+				for (int i = 0; i < rb; i++) launch_round();
 				r += rb;
 				HIPCHK(hipGetLastError());
 				int nact = 0;
-				HIPCHK(hipMemcpyAsync(&nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
-				HIPCHK(hipStreamSynchronize(st));
+				if (piped) {
+					HIPCHK(hipMemcpyAsync(ctx->tv_pin, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
+					HIPCHK(hipEventRecord(ctx->ev_tvpoll, st));
+					if (r < max_rounds) {   // runs while the host waits; a no-op once every pair stopped
+						launch_round();
+						r++;
+					}
+					HIPCHK(hipGetLastError());
+					HIPCHK(hipEventSynchronize(ctx->ev_tvpoll));
+					nact = ctx->tv_pin[0];
+					take_pending();
+				} else {
+					HIPCHK(hipMemcpyAsync(&nact, d_nact, sizeof(int), hipMemcpyDeviceToHost, st));
+					HIPCHK(hipStreamSynchronize(st));
+				}
 				if (nact == 0) break;
 				batch = 2;
 			}
-			HIPCHK(hipMemcpyAsync(h_state.data(), d_state, P * sizeof(int), hipMemcpyDeviceToHost, st));
-			HIPCHK(hipStreamSynchronize(st));
-			guess = 0;
-			for (int pr = 0; pr < P; pr++) {
-				ctx->tv_pixel_iters += (uint64_t)std::max(h_state[pr], 0) * (uint64_t)n;
-				guess = std::max(guess, h_state[pr]);
+			if (piped) {
+				pend_slot = nwarp & 1;
+				HIPCHK(hipMemcpyAsync(ctx->tv_pin + 1 + (size_t)pend_slot * P, d_state, P * sizeof(int),
+				                      hipMemcpyDeviceToHost, st));
+				pend_s = s;
+				pend_w = w;
+				pend_n = (size_t)n;
+			} else {
+				HIPCHK(hipMemcpyAsync(h_state.data(), d_state, P * sizeof(int), hipMemcpyDeviceToHost, st));
+				HIPCHK(hipStreamSynchronize(st));
+				take_counts(h_state.data(), s, w, (size_t)n);
 			}
-			if (iters)
-				for (int pr = 0; pr < P; pr++) iters[((size_t)pr * nscales + s) * prm.warps + w] = h_state[pr];
-			if (prm.verbose)
-				for (int pr = 0; pr < P; pr++)
-					fprintf(stderr, "Pair %d scale %d warping %d: iterations %d\n", pr, s, w, h_state[pr]);
 		}
 		// the flow of every pair into the pyramid level
 		if (prm.warps > 0)
@@ -974,6 +1012,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 	HIPCHK(hipMemcpyAsync(u1out, U1 + pl.off[0], (size_t)P * size0 * sizeof(float), hipMemcpyDeviceToDevice, st));
 	HIPCHK(hipMemcpyAsync(u2out, U2 + pl.off[0], (size_t)P * size0 * sizeof(float), hipMemcpyDeviceToDevice, st));
 	HIPCHK(hipStreamSynchronize(st));
+	take_pending();
 }
 
 bm4dof_ctx *g_tv_ctx = nullptr;
