@@ -536,30 +536,40 @@ void tv_iter_kernel(const float *__restrict__ I1wx, const float *__restrict__ I1
 }
 
 // The stopping test of a round: error of iteration j = (sum over the
-// tiles' partials, in tile order) / size, then tv_round_decide.  One block
-// per pair.
-__global__ void tv_round_kernel(const double *__restrict__ part, int K, int ntiles, int size, float eps2,
-                                int *__restrict__ ctl, int *__restrict__ cur, int *__restrict__ iters,
-                                int *__restrict__ mode, int *__restrict__ state, int *__restrict__ n_active)
+// tiles' partials) / size, then tv_round_decide.  One wave per pair: 256
+// strided sums (sum v: tiles v, v + 256, ...) held four to a lane, then a
+// tree over them (sum v += sum v + s for s = 128, 64, ..., 1).
+__global__ __launch_bounds__(64)
+void tv_round_kernel(const double *__restrict__ part, int K, int ntiles, int size, float eps2,
+                     int *__restrict__ ctl, int *__restrict__ cur, int *__restrict__ iters,
+                     int *__restrict__ mode, int *__restrict__ state, int *__restrict__ n_active)
 {
-	const int pr = blockIdx.x;
+	const int pr = blockIdx.x, lane = threadIdx.x;
 	const int kk = ctl[pr];
 	if (kk <= 0) return;
-	__shared__ double s[TV_THREADS];
 	__shared__ float errs[TV_K];
-	for (int j = 0; j < kk; j++) {
-		double e = 0.0;
-		for (int t = threadIdx.x; t < ntiles; t += TV_THREADS) e += part[((size_t)pr * K + j) * ntiles + t];
-		s[threadIdx.x] = e;
-		__syncthreads();
-		for (int st = TV_THREADS / 2; st > 0; st >>= 1) {
-			if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-			__syncthreads();
+#pragma unroll
+	for (int j = 0; j < TV_K; j++) {
+		if (j >= kk) continue;
+		const double *pj = part + ((size_t)pr * K + j) * ntiles;
+		double v[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			double e = 0.0;
+			for (int t = lane + 64 * i; t < ntiles; t += 256) e += pj[t];
+			v[i] = e;
 		}
-		if (threadIdx.x == 0) errs[j] = (float)s[0] / (float)size;
-		__syncthreads();
+		v[0] = v[0] + v[2];   // s = 128
+		v[1] = v[1] + v[3];
+		double x = v[0] + v[1];   // s = 64
+#pragma unroll
+		for (int st = 32; st > 0; st >>= 1) {
+			const double o = __shfl_down(x, st, 64);
+			if (lane < st) x = x + o;
+		}
+		if (lane == 0) errs[j] = (float)x / (float)size;
 	}
-	if (threadIdx.x == 0) tv_round_decide(pr, kk, errs, eps2, K, ctl, cur, iters, mode, state, n_active);
+	if (lane == 0) tv_round_decide(pr, kk, errs, eps2, K, ctl, cur, iters, mode, state, n_active);
 }
 
 // Per pair: run state of the solve of one warp.
@@ -938,7 +948,7 @@ void tv_flow_pairs(bm4dof_ctx *ctx, const bm4dof_tvl1_params &prm, const float *
 				hipLaunchKernelGGL((tv_iter_kernel<TV_TX, TV_TY, TV_K, TV_NT>), dim3(ntiles, P), dim3(TV_NT), 0, st,
 				                   I1wx, I1wy, grad, rho_c, SA, SB, nx, ny, n, tiles_x, l_t, prm.theta, taut, d_ctl,
 				                   d_cur, d_part);
-			hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(TV_THREADS), 0, st, d_part, TV_K, ntiles, n, eps2,
+			hipLaunchKernelGGL(tv_round_kernel, dim3(P), dim3(64), 0, st, d_part, TV_K, ntiles, n, eps2,
 			                   d_ctl, d_cur, d_iters, d_mode, d_state, d_nact);
 		};
 		hipLaunchKernelGGL(tv_cgrad_kernel, dim3(g, P), dim3(TV_THREADS), 0, st, I1, nx, ny, n, I1x, I1y);
