@@ -3363,11 +3363,11 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	__shared__ __align__(16) float4 stage4[NBR * QR + ZQ];
 	const float *stage = reinterpret_cast<const float *>(stage4);
 	__shared__ float swin[SWN];
-	// per row of a chunk (two chunks in flight): x = stage index and y =
-	// window index of the block pixel (0, 0) at the bin's first frame (add
-	// f SL + the lane's slice index), z = weight bits; rowm = (row index e,
-	// bit dt set for the slices at the bin's frames)
-	__shared__ int4 rrec[2][NBR + 1];
+	// per row j of a chunk (two chunks in flight): x = window index of the
+	// block pixel (0, 0) at the bin's first frame (add f SL + the lane's
+	// slice index; the stage index is j CC P more), y = weight bits; rowm =
+	// (row index e, bit dt set for the slices at the bin's frames)
+	__shared__ int2 rrec[2][NBR + 1];
 	__shared__ uint2 rowm[2][NBR];
 	const int lane = threadIdx.x;
 	// bins are dealt to the 8 XCDs in contiguous eighths (x fastest)
@@ -3416,7 +3416,7 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	}
 	for (int i = lane; i < SWN; i += 64) swin[i] = i < P ? win[i] : 0.f;
 	for (int i = lane; i < ZQ; i += 64) stage4[NBR * QR + i] = make_float4(0.f, 0.f, 0.f, 0.f);
-	if (lane < 2) rrec[lane][NBR] = make_int4(NBR * QR * 4, 0, 0, 0);
+	if (lane < 2) rrec[lane][NBR] = make_int2(0, 0);   // stage index NBR CC P = the zero area
 	const int pix_l = ly * sPx + lx;
 	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
 	const float4 *est4 = reinterpret_cast<const float4 *>(est);
@@ -3460,7 +3460,7 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 			const int dlo = max(0, t0 - qt), dhi = min(sPt - 1, t0 + TB - 1 - qt);
 			const uint32_t dm = (msp != 0 && dhi >= dlo) ? ((2u << dhi) - (1u << dlo)) : 0u;
 			const int wb = (t0 - qt) * SL - (dy0 * sPx + dx0);
-			rrec[c & 1][lane] = make_int4(lane * CC * P + wb, wb, (int)recN.z, 0);
+			rrec[c & 1][lane] = make_int2(wb, (int)recN.z);
 			rowm[c & 1][lane] = make_uint2(eN, dm);
 		} else if (lane < NBR) {
 			rowm[c & 1][lane] = make_uint2(0u, 0u);   // past the list's end: nothing staged
@@ -3530,19 +3530,20 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 					jj[u] = mine ? __ffs(mine) - 1 : NBR;
 					mine &= mine - 1;
 				}
-				int4 rr[RPR];
+				int2 rr[RPR];
 				float vv[RPR], xx[RPR][CC];
 #pragma unroll
 				for (int u = 0; u < RPR; u++) rr[u] = rrec[c & 1][jj[u]];
 #pragma unroll
 				for (int u = 0; u < RPR; u++) {
-					vv[u] = swin[rr[u].y + plf[f]];
+					vv[u] = swin[rr[u].x + plf[f]];
+					const int sx = jj[u] * (CC * P) + rr[u].x + plf[f];
 #pragma unroll
-					for (int cc = 0; cc < CC; cc++) xx[u][cc] = stage[rr[u].x + cc * P + plf[f]];
+					for (int cc = 0; cc < CC; cc++) xx[u][cc] = stage[sx + cc * P];
 				}
 #pragma unroll
 				for (int u = 0; u < RPR; u++) {
-					aw[f] = aw[f] + __int_as_float(rr[u].z) * vv[u];
+					aw[f] = aw[f] + __int_as_float(rr[u].y) * vv[u];
 #pragma unroll
 					for (int cc = 0; cc < CC; cc++) an[f][cc] = an[f][cc] + xx[u][cc];
 				}
