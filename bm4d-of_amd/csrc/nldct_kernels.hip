@@ -3426,12 +3426,15 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	// chunk c+1 are in flight while chunk c is folded.
 	uint32_t eN = 0;
 	uint4 recN = make_uint4(0, 0, 0, 0);
+	// The loads are unconditional (index clamped to the bin's last entry;
+	// meta_commit uses only the lanes of the chunk's rows), so their results
+	// land in the registers that carry them to meta_commit and nothing waits
+	// for them before it.
 	auto meta_load = [&](int c) {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
-		if (c < nch && lane < NBR && kc + lane < k1) {
-			eN = blist[kc + lane];
-			recN = erec[eN];
-		}
+		const uint32_t k = min(kc + (uint32_t)lane, k1 - 1u);
+		eN = blist[k];
+		recN = erec[eN];
 	};
 	// the chunk's records into rrec / rowm [c & 1]; returns the lane's
 	// coverage mask (lanes = rows) and the rows present at each frame
@@ -3511,6 +3514,15 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	int plf[TB];
 #pragma unroll
 	for (int f = 0; f < TB; f++) plf[f] = f * SL + pix_l;
+	// the sums' first values are complete here (an empty asm that reads and
+	// writes them), so the fold below waits for no global load of its own:
+	// the slices of the next chunk stay in flight while a chunk is folded
+#pragma unroll
+	for (int f = 0; f < TB; f++) {
+		asm volatile("" : "+v"(aw[f]));
+#pragma unroll
+		for (int cc = 0; cc < CC; cc++) asm volatile("" : "+v"(an[f][cc]));
+	}
 	for (int c = 0; c < nch; c++) {
 		uint64_t mN = 0;
 		if (c + 1 < nch) mN = meta_commit(c + 1, vmN);
