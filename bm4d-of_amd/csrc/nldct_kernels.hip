@@ -3340,8 +3340,18 @@ void bin_bounds_kernel(const uint32_t *__restrict__ keys, const uint32_t *__rest
 
 // CC channels; NBR (<= 32) rows per chunk; patches of 4 PQ samples, frame
 // slices of 4 SQ samples (PQ <= 16).
+//
+// LDS, in floats: a zero area Z, then NBR + 1 slots of CC P + 4 floats.
+// Slot j >= 0 holds chunk row j: its samples (channel-major) and, after
+// them, its record (byte offset wb4 of the window / sample index of the
+// block pixel (0, 0) at the bin's first frame, weight).  Slot -1 holds the
+// window (its first P floats) and the record (-4 |Z|, +0): a round's
+// missing rows have index -1 (the lowest set bit of an empty mask), so
+// their samples and window values are read from Z, and every round adds
+// RPR terms without a select (adding +0 leaves a sum that is not -0
+// unchanged, and these sums start at +0 and never become -0).
 template <int CC, int NBR, int PQ, int SQ>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CC == 1 ? 4 : 3)))
 void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__restrict__ blist,
                          const uint4 *__restrict__ erec, const float *__restrict__ est,
                          const float *__restrict__ win, StepConst sc, int nbx, int nby, int nbins,
@@ -3349,25 +3359,20 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 {
 	static_assert(NBR <= 32, "row masks are 32 bits");
 	constexpr int TB = AGG_TB;
-	constexpr int P = 4 * PQ, SL = 4 * SQ, QR = CC * PQ;   // QR: float4 of one row
+	constexpr int P = 4 * PQ, SL = 4 * SQ, QR = CC * PQ;   // QR: float4 of one row's samples
+	constexpr int QS = QR + 1;                             // float4 per slot (samples, record)
 	constexpr int SPL = (NBR * QR + 63) / 64;              // float4 staged per lane per chunk
 	constexpr int RPR = CC == 1 ? 4 : 2;                   // rows folded per round
 	static_assert(SPL * 64 == NBR * QR, "the staging fills the chunk's slots exactly");
-	// a round's missing rows read the record NBR: weight +0 and samples +0
-	// from a zeroed area after the chunk's slots, so that every round adds
-	// RPR terms without a select (adding +0 leaves a sum that is not -0
-	// unchanged, and these sums start at +0 and never become -0)
 	constexpr int ZR = (TB - 1) * SL + 64;                 // span of the lane offsets f SL + pix_l
-	constexpr int ZQ = ((CC - 1) * P + ZR + 3) / 4;
-	constexpr int SWN = P > ZR ? P : ZR;
-	__shared__ __align__(16) float4 stage4[NBR * QR + ZQ];
-	const float *stage = reinterpret_cast<const float *>(stage4);
-	__shared__ float swin[SWN];
-	// per row j of a chunk (two chunks in flight): x = window index of the
-	// block pixel (0, 0) at the bin's first frame (add f SL + the lane's
-	// slice index; the stage index is j CC P more), y = weight bits; rowm =
-	// (row index e, bit dt set for the slices at the bin's frames)
-	__shared__ int2 rrec[2][NBR + 1];
+	constexpr int ZQ = ((CC - 1) * P + ZR + 3) / 4;        // float4 of Z
+	__shared__ __align__(16) float4 lds4[ZQ + (NBR + 1) * QS];
+	float *const slotm1 = reinterpret_cast<float *>(lds4 + ZQ);   // slot -1
+	float *const slot0 = slotm1 + 4 * QS;                          // slot 0
+	constexpr int SB = 16 * QS;                                    // bytes per slot
+	constexpr int RECB = 4 * CC * P;                               // record offset in a slot
+	// per row of a chunk (two chunks in flight): (row index e, bit dt set
+	// for the slices at the bin's frames)
 	__shared__ uint2 rowm[2][NBR];
 	const int lane = threadIdx.x;
 	// bins are dealt to the 8 XCDs in contiguous eighths (x fastest)
@@ -3414,30 +3419,32 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 				an[f][c] = num[(size_t)(t0 + f) * sc.WHC + (size_t)c * sc.WH + (size_t)y * sc.W + x];
 		}
 	}
-	for (int i = lane; i < SWN; i += 64) swin[i] = i < P ? win[i] : 0.f;
-	for (int i = lane; i < ZQ; i += 64) stage4[NBR * QR + i] = make_float4(0.f, 0.f, 0.f, 0.f);
-	if (lane < 2) rrec[lane][NBR] = make_int2(0, 0);   // stage index NBR CC P = the zero area
+	for (int i = lane; i < ZQ; i += 64) lds4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	for (int i = lane; i < P; i += 64) slotm1[i] = win[i];
+	if (lane == 0) *reinterpret_cast<int2 *>(slotm1 + CC * P) = make_int2(-16 * ZQ, 0);
 	const int pix_l = ly * sPx + lx;
 	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
 	const float4 *est4 = reinterpret_cast<const float4 *>(est);
 	const int nch = (int)((k1 - k0 + NBR - 1) / NBR);
 
 	// Software pipeline: the row records of chunk c+2 and the slices of
-	// chunk c+1 are in flight while chunk c is folded.
+	// chunk c+1 are in flight while chunk c is folded.  The loads are
+	// unconditional (index clamped to the bin's last entry; meta_commit uses
+	// only the lanes of the chunk's rows), so their results land in the
+	// registers that carry them to meta_commit and nothing waits for them
+	// before it.
 	uint32_t eN = 0;
 	uint4 recN = make_uint4(0, 0, 0, 0);
-	// The loads are unconditional (index clamped to the bin's last entry;
-	// meta_commit uses only the lanes of the chunk's rows), so their results
-	// land in the registers that carry them to meta_commit and nothing waits
-	// for them before it.
 	auto meta_load = [&](int c) {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
 		const uint32_t k = min(kc + (uint32_t)lane, k1 - 1u);
 		eN = blist[k];
 		recN = erec[eN];
 	};
-	// the chunk's records into rrec / rowm [c & 1]; returns the lane's
-	// coverage mask (lanes = rows) and the rows present at each frame
+	// the chunk's row masks into rowm[c & 1] and its records (lane = row)
+	// into rw, for the slot write with the chunk's samples; returns the
+	// lane's coverage mask (lanes = rows) and the rows present at each frame
+	int2 rw = make_int2(0, 0);
 	auto meta_commit = [&](int c, uint32_t (&vm)[TB]) -> uint64_t {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
 		const bool rv = lane < NBR && kc + lane < k1;
@@ -3463,7 +3470,7 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 			const int dlo = max(0, t0 - qt), dhi = min(sPt - 1, t0 + TB - 1 - qt);
 			const uint32_t dm = (msp != 0 && dhi >= dlo) ? ((2u << dhi) - (1u << dlo)) : 0u;
 			const int wb = (t0 - qt) * SL - (dy0 * sPx + dx0);
-			rrec[c & 1][lane] = make_int2(wb, (int)recN.z);
+			rw = make_int2(4 * wb, (int)recN.z);
 			rowm[c & 1][lane] = make_uint2(eN, dm);
 		} else if (lane < NBR) {
 			rowm[c & 1][lane] = make_uint2(0u, 0u);   // past the list's end: nothing staged
@@ -3501,7 +3508,12 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 #define AGG_BIN_STAGE_WRITE()                                                                       \
 	{                                                                                               \
 		_Pragma("unroll") for (int m = 0; m < SPL; m++)                                           \
-			if ((sneed >> m) & 1u) stage4[lane + 64 * m] = sv[m];                                  \
+			if ((sneed >> m) & 1u) {                                                                \
+				const int i = lane + 64 * m;                                                        \
+				const int j = i / QR;                                                               \
+				lds4[ZQ + QS + j * QS + (i - j * QR)] = sv[m];                                      \
+			}                                                                                       \
+		if (lane < NBR) *reinterpret_cast<int2 *>(slot0 + 4 * QS * lane + CC * P) = rw;            \
 	}
 	uint32_t vmC[TB], vmN[TB];
 	meta_load(0);
@@ -3510,10 +3522,10 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	__syncthreads();
 	AGG_BIN_STAGE_ISSUE(0);
 	AGG_BIN_STAGE_WRITE();
-	// the lane's slice index at each frame of the bin
-	int plf[TB];
+	// byte offset of the lane's slice at each frame of the bin
+	int plf4[TB];
 #pragma unroll
-	for (int f = 0; f < TB; f++) plf[f] = f * SL + pix_l;
+	for (int f = 0; f < TB; f++) plf4[f] = 4 * (f * SL + pix_l);
 	// the sums' first values are complete here (an empty asm that reads and
 	// writes them), so the fold below waits for no global load of its own:
 	// the slices of the next chunk stay in flight while a chunk is folded
@@ -3536,22 +3548,27 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 		for (int f = 0; f < TB; f++) {
 			uint32_t mine = mine_sp & vmC[f];
 			while (mine) {
-				int jj[RPR];
+				int so[RPR];   // byte offset of the row's slot from slot 0 (-SB: no row)
 #pragma unroll
 				for (int u = 0; u < RPR; u++) {
-					jj[u] = mine ? __ffs(mine) - 1 : NBR;
+					// the lowest set bit's index, -1 for an empty mask
+					int jj;
+					asm("v_ffbl_b32 %0, %1" : "=v"(jj) : "v"(mine));
 					mine &= mine - 1;
+					so[u] = __mul24(jj, SB);
 				}
+				const char *s0 = reinterpret_cast<const char *>(slot0);
+				const char *wn = reinterpret_cast<const char *>(slotm1) + plf4[f];
 				int2 rr[RPR];
 				float vv[RPR], xx[RPR][CC];
 #pragma unroll
-				for (int u = 0; u < RPR; u++) rr[u] = rrec[c & 1][jj[u]];
+				for (int u = 0; u < RPR; u++) rr[u] = *reinterpret_cast<const int2 *>(s0 + so[u] + RECB);
 #pragma unroll
 				for (int u = 0; u < RPR; u++) {
-					vv[u] = swin[rr[u].x + plf[f]];
-					const int sx = jj[u] * (CC * P) + rr[u].x + plf[f];
+					vv[u] = *reinterpret_cast<const float *>(wn + rr[u].x);
+					const char *sp = s0 + (so[u] + rr[u].x + plf4[f]);
 #pragma unroll
-					for (int cc = 0; cc < CC; cc++) xx[u][cc] = stage[sx + cc * P];
+					for (int cc = 0; cc < CC; cc++) xx[u][cc] = *reinterpret_cast<const float *>(sp + 4 * cc * P);
 				}
 #pragma unroll
 				for (int u = 0; u < RPR; u++) {
