@@ -3429,15 +3429,16 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 
 	// Software pipeline: the row records of chunk c+2 and the slices of
 	// chunk c+1 are in flight while chunk c is folded.  The loads are
-	// unconditional (index clamped to the bin's last entry; meta_commit uses
-	// only the lanes of the chunk's rows), so their results land in the
-	// registers that carry them to meta_commit and nothing waits for them
-	// before it.
+	// unconditional (index clamped to the bin's last entry, and the lanes
+	// past the chunk's NBR rows all at its first entry, one line;
+	// meta_commit uses only the lanes of the chunk's rows), so their results
+	// land in the registers that carry them to meta_commit and nothing waits
+	// for them before it.
 	uint32_t eN = 0;
 	uint4 recN = make_uint4(0, 0, 0, 0);
 	auto meta_load = [&](int c) {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
-		const uint32_t k = min(kc + (uint32_t)lane, k1 - 1u);
+		const uint32_t k = min(kc + (uint32_t)(lane < NBR ? lane : 0), k1 - 1u);
 		eN = blist[k];
 		recN = erec[eN];
 	};
