@@ -3427,21 +3427,25 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	const float4 *est4 = reinterpret_cast<const float4 *>(est);
 	const int nch = (int)((k1 - k0 + NBR - 1) / NBR);
 
-	// Software pipeline: the row records of chunk c+2 and the slices of
-	// chunk c+1 are in flight while chunk c is folded.  The loads are
-	// unconditional (index clamped to the bin's last entry, and the lanes
-	// past the chunk's NBR rows all at its first entry, one line;
-	// meta_commit uses only the lanes of the chunk's rows), so their results
-	// land in the registers that carry them to meta_commit and nothing waits
-	// for them before it.
-	uint32_t eN = 0;
-	uint4 recN = make_uint4(0, 0, 0, 0);
-	auto meta_load = [&](int c) {
+	// Software pipeline: while chunk c is folded, the slices of chunk c+1,
+	// the row records of chunk c+2 and the list entries of chunk c+3 are in
+	// flight (eA, rA: chunk c+1's entries and records, eB, rB: chunk c+2's,
+	// eC: chunk c+3's entries), so no record load waits for its entry load.
+	// The loads are unconditional (index clamped to the bin's last entry,
+	// and the lanes past the chunk's NBR rows all at its first entry, one
+	// line; meta_commit uses only the lanes of the chunk's rows), so their
+	// results land in the registers that carry them and nothing waits for
+	// them before they are used.
+	auto ent_load = [&](int c) -> uint32_t {
 		const uint32_t kc = k0 + (uint32_t)c * NBR;
-		const uint32_t k = min(kc + (uint32_t)(lane < NBR ? lane : 0), k1 - 1u);
-		eN = blist[k];
-		recN = erec[eN];
+		return blist[min(kc + (uint32_t)(lane < NBR ? lane : 0), k1 - 1u)];
 	};
+	uint32_t eA = ent_load(0);
+	uint4 rA = erec[eA];
+	uint32_t eB = 0, eC = 0;
+	uint4 rB = make_uint4(0, 0, 0, 0);
+	uint32_t &eN = eA;
+	uint4 &recN = rA;
 	// the chunk's row masks into rowm[c & 1] and its records (lane = row)
 	// into rw, for the slot write with the chunk's samples; returns the
 	// lane's coverage mask (lanes = rows) and the rows present at each frame
@@ -3517,9 +3521,10 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 		if (lane < NBR) *reinterpret_cast<int2 *>(slot0 + 4 * QS * lane + CC * P) = rw;            \
 	}
 	uint32_t vmC[TB], vmN[TB];
-	meta_load(0);
 	uint64_t mC = meta_commit(0, vmC);
-	meta_load(1);
+	eA = ent_load(1);
+	rA = erec[eA];
+	eB = ent_load(2);
 	__syncthreads();
 	AGG_BIN_STAGE_ISSUE(0);
 	AGG_BIN_STAGE_WRITE();
@@ -3539,7 +3544,8 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	for (int c = 0; c < nch; c++) {
 		uint64_t mN = 0;
 		if (c + 1 < nch) mN = meta_commit(c + 1, vmN);
-		meta_load(c + 2);
+		rB = erec[eB];
+		eC = ent_load(c + 3);
 		__syncthreads();
 		if (c + 1 < nch) AGG_BIN_STAGE_ISSUE(c + 1);
 		// each lane folds the rows that cover its pixel, frame by frame, in
@@ -3581,6 +3587,9 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 		}
 		__syncthreads();
 		if (c + 1 < nch) AGG_BIN_STAGE_WRITE();
+		eA = eB;
+		rA = rB;
+		eB = eC;
 		mC = mN;
 #pragma unroll
 		for (int f = 0; f < TB; f++) vmC[f] = vmN[f];
