@@ -94,7 +94,8 @@ __global__ void first_index_kernel(const uint32_t *, int, int, uint32_t *);
 template <int CC> __global__ void agg_exact_kernel(AGG_ARGS);
 #define AGG2_ARGS const uint32_t *, const uint32_t *, const uint4 *, const float *, const float *, \
 	StepConst, int, int, float *, float *, uint32_t *
-__global__ void pack_rows_kernel(const uint32_t *, const uint32_t *, const float *, int64_t, uint4 *);
+__global__ void pack_rows_kernel(const uint32_t *, const uint32_t *, const float *, int64_t, uint32_t, uint32_t,
+                                 uint4 *);
 template <int CC, int CAP, int WP> __global__ void agg_chunk_kernel(AGG2_ARGS);
 __global__ void bin_hist1_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, int, int, int, uint32_t *);
 __global__ void bin_scatter1_kernel(const uint32_t *, const uint32_t *, int64_t, StepConst, int, int, int,
@@ -666,12 +667,14 @@ void dscan(bm4dof_ctx *ctx, int op, const uint32_t *in, int64_t n_in, int64_t n,
 
 // Binned aggregation: shapes it takes (64-sample patches 4x4x4 or 8x8x1,
 // one block overlapping at most 2 x 2 tile interiors) and the list sizes
-// (at most 8 bins per row, pair counts in 32 bits).
+// (at most 8 bins per row, pair counts in 32 bits; rows and frames below
+// 2^16, as the row records hold them in 16 bits each).
 bool agg_binned_ok(const bm4dof_ctx *ctx, const StepConst &sc, int64_t Mub)
 {
 	return ctx->agg_binned && ctx->agg_chunked && sc.wTmp >= 8 && sc.hTmp >= 8 &&
 	       ((sc.sPx == 4 && sc.sPt == 4) || (sc.sPx == 8 && sc.sPt == 1)) &&
-	       (sc.C == 1 || sc.C == 3 || sc.C == 4) && 8 * Mub < ((int64_t)1 << 32) - RS_TILE;
+	       (sc.C == 1 || sc.C == 3 || sc.C == 4) && sc.H < 65536 && sc.F < 65536 &&
+	       8 * Mub < ((int64_t)1 << 32) - RS_TILE;
 }
 
 struct BinGrid { int nbx, nby, nbins; };
@@ -766,7 +769,7 @@ void aggregate_chunk(bm4dof_ctx *ctx, const StepConst &sc, const std::vector<uin
 	if (chunked) {
 		uint4 *d_erec = ctx->erec.as<uint4>((size_t)Mub);
 		hipLaunchKernelGGL(pack_rows_kernel, dim3(nblk(Mub, 256)), dim3(256), 0, ctx->stream, epos, etile, ew, Mub,
-		                   d_erec);
+		                   (uint32_t)sc.W, (uint32_t)sc.WH, d_erec);
 		HIPCHK(hipGetLastError());
 		if (t0c == 0 && t1c == sc.F && agg_binned_ok(ctx, sc, Mub)) {
 			// the bin lists were built on the side stream (agg_bin_lists)

@@ -2435,13 +2435,20 @@ __global__ void emit_entries_kernel(const uint32_t *__restrict__ idx, const uint
 	}
 }
 
-// erec[e] = (position, tile, weight bits, 0): the row metadata the chunked
-// aggregation reads, one 16-byte record per row.
+// erec[e] = (position, tile, weight bits, row | frame << 16 of the
+// position): the row metadata the chunked and binned aggregations read,
+// one 16-byte record per row (the binned form takes row and frame from it
+// instead of dividing the position once per bin).
 __global__ void pack_rows_kernel(const uint32_t *__restrict__ epos, const uint32_t *__restrict__ etile,
-                                 const float *__restrict__ entry_w, int64_t M, uint4 *__restrict__ erec)
+                                 const float *__restrict__ entry_w, int64_t M, uint32_t W, uint32_t WH,
+                                 uint4 *__restrict__ erec)
 {
 	const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-	if (e < M) erec[e] = make_uint4(epos[e], etile[e], __float_as_uint(entry_w[e]), 0u);
+	if (e < M) {
+		const uint32_t q = epos[e];
+		const uint32_t qt = q / WH, qy = (q - qt * WH) / W;
+		erec[e] = make_uint4(q, etile[e], __float_as_uint(entry_w[e]), qy | (qt << 16));
+	}
 }
 
 #define AGG_CAPB 1024   // rows of one band held in LDS
@@ -3423,7 +3430,6 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 	for (int i = lane; i < P; i += 64) slotm1[i] = win[i];
 	if (lane == 0) *reinterpret_cast<int2 *>(slotm1 + CC * P) = make_int2(-16 * ZQ, 0);
 	const int pix_l = ly * sPx + lx;
-	const float rcpW = 1.f / (float)sc.W, rcpWH = 1.f / (float)sc.WH;
 	const float4 *est4 = reinterpret_cast<const float4 *>(est);
 	const int nch = (int)((k1 - k0 + NBR - 1) / NBR);
 
@@ -3457,10 +3463,9 @@ void agg_bin_fold_kernel(const uint32_t *__restrict__ boff, const uint32_t *__re
 		int qt = 0;
 		if (rv) {
 			const uint32_t q = recN.x;
-			qt = (int)udiv_rcp(q, (uint32_t)sc.WH, rcpWH);
-			const uint32_t rr = q - (uint32_t)qt * (uint32_t)sc.WH;
-			const int qy = (int)udiv_rcp(rr, (uint32_t)sc.W, rcpW);
-			const int qx = (int)(rr - (uint32_t)qy * (uint32_t)sc.W);
+			qt = (int)(recN.w >> 16);
+			const int qy = (int)(recN.w & 0xffffu);
+			const int qx = (int)(q - (uint32_t)qt * (uint32_t)sc.WH - (uint32_t)qy * (uint32_t)sc.W);
 			const int dx0 = qx - x0, dy0 = qy - y0;
 			const uint32_t cols = span8(dx0, dx0 + sPx), rows = span8(dy0, dy0 + sPx);
 			uint64_t rsel = 0;
