@@ -1791,11 +1791,29 @@ template __global__ void coef_kernel<8>(const float *, int, int, int, const floa
 
 // Coefficient rows of the N patches at anchors gidx[0..N) from a
 // coef_kernel table: lane k gets z[n] = coef[gidx[n]*64 + k] (rows n >= N zero).
+// VIDX: the N row indices come from one vector load (lane n < N holds
+// gidx[n]) and are read back lane by lane, so that the row loads do not
+// wait for one scalar load each; rows n >= N load row gidx of lane n = 0
+// (anchor 0) and are set to 0.  (The step-2 filter keeps the scalar loads:
+// there the 32 indices held in scalar registers spill and the filter is
+// slower.)
+template <bool VIDX = false>
 __device__ __forceinline__ void fast_rows(const float *__restrict__ coef, const uint32_t *__restrict__ gidx, int N,
                                           float (&z)[FAST_NMAX])
 {
+	static_assert(FAST_NMAX <= 64, "one index per lane");
+	if (VIDX) {
+		const uint32_t my_idx = (int)threadIdx.x < N ? gidx[threadIdx.x] : 0u;
 #pragma unroll
-	for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? coef[(size_t)gidx[n] * FAST_P + threadIdx.x] : 0.f;
+		for (int n = 0; n < FAST_NMAX; n++) {
+			const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)my_idx, n);
+			const float v = coef[(size_t)r * FAST_P + threadIdx.x];
+			z[n] = n < N ? v : 0.f;
+		}
+	} else {
+#pragma unroll
+		for (int n = 0; n < FAST_NMAX; n++) z[n] = n < N ? coef[(size_t)gidx[n] * FAST_P + threadIdx.x] : 0.f;
+	}
 }
 
 // Whether all N coefficient rows are bitwise equal (every lane must call
@@ -1875,7 +1893,7 @@ void filter1_fast_kernel(const float *__restrict__ img, const uint32_t *__restri
 	const uint32_t row0 = base[gl];
 	const uint32_t *gidx = idx + (size_t)g * sc.npmax;
 	float z[FAST_NMAX];
-	if (coef) fast_rows(coef + c * cstride * FAST_P, gidx, N, z);
+	if (coef) fast_rows<true>(coef + c * cstride * FAST_P, gidx, N, z);
 	if (c == 0) {
 		if (lane < N) etile[row0 + lane] = gtile[g];
 		if ((!coef || fast_rows_equal(z, N)) &&
